@@ -1,0 +1,381 @@
+// kernels_vec.hip -- CG vector kernels with fused, deterministic reductions.
+//
+// Reference counterparts: alpha_kernel / beta_kernel
+// (conjugate_gradient_kernels.hpp:5-38) and cublas_dot
+// (cublas_wrapper.hpp:62-84, called at conjugate_gradient_gpu.hpp:42,60,75,85).
+// Differences by design:
+//   * the dots are fused into the kernels that already stream the vectors
+//     (r.r into the x/r update, b.b and r.r into the initial residual,
+//     p.Ap into the operator apply), accumulated in double;
+//   * every reduction is two-stage with a fixed order (per-thread sums in
+//     index order, a wave butterfly, waves summed in order, block sums in
+//     block order), so results are bitwise reproducible run to run;
+//   * the CG scalars (alpha, beta, r.r, the stop flag) never leave the
+//     device: each consumer kernel re-reduces the previous kernel's block
+//     sums itself, and block 0 records them in a cg_state.
+#include "sipg_internal.hpp"
+
+#include <cmath>
+
+namespace sipg_internal {
+
+namespace {
+
+constexpr int NT = kVecThreads;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over the block; the result is returned to every thread.
+__device__ __forceinline__ double block_sum_all(double v) {
+    __shared__ double red[NT / 64];
+    __shared__ double total;
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();   // protect red/total from a previous call
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i) s += red[i];
+        total = s;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Fixed-order sum of np block partials, identical in every block.
+__device__ __forceinline__ double reduce_partials_dev(const double *__restrict__ part, int np) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += NT) s += part[i];
+    return block_sum_all(s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void dot_kernel(const T *__restrict__ a, const T *__restrict__ b,
+                                                 int64_t n, double *__restrict__ part) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        s += (double)a[i] * (double)b[i];
+    s = block_sum_all(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(NT) void reduce_kernel(const double *__restrict__ part, int np,
+                                                    double *__restrict__ out) {
+    const double s = reduce_partials_dev(part, np);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void axpy_kernel(T alpha, const T *__restrict__ x, T *__restrict__ y,
+                                                  int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        y[i] = y[i] + alpha * x[i];
+}
+
+// beta_kernel (conjugate_gradient_kernels.hpp:26-38): p = r + beta p
+template <typename T>
+__global__ __launch_bounds__(NT) void xpby_kernel(const T *__restrict__ r, T beta, T *__restrict__ p,
+                                                  int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        p[i] = r[i] + beta * p[i];
+}
+
+// alpha_kernel (conjugate_gradient_kernels.hpp:5-23)
+template <typename T>
+__global__ __launch_bounds__(NT) void alpha_kernel(T alpha, T *__restrict__ x, const T *__restrict__ p,
+                                                   T *__restrict__ r, const T *__restrict__ Ap,
+                                                   int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        x[i] = x[i] + alpha * p[i];
+        r[i] = r[i] - alpha * Ap[i];
+    }
+}
+
+// r = b - r (r holds A x on entry), p = r, block sums of r.r and b.b
+// (conjugate_gradient_gpu.hpp:42-60).
+template <typename T>
+__global__ __launch_bounds__(NT) void cg_init_kernel(const T *__restrict__ b, T *__restrict__ r,
+                                                     T *__restrict__ p, int64_t n,
+                                                     double *__restrict__ prr,
+                                                     double *__restrict__ pbb) {
+    double srr = 0.0, sbb = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        const T bi = b[i];
+        const T ri = bi + T(-1) * r[i];
+        r[i] = ri;
+        p[i] = ri;
+        srr += (double)ri * (double)ri;
+        sbb += (double)bi * (double)bi;
+    }
+    srr = block_sum_all(srr);
+    sbb = block_sum_all(sbb);
+    if (threadIdx.x == 0) {
+        prr[blockIdx.x] = srr;
+        pbb[blockIdx.x] = sbb;
+    }
+}
+
+__global__ __launch_bounds__(NT) void cg_init_finalize_kernel(const double *__restrict__ prr,
+                                                              const double *__restrict__ pbb,
+                                                              int np, cg_state *st, double tol,
+                                                              int tol_mode, int maxit) {
+    const double rr = reduce_partials_dev(prr, np);
+    const double bb = reduce_partials_dev(pbb, np);
+    if (threadIdx.x == 0) {
+        const double stop = tol_mode == SIPG_TOL_REL ? tol * sqrt(bb) : tol;
+        st->s[0].rr = rr;
+        st->s[0].alpha = st->s[0].beta = st->s[0].pAp = 0.0;
+        st->s[1] = st->s[0];
+        st->rr0 = rr;
+        st->bb = bb;
+        st->stop = stop;
+        st->it = 0;
+        st->maxit = maxit;
+        int done = (sqrt(rr) > stop && maxit > 0) ? 0 : 1;   // while-condition of :69
+        if (!isfinite(rr) || !isfinite(bb)) done = 3;
+        st->done = done;
+    }
+}
+
+// x += alpha p; r -= alpha Ap with alpha = r.r / p.Ap from the apply's
+// block sums; writes block sums of the new r.r.
+template <typename T>
+__global__ __launch_bounds__(NT) void cg_update_xr_kernel(T *__restrict__ x, const T *__restrict__ p,
+                                                          T *__restrict__ r, const T *__restrict__ Ap,
+                                                          int64_t n, const double *__restrict__ ppAp,
+                                                          int np, double *__restrict__ prr,
+                                                          cg_state *st, int slot) {
+    if (st->done) return;
+    const double pAp = reduce_partials_dev(ppAp, np);
+    const double rr = st->s[slot].rr;
+    const double alpha_d = rr / pAp;
+    if (!(pAp > 0.0) || !isfinite(alpha_d)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->s[slot].pAp = pAp;
+            st->done = isfinite(pAp) ? 2 : 3;
+        }
+        return;
+    }
+    const T alpha = (T)alpha_d;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        x[i] = x[i] + alpha * p[i];
+        const T ri = r[i] - alpha * Ap[i];
+        r[i] = ri;
+        s += (double)ri * (double)ri;
+    }
+    s = block_sum_all(s);
+    if (threadIdx.x == 0) {
+        prr[blockIdx.x] = s;
+        if (blockIdx.x == 0) {
+            st->s[slot].pAp = pAp;
+            st->s[slot].alpha = alpha_d;
+        }
+    }
+}
+
+// p = r + beta p, beta = r.r(new) / r.r(old); block 0 advances the state.
+template <typename T>
+__global__ __launch_bounds__(NT) void cg_update_p_kernel(const T *__restrict__ r, T *__restrict__ p,
+                                                         int64_t n, const double *__restrict__ prr,
+                                                         int np, cg_state *st, int slot) {
+    if (st->done) return;
+    const double rr_new = reduce_partials_dev(prr, np);
+    const double rr = st->s[slot].rr;
+    const double beta_d = rr_new / rr;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->s[slot].beta = beta_d;
+        st->s[slot ^ 1].rr = rr_new;
+        const int it = st->it + 1;
+        st->it = it;
+        int done = 0;
+        if (!(sqrt(rr_new) > st->stop) || it >= st->maxit) done = 1;
+        if (!isfinite(rr_new)) done = 3;
+        st->done = done;
+    }
+    const T beta = (T)beta_d;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        p[i] = r[i] + beta * p[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void residual_kernel(const T *__restrict__ b, const T *__restrict__ Ax,
+                                                      T *__restrict__ r, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        r[i] = b[i] - Ax[i];
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(NT) void convert_kernel(const S *__restrict__ src, D *__restrict__ dst,
+                                                     int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        dst[i] = (D)src[i];
+}
+
+// Load vector of the Fourier problem with g = 0 (sipg_sem_2d.hpp:248-262):
+// b(i1,i2,e) = (w_i1 h/2)(w_i2 h/2) f(x_e + h(z_i1+1)/2, y_e + h(z_i2+1)/2),
+// x_e = col*h, y_e = 1 - (row+1)*h.
+template <typename T>
+__global__ __launch_bounds__(NT) void rhs_fourier_kernel(int p, int dim, const double *__restrict__ tab,
+                                                         int nmodes, const double *__restrict__ am,
+                                                         const int *__restrict__ mm,
+                                                         const int *__restrict__ nn, T *__restrict__ b) {
+    const int n1 = p + 1;
+    const int64_t noe = (int64_t)dim * dim;
+    const int64_t n = noe * n1 * n1;
+    const double *w = tab;
+    const double *z = tab + 3 * n1 + n1 * n1;
+    const double h = 1.0 / dim;
+    for (int64_t idx = (int64_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * NT) {
+        const int64_t e = idx % noe;
+        const int node = (int)(idx / noe);
+        const int i1 = node / n1, i2 = node % n1;
+        const int col = (int)(e % dim), row = (int)(e / dim);
+        const double xe = col * h, ye = 1.0 - (row + 1) * h;
+        const double X = xe + (z[i1] * .5 + .5) / dim;
+        const double Y = ye + (z[i2] * .5 + .5) / dim;
+        double f = 0.0;
+        for (int k = 0; k < nmodes; ++k) {
+            const double m = mm[k], q = nn[k];
+            f += am[k] * M_PI * M_PI * (m * m + q * q) * sin(m * M_PI * X) * sin(q * M_PI * Y);
+        }
+        b[idx] = (T)((w[i1] * .5 / dim) * (w[i2] * .5 / dim) * f);
+    }
+}
+
+}  // namespace
+
+int vec_num_blocks(int64_t n) {
+    int64_t nb = (n + NT * 4 - 1) / (NT * 4);
+    if (nb < 1) nb = 1;
+    if (nb > 1024) nb = 1024;
+    return (int)nb;
+}
+
+#define DISPATCH_T(dtype, KERNEL_CALL_F64, KERNEL_CALL_F32) \
+    do {                                                   \
+        if ((dtype) == SIPG_F64) { KERNEL_CALL_F64; }      \
+        else { KERNEL_CALL_F32; }                          \
+    } while (0)
+
+int launch_dot_partials(int dtype, const void *a, const void *b, int64_t n, double *partials,
+                        int nblk, hipStream_t s) {
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(dot_kernel<double>, dim3(nblk), dim3(NT), 0, s, (const double *)a, (const double *)b, n, partials),
+        hipLaunchKernelGGL(dot_kernel<float>, dim3(nblk), dim3(NT), 0, s, (const float *)a, (const float *)b, n, partials));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_reduce_partials(const double *partials, int np, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(NT), 0, s, partials, np, out);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_axpy(int dtype, double alpha, const void *x, void *y, int64_t n, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(axpy_kernel<double>, dim3(nb), dim3(NT), 0, s, alpha, (const double *)x, (double *)y, n),
+        hipLaunchKernelGGL(axpy_kernel<float>, dim3(nb), dim3(NT), 0, s, (float)alpha, (const float *)x, (float *)y, n));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_xpby(int dtype, const void *r, double beta, void *p, int64_t n, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(xpby_kernel<double>, dim3(nb), dim3(NT), 0, s, (const double *)r, beta, (double *)p, n),
+        hipLaunchKernelGGL(xpby_kernel<float>, dim3(nb), dim3(NT), 0, s, (const float *)r, (float)beta, (float *)p, n));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_alpha_update(int dtype, double alpha, void *x, const void *p, void *r, const void *Ap,
+                        int64_t n, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(alpha_kernel<double>, dim3(nb), dim3(NT), 0, s, alpha, (double *)x, (const double *)p, (double *)r, (const double *)Ap, n),
+        hipLaunchKernelGGL(alpha_kernel<float>, dim3(nb), dim3(NT), 0, s, (float)alpha, (float *)x, (const float *)p, (float *)r, (const float *)Ap, n));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_cg_init_residual(int dtype, const void *b, void *r, void *p, int64_t n, double *part_rr,
+                            double *part_bb, int nblk, hipStream_t s) {
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(cg_init_kernel<double>, dim3(nblk), dim3(NT), 0, s, (const double *)b, (double *)r, (double *)p, n, part_rr, part_bb),
+        hipLaunchKernelGGL(cg_init_kernel<float>, dim3(nblk), dim3(NT), 0, s, (const float *)b, (float *)r, (float *)p, n, part_rr, part_bb));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_cg_init_finalize(const double *part_rr, const double *part_bb, int np, cg_state *st,
+                            double tol, int tol_mode, int maxit, hipStream_t s) {
+    hipLaunchKernelGGL(cg_init_finalize_kernel, dim3(1), dim3(NT), 0, s, part_rr, part_bb, np, st, tol,
+                       tol_mode, maxit);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *Ap, int64_t n,
+                        const double *part_pAp, int np_pAp, double *part_rr, int nblk, cg_state *st,
+                        int slot, hipStream_t s) {
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(cg_update_xr_kernel<double>, dim3(nblk), dim3(NT), 0, s, (double *)x, (const double *)p, (double *)r, (const double *)Ap, n, part_pAp, np_pAp, part_rr, st, slot),
+        hipLaunchKernelGGL(cg_update_xr_kernel<float>, dim3(nblk), dim3(NT), 0, s, (float *)x, (const float *)p, (float *)r, (const float *)Ap, n, part_pAp, np_pAp, part_rr, st, slot));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_cg_update_p(int dtype, const void *r, void *p, int64_t n, const double *part_rr, int np_rr,
+                       int nblk, cg_state *st, int slot, hipStream_t s) {
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(cg_update_p_kernel<double>, dim3(nblk), dim3(NT), 0, s, (const double *)r, (double *)p, n, part_rr, np_rr, st, slot),
+        hipLaunchKernelGGL(cg_update_p_kernel<float>, dim3(nblk), dim3(NT), 0, s, (const float *)r, (float *)p, n, part_rr, np_rr, st, slot));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_residual(int dtype, const void *b, const void *Ax, void *r, int64_t n, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(residual_kernel<double>, dim3(nb), dim3(NT), 0, s, (const double *)b, (const double *)Ax, (double *)r, n),
+        hipLaunchKernelGGL(residual_kernel<float>, dim3(nb), dim3(NT), 0, s, (const float *)b, (const float *)Ax, (float *)r, n));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_convert(int from_dtype, int to_dtype, const void *src, void *dst, int64_t n, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    if (from_dtype == SIPG_F64 && to_dtype == SIPG_F32)
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nb), dim3(NT), 0, s, (const double *)src, (float *)dst, n);
+    else if (from_dtype == SIPG_F32 && to_dtype == SIPG_F64)
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(nb), dim3(NT), 0, s, (const float *)src, (double *)dst, n);
+    else if (from_dtype == SIPG_F64)
+        hipLaunchKernelGGL((convert_kernel<double, double>), dim3(nb), dim3(NT), 0, s, (const double *)src, (double *)dst, n);
+    else
+        hipLaunchKernelGGL((convert_kernel<float, float>), dim3(nb), dim3(NT), 0, s, (const float *)src, (float *)dst, n);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_rhs_fourier(int dtype, int p, int dim, const void *tab_f64, int nmodes, const double *a_dev,
+                       const int *m_dev, const int *n_dev, void *b, hipStream_t s) {
+    const int64_t n = (int64_t)dim * dim * (p + 1) * (p + 1);
+    int64_t nb = (n + NT - 1) / NT;
+    if (nb > 4096) nb = 4096;
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(rhs_fourier_kernel<double>, dim3((int)nb), dim3(NT), 0, s, p, dim, (const double *)tab_f64, nmodes, a_dev, m_dev, n_dev, (double *)b),
+        hipLaunchKernelGGL(rhs_fourier_kernel<float>, dim3((int)nb), dim3(NT), 0, s, p, dim, (const double *)tab_f64, nmodes, a_dev, m_dev, n_dev, (float *)b));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+}  // namespace sipg_internal

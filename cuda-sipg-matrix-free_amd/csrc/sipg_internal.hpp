@@ -1,0 +1,121 @@
+// sipg_internal.hpp -- shared definitions of libsipg_hip (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstddef>
+#include <vector>
+
+#include "../../include/sipg_hip.h"
+
+#define SIPG_HIP_TRY(expr)                                  \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) {                             \
+            sipg_internal::set_last_hip_error(_e);          \
+            return _e == hipErrorOutOfMemory ? SIPG_ERR_NOMEM : SIPG_ERR_HIP; \
+        }                                                   \
+    } while (0)
+
+namespace sipg_internal {
+
+// Highest degree with a compiled fused kernel.
+constexpr int kMaxDegree = 16;
+
+void set_last_hip_error(hipError_t e);
+
+// ---- host basis numerics (basis.cpp) --------------------------------------
+// GLL nodes/weights on [-1,1], p+1 points (LGL_quadrature_table.hpp:84-146).
+void gll(int p, double *nodes, double *weights);
+// D[mode*(p+1)+node] = phi'_mode(z_node) (sem_function.hpp:86-110).
+void dphi(int p, const double *nodes, double *D);
+// K1[a*(p+1)+b] = sum_k w_k D_a(k) D_b(k) (laplacian_operator.hpp:31-43).
+void stiffness_1d(int p, const double *w, const double *D, double *K1);
+
+// Device table layout used by every kernel, in the op's dtype:
+//   [0, N1)              w       LGL weights
+//   [N1, N1+N1^2)        K1      1-D stiffness, row-major
+//   [.., +N1)            D0      D0[i] = D(i, node 0)  = phi'_i(-1)
+//   [.., +N1)            DP      DP[i] = D(i, node p)  = phi'_i(+1)
+//   [.., +N1)            z       LGL nodes
+inline int table_len(int p) { return (p + 1) * (p + 1) + 4 * (p + 1); }
+void build_tables(int p, std::vector<double> &tab);
+
+// ---- CG scalar state, resident on the device ------------------------------
+// Two slots alternate between iterations so that a kernel never reads a
+// scalar that a block of the same kernel writes.
+struct cg_slot {
+    double rr;     // r.r entering the iteration
+    double alpha;
+    double beta;
+    double pAp;
+};
+struct cg_state {
+    cg_slot s[2];
+    double bb;       // b.b
+    double stop;     // stopping threshold on sqrt(r.r)
+    double rr0;      // initial r.r
+    int it;          // iterations done
+    int maxit;
+    int done;        // 0 running, 1 finished (converged or cap), 2 breakdown, 3 non-finite
+    int pad;
+};
+
+// Reduction partial buffers hold at most this many block sums.
+constexpr int kMaxPartials = 2048;
+constexpr int kVecThreads = 256;
+
+// ---- launchers (kernels_apply.hip) ----------------------------------------
+// Number of thread blocks the fused apply uses for a mesh (= partial count).
+int apply_num_blocks(int dtype, int p, int dimx, int dimy);
+// y = A x; with partials != nullptr also writes per-block sums of x.y and
+// returns early when st->done is set.
+int launch_apply(int dtype, int p, const void *x, void *y, const void *tab, int dimx, int dimy,
+                 double pen, double *partials, const cg_state *st, hipStream_t s);
+
+// ---- launchers (kernels_vec.hip) ------------------------------------------
+int vec_num_blocks(int64_t n);
+int launch_dot_partials(int dtype, const void *a, const void *b, int64_t n, double *partials,
+                        int nblk, hipStream_t s);
+int launch_reduce_partials(const double *partials, int np, double *out, hipStream_t s);
+int launch_axpy(int dtype, double alpha, const void *x, void *y, int64_t n, hipStream_t s);
+int launch_xpby(int dtype, const void *r, double beta, void *p, int64_t n, hipStream_t s);
+int launch_alpha_update(int dtype, double alpha, void *x, const void *p, void *r, const void *Ap,
+                        int64_t n, hipStream_t s);
+// CG pieces with device-resident scalars
+int launch_cg_init_residual(int dtype, const void *b, void *r, void *p, int64_t n,
+                            double *part_rr, double *part_bb, int nblk, hipStream_t s);
+int launch_cg_init_finalize(const double *part_rr, const double *part_bb, int np, cg_state *st,
+                            double tol, int tol_mode, int maxit, hipStream_t s);
+int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *Ap, int64_t n,
+                        const double *part_pAp, int np_pAp, double *part_rr, int nblk,
+                        cg_state *st, int slot, hipStream_t s);
+int launch_cg_update_p(int dtype, const void *r, void *p, int64_t n, const double *part_rr,
+                       int np_rr, int nblk, cg_state *st, int slot, hipStream_t s);
+int launch_residual(int dtype, const void *b, const void *Ax, void *r, int64_t n, hipStream_t s);
+int launch_convert(int from_dtype, int to_dtype, const void *src, void *dst, int64_t n,
+                   hipStream_t s);
+int launch_rhs_fourier(int dtype, int p, int dim, const void *tab_f64, int nmodes,
+                       const double *a_dev, const int *m_dev, const int *n_dev, void *b,
+                       hipStream_t s);
+
+}  // namespace sipg_internal
+
+// ---- the opaque operator --------------------------------------------------
+struct sipg_op {
+    int dimx = 0, dimy = 0, p = 0, dtype = SIPG_F64;
+    double pen = 0;
+    int64_t n = 0;
+    hipStream_t stream = nullptr;
+    std::vector<double> h_tab;        // host copy (double) of the tables
+    void *d_tab = nullptr;            // tables in the op's dtype
+    void *d_tab64 = nullptr;          // tables in double (for RHS and FP64 helpers)
+    double *d_part = nullptr;         // 4 x kMaxPartials reduction partials
+    double *d_scalar = nullptr;       // small device scalars
+    sipg_internal::cg_state *d_state = nullptr;
+    sipg_internal::cg_state *h_state = nullptr;   // pinned mirror
+    // cached CG workspace (r, p, Ap) in the op's dtype, allocated on first use
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+};

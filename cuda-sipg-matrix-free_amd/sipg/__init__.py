@@ -1,0 +1,283 @@
+"""Python front end of libsipg_hip.so (ctypes over the C-ABI in
+include/sipg_hip.h).
+
+This module holds no numerics: every operation is a call into the HIP
+library.  If the library is not built, loading fails loudly
+(`SipgLibraryMissing`); there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libsipg_hip.so")
+
+F64, F32 = 0, 1
+TOL_ABS, TOL_REL = 0, 1
+
+OK = 0
+ERR_INVALID = 1
+ERR_ALIAS = 2
+ERR_UNSUPPORTED = 3
+ERR_HIP = 4
+ERR_NOMEM = 5
+NOT_CONVERGED = 6
+ERR_NONFINITE = 7
+ERR_BREAKDOWN = 8
+ERR_BUDGET = 9
+ERR_CANCELLED = 10
+ERR_NOT_FOUND = 11
+
+
+class SipgLibraryMissing(RuntimeError):
+    pass
+
+
+class SipgError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__(f"{what}: {strerror(status)} (status {status})")
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _declare(L):
+    i = ctypes.c_int
+    d = ctypes.c_double
+    sz = ctypes.c_size_t
+    i64 = ctypes.c_int64
+    sig = {
+        "sipg_strerror": (ctypes.c_char_p, [i]),
+        "sipg_max_degree": (i, []),
+        "sipg_device_count": (i, [_ip]),
+        "sipg_set_device": (i, [i]),
+        "sipg_device_synchronize": (i, []),
+        "sipg_malloc": (i, [ctypes.POINTER(_vp), sz]),
+        "sipg_free": (i, [_vp]),
+        "sipg_memcpy_h2d": (i, [_vp, _vp, sz]),
+        "sipg_memcpy_d2h": (i, [_vp, _vp, sz]),
+        "sipg_memcpy_d2d": (i, [_vp, _vp, sz]),
+        "sipg_memset_zero": (i, [_vp, sz]),
+        "sipg_op_create": (i, [ctypes.POINTER(_vp), i, i, i, d, i, _vp]),
+        "sipg_op_destroy": (i, [_vp]),
+        "sipg_op_size": (i, [_vp, ctypes.POINTER(i64)]),
+        "sipg_op_set_stream": (i, [_vp, _vp]),
+        "sipg_op_tables": (i, [_vp, _dp, _dp, _dp, _dp]),
+        "sipg_apply": (i, [_vp, _vp, _vp]),
+        "sipg_dot": (i, [_vp, _vp, _vp, _dp]),
+        "sipg_axpy": (i, [_vp, d, _vp, _vp]),
+        "sipg_xpby": (i, [_vp, _vp, d, _vp]),
+        "sipg_alpha_update": (i, [_vp, d, _vp, _vp, _vp, _vp]),
+        "sipg_cg": (i, [_vp, _vp, _vp, d, i, i, _ip, _dp]),
+        "sipg_true_relres": (i, [_vp, _vp, _vp, _dp]),
+        "sipg_forcing_draw": (i, [ctypes.c_uint64, i, i, i, _ip, _dp, _ip, _ip]),
+        "sipg_rhs_fourier": (i, [_vp, i, _dp, _ip, _ip, _vp]),
+        "sipg_time_apply": (i, [_vp, _vp, _vp, i, _dp]),
+        "sipg_upload": (i, [_vp, _dp, _vp]),
+        "sipg_download": (i, [_vp, _vp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def lib():
+    """Load libsipg_hip.so from the package tree (raises if absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SipgLibraryMissing(
+                f"{LIB_PATH} is not built; run `make -C {PKG_DIR}` or __graft_entry__.build()")
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def strerror(status):
+    return lib().sipg_strerror(status).decode()
+
+
+def check(status, what=""):
+    if status != OK:
+        raise SipgError(status, what)
+    return status
+
+
+def max_degree():
+    return lib().sipg_max_degree()
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    check(lib().sipg_device_count(ctypes.byref(c)), "device_count")
+    return c.value
+
+
+def synchronize():
+    check(lib().sipg_device_synchronize(), "synchronize")
+
+
+def _np_dtype(dtype):
+    return np.float64 if dtype == F64 else np.float32
+
+
+class DeviceBuffer:
+    """A device allocation from the library allocator."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = _vp()
+        check(lib().sipg_malloc(ctypes.byref(self.ptr), self.nbytes), "malloc")
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes == self.nbytes
+        check(lib().sipg_memcpy_h2d(self.ptr, arr.ctypes.data_as(_vp), self.nbytes), "h2d")
+
+    def download(self, dtype, count=None):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize if count is None else count, dtype=dtype)
+        check(lib().sipg_memcpy_d2h(out.ctypes.data_as(_vp), self.ptr, out.nbytes), "d2h")
+        return out
+
+    def zero(self):
+        check(lib().sipg_memset_zero(self.ptr, self.nbytes), "memset")
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().sipg_free(self.ptr)
+        self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Operator:
+    """sipg_op: a dimx x dimy mesh, degree p, penalty pen, dtype."""
+
+    def __init__(self, dimx, dimy, p, pen, dtype=F64, stream=None):
+        self.dimx, self.dimy, self.p, self.pen, self.dtype = dimx, dimy, p, pen, dtype
+        self.h = _vp()
+        check(lib().sipg_op_create(ctypes.byref(self.h), dimx, dimy, p, float(pen), dtype,
+                                   _vp(stream) if stream else None), "op_create")
+        n = ctypes.c_int64()
+        check(lib().sipg_op_size(self.h, ctypes.byref(n)), "op_size")
+        self.n = n.value
+        self.itemsize = 8 if dtype == F64 else 4
+
+    def vector(self):
+        return DeviceBuffer(self.n * self.itemsize)
+
+    def upload(self, host, dev):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        assert host.size == self.n
+        check(lib().sipg_upload(self.h, host.ctypes.data_as(_dp), dev.ptr), "upload")
+
+    def download(self, dev):
+        out = np.empty(self.n, dtype=np.float64)
+        check(lib().sipg_download(self.h, dev.ptr, out.ctypes.data_as(_dp)), "download")
+        return out
+
+    def to_device(self, host):
+        v = self.vector()
+        self.upload(host, v)
+        return v
+
+    def apply(self, x, y):
+        return lib().sipg_apply(self.h, x.ptr, y.ptr)
+
+    def apply_host(self, xh):
+        x = self.to_device(xh)
+        y = self.vector()
+        check(self.apply(x, y), "apply")
+        out = self.download(y)
+        x.free()
+        y.free()
+        return out
+
+    def dot(self, a, b):
+        r = ctypes.c_double()
+        check(lib().sipg_dot(self.h, a.ptr, b.ptr, ctypes.byref(r)), "dot")
+        return r.value
+
+    def axpy(self, alpha, x, y):
+        check(lib().sipg_axpy(self.h, alpha, x.ptr, y.ptr), "axpy")
+
+    def xpby(self, r, beta, p):
+        check(lib().sipg_xpby(self.h, r.ptr, beta, p.ptr), "xpby")
+
+    def alpha_update(self, alpha, x, p, r, Ap):
+        check(lib().sipg_alpha_update(self.h, alpha, x.ptr, p.ptr, r.ptr, Ap.ptr), "alpha_update")
+
+    def cg(self, x, b, tol, tol_mode=TOL_REL, maxit=0):
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        st = lib().sipg_cg(self.h, x.ptr, b.ptr, tol, tol_mode, maxit, ctypes.byref(it), ctypes.byref(res))
+        return st, it.value, res.value
+
+    def true_relres(self, x, b):
+        r = ctypes.c_double()
+        check(lib().sipg_true_relres(self.h, x.ptr, b.ptr, ctypes.byref(r)), "true_relres")
+        return r.value
+
+    def rhs_fourier(self, modes, b):
+        a, m, n = modes
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        check(lib().sipg_rhs_fourier(self.h, len(a), a.ctypes.data_as(_dp), m.ctypes.data_as(_ip),
+                                     n.ctypes.data_as(_ip), b.ptr), "rhs_fourier")
+
+    def time_apply(self, x, y, reps):
+        ms = ctypes.c_double()
+        check(lib().sipg_time_apply(self.h, x.ptr, y.ptr, reps, ctypes.byref(ms)), "time_apply")
+        return ms.value
+
+    def tables(self):
+        n1 = self.p + 1
+        z, w = np.zeros(n1), np.zeros(n1)
+        D, K = np.zeros(n1 * n1), np.zeros(n1 * n1)
+        check(lib().sipg_op_tables(self.h, z.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
+                                   D.ctypes.data_as(_dp), K.ctypes.data_as(_dp)), "tables")
+        return z, w, D.reshape(n1, n1), K.reshape(n1, n1)
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib().sipg_op_destroy(self.h)
+        self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def forcing_draw(seed, kmin=4, kmax=4, mmax=4):
+    K = max(kmax, 1)
+    a = np.zeros(K)
+    m = np.zeros(K, dtype=np.int32)
+    n = np.zeros(K, dtype=np.int32)
+    nm = ctypes.c_int()
+    check(lib().sipg_forcing_draw(ctypes.c_uint64(seed), kmin, kmax, mmax, ctypes.byref(nm),
+                                  a.ctypes.data_as(_dp), m.ctypes.data_as(_ip), n.ctypes.data_as(_ip)),
+          "forcing_draw")
+    k = nm.value
+    return a[:k].copy(), m[:k].copy(), n[:k].copy()
+
+
+def pen_sigma(p):
+    """Kernel penalty for sigma = 10 (p+1)^2 / h: pen = sigma/2 = 5 (p+1)^2."""
+    return 5.0 * (p + 1) ** 2

@@ -1,0 +1,143 @@
+/*
+ * sipg_hip.h -- C-ABI of the MI355X-native matrix-free SIPG/SEM Poisson
+ * solver (libsipg_hip.so, gfx950).
+ *
+ * Plain C: opaque handles, plain pointers and sizes.  Device pointers are
+ * ordinary HIP device allocations (sipg_malloc or hipMalloc).  Every
+ * function returns an int status (SIPG_OK = 0) and never exits the process;
+ * sipg_strerror() turns a status into text.
+ *
+ * Vector layout ("mode_vector" order): the dof (i1,i2,e) of an element
+ * e = dimx*y + x lives at index ((i1*(p+1)+i2)*noe + e), noe = dimx*dimy,
+ * i1 the x-node, i2 the y-node (reference mode_vector.hpp:108-120,
+ * mesh_info.hpp:231-235).  The device layout IS this order, so a reference
+ * mode_vector's device array can be passed as is.
+ *
+ * Penalty: `pen` is the reference kernels' pen (sipg_sem_2d_gpu_kernels.hpp:
+ * 243-244, 308-309); the physical SIPG sigma of sigma/h*int[u][v] is 2*pen.
+ */
+#ifndef SIPG_HIP_H
+#define SIPG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    SIPG_OK = 0,
+    SIPG_ERR_INVALID = 1,      /* bad argument (sizes, null pointers, degree) */
+    SIPG_ERR_ALIAS = 2,        /* input and output of an apply overlap        */
+    SIPG_ERR_UNSUPPORTED = 3,  /* degree / dtype combination not built         */
+    SIPG_ERR_HIP = 4,          /* a HIP runtime call failed                    */
+    SIPG_ERR_NOMEM = 5,        /* device allocation failed                     */
+    SIPG_NOT_CONVERGED = 6,    /* iteration cap reached before the tolerance   */
+    SIPG_ERR_NONFINITE = 7,    /* NaN/Inf met in a residual or dot product     */
+    SIPG_ERR_BREAKDOWN = 8,    /* CG breakdown: p.Ap <= 0 (A not SPD)          */
+    SIPG_ERR_BUDGET = 9,       /* request exceeds the configured memory budget */
+    SIPG_ERR_CANCELLED = 10,   /* request cancelled or timed out               */
+    SIPG_ERR_NOT_FOUND = 11    /* unknown request / handle id                  */
+};
+
+/* ---- enums ------------------------------------------------------------- */
+enum { SIPG_F64 = 0, SIPG_F32 = 1 };                 /* vector element type */
+enum { SIPG_TOL_ABS = 0, SIPG_TOL_REL = 1 };          /* CG stopping test    */
+
+typedef struct sipg_op sipg_op;   /* operator: mesh, degree, penalty, tables */
+
+/* ---- library ----------------------------------------------------------- */
+const char *sipg_strerror(int status);
+/* Highest polynomial degree the fused kernels are built for. */
+int sipg_max_degree(void);
+/* Device count / selection helpers (thin HIP wrappers, no torch). */
+int sipg_device_count(int *count);
+int sipg_set_device(int device);
+int sipg_device_synchronize(void);
+
+/* ---- device memory (library allocator; plain HIP device memory) -------- */
+int sipg_malloc(void **ptr, size_t bytes);
+int sipg_free(void *ptr);
+int sipg_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int sipg_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int sipg_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int sipg_memset_zero(void *dst, size_t bytes);
+
+/* ---- operator ---------------------------------------------------------- */
+/* Replaces the reference's process-global __constant__ tables
+ * (device_const.hpp:10-18, 70-174) and the launch set-up of the
+ * sipg_sem_2d constructor (sipg_sem_2d.hpp:96-113): the op owns its degree
+ * tables in device memory, so ops of different degrees coexist.
+ * stream: a hipStream_t, or NULL for the default stream. */
+int sipg_op_create(sipg_op **op, int dimx, int dimy, int p, double pen, int dtype, void *stream);
+int sipg_op_destroy(sipg_op *op);
+/* n = dimx*dimy*(p+1)^2 */
+int sipg_op_size(const sipg_op *op, int64_t *n);
+int sipg_op_set_stream(sipg_op *op, void *stream);
+/* Host copies of the op's tables (lengths p+1, p+1, (p+1)^2, (p+1)^2):
+ * LGL nodes/weights (LGL_quadrature_table.hpp:84-146), D[mode*(p+1)+node]
+ * (device_const.hpp:115-127), K1 (laplacian_operator.hpp:31-43). */
+int sipg_op_tables(const sipg_op *op, double *nodes, double *weights, double *D, double *K1);
+
+/* y = A x.  Replaces sipg_sem_2d<T>::_mvm (sipg_sem_2d.hpp:167-201): the
+ * three launches volume + flux_term6a + flux_term6b become one fused
+ * kernel.  x, y: device arrays of n elements of the op's dtype; they must
+ * not overlap (SIPG_ERR_ALIAS).  Asynchronous on the op's stream. */
+int sipg_apply(sipg_op *op, const void *x, void *y);
+
+/* ---- CG vector kernels (op's dtype, length n, op's stream) ------------- */
+/* *result = a.b, accumulated in double, deterministic order.  Replaces
+ * cublas_dot (cublas_wrapper.hpp:62-84).  Synchronous (result on host). */
+int sipg_dot(sipg_op *op, const void *a, const void *b, double *result);
+/* y += alpha x */
+int sipg_axpy(sipg_op *op, double alpha, const void *x, void *y);
+/* p = r + beta p   (beta_kernel, conjugate_gradient_kernels.hpp:26-38) */
+int sipg_xpby(sipg_op *op, const void *r, double beta, void *p);
+/* x += alpha p; r -= alpha Ap   (alpha_kernel, conjugate_gradient_kernels.hpp:5-23) */
+int sipg_alpha_update(sipg_op *op, double alpha, void *x, const void *p, void *r, const void *Ap);
+
+/* ---- CG solve ---------------------------------------------------------- */
+/* Unpreconditioned CG on A x = b with device-resident scalars.  Replaces
+ * conjugate_gradient (conjugate_gradient_gpu.hpp:18-109).  x: initial
+ * guess in, solution out.  tol_mode SIPG_TOL_ABS stops on sqrt(r.r) <= tol
+ * (the reference's test, :63,69), SIPG_TOL_REL on sqrt(r.r) <= tol*||b||.
+ * maxit <= 0 means n (the reference's cap, :65).  On return *iters is the
+ * iteration count and *res_norm the final sqrt(r.r) of the recursive
+ * residual.  Returns SIPG_OK, SIPG_NOT_CONVERGED, SIPG_ERR_BREAKDOWN or
+ * SIPG_ERR_NONFINITE (x then holds the last iterate). */
+int sipg_cg(sipg_op *op, void *x, const void *b, double tol, int tol_mode, int maxit,
+            int *iters, double *res_norm);
+
+/* ||b - A x||_2 / ||b||_2 in FP64 (op must be SIPG_F64). */
+int sipg_true_relres(sipg_op *op, const void *x, const void *b, double *relres);
+
+/* ---- right-hand side and error norms (device) -------------------------- */
+/* Seeded Fourier forcing u = sum_k a_k sin(m_k pi x) sin(n_k pi y) drawn
+ * from std::mt19937_64(seed): K = kmin + r%(kmax-kmin+1), then per mode
+ * a = -1 + 2*((r>>11)*2^-53), m = 1 + r%mmax, n = 1 + r%mmax, one raw
+ * 64-bit draw r each, in that order.  Arrays must hold kmax entries. */
+int sipg_forcing_draw(uint64_t seed, int kmin, int kmax, int mmax,
+                      int *nmodes, double *a, int *m, int *n);
+/* b = load vector of the Fourier problem on the op's square mesh, with
+ * homogeneous Dirichlet data (sipg_sem_2d.hpp:257-262 with g = 0), in the
+ * op's dtype, computed on the device. */
+int sipg_rhs_fourier(sipg_op *op, int nmodes, const double *a, const int *m, const int *n, void *b);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Runs `reps` back-to-back applies y = A x on the op's stream between two
+ * HIP events and returns the mean launch-to-launch time in milliseconds
+ * (the reference's event-timed driver: mvm_performance_test.cu:44-65). */
+int sipg_time_apply(sipg_op *op, const void *x, void *y, int reps, double *ms_per_apply);
+
+/* ---- layout helpers ---------------------------------------------------- */
+/* Device layout == host_mode_vector layout, so these are typed copies;
+ * they also convert between FP64 host data and an FP32 op. */
+int sipg_upload(sipg_op *op, const double *host, void *dev);
+int sipg_download(sipg_op *op, const void *dev, double *host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,96 @@
+"""GPU parity of the fused operator apply against the oracle (the
+reference's volume + flux_term6a + flux_term6b restated).  FP64 bar:
+max|y_gpu - y_oracle| <= 1e-12 * max|y_oracle|."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL64 = 1e-12
+
+
+def _x(n, seed):
+    return np.random.default_rng(seed).uniform(-1, 1, n)
+
+
+@pytest.mark.parametrize("p", list(range(1, 17)))
+@pytest.mark.parametrize("dx,dy", [(5, 3), (17, 9), (1, 1), (16, 4), (33, 2)])
+def test_apply_f64_matches_oracle(sipg, oracle, p, dx, dy):
+    pen = 5.0 * (p + 1) ** 2
+    op = sipg.Operator(dx, dy, p, pen, sipg.F64)
+    x = _x(op.n, 100 * p + dx)
+    y = op.apply_host(x)
+    ref = oracle.apply(dx, dy, p, pen, x)
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    assert err <= TOL64, err
+
+
+@pytest.mark.parametrize("p", [2, 3, 8, 12, 15, 16])
+@pytest.mark.parametrize("dx,dy", [(7, 5), (16, 8)])
+def test_apply_f32_close_to_oracle(sipg, oracle, p, dx, dy):
+    pen = 5.0 * (p + 1) ** 2
+    op = sipg.Operator(dx, dy, p, pen, sipg.F32)
+    x = _x(op.n, 7 * p + dy).astype(np.float32).astype(np.float64)
+    y = op.apply_host(x)
+    ref = oracle.apply(dx, dy, p, pen, x)
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    # FP32 arithmetic: sums of O(p) terms of size O(pen)
+    assert err <= 2e-6 * (p + 1), err
+
+
+@pytest.mark.parametrize("name", ["d4p3", "d3p8", "d2p15", "r5x3p2"])
+def test_apply_golden_vectors(sipg, name):
+    g = np.load(os.path.join(GOLD, "apply_vectors.npz"))
+    dx, dy, p, pen = g[name + "_shape"]
+    op = sipg.Operator(int(dx), int(dy), int(p), float(pen), sipg.F64)
+    y = op.apply_host(g[name + "_x"])
+    ref = g[name + "_y"]
+    assert np.max(np.abs(y - ref)) <= TOL64 * np.max(np.abs(ref))
+
+
+def test_apply_full_size_c2(sipg, oracle):
+    # configuration C2: 256 x 256 elements, p = 8, pen = 5 (p+1)^2
+    dim, p = 256, 8
+    pen = 5.0 * (p + 1) ** 2
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    x = _x(op.n, 1)
+    y = op.apply_host(x)
+    ref = oracle.apply(dim, dim, p, pen, x)
+    assert np.max(np.abs(y - ref)) <= TOL64 * np.max(np.abs(ref))
+
+
+def test_apply_deterministic_and_linear(sipg):
+    op = sipg.Operator(24, 20, 6, 245.0, sipg.F64)
+    x1, x2 = _x(op.n, 1), _x(op.n, 2)
+    y1 = op.apply_host(x1)
+    assert np.array_equal(y1, op.apply_host(x1))
+    y2 = op.apply_host(x2)
+    y12 = op.apply_host(2.0 * x1 - 3.0 * x2)
+    assert np.max(np.abs(y12 - (2 * y1 - 3 * y2))) <= 1e-12 * np.max(np.abs(y12))
+
+
+def test_apply_symmetric(sipg):
+    op = sipg.Operator(9, 7, 5, 180.0, sipg.F64)
+    x, z = _x(op.n, 3), _x(op.n, 4)
+    ax, az = op.apply_host(x), op.apply_host(z)
+    assert abs(z @ ax - x @ az) <= 1e-12 * abs(z @ ax)
+
+
+def test_apply_rejects_aliasing(sipg):
+    op = sipg.Operator(4, 4, 3, 80.0, sipg.F64)
+    v = op.to_device(_x(op.n, 5))
+    assert op.apply(v, v) == sipg.ERR_ALIAS
+
+
+def test_tables_match_oracle(sipg, oracle):
+    for p in (1, 4, 8, 15, 16):
+        op = sipg.Operator(2, 2, p, 1.0, sipg.F64)
+        z, w, D, K = op.tables()
+        assert np.max(np.abs(z - oracle.gll(p)[0])) < 1e-15 * (p + 1)
+        assert np.max(np.abs(w - oracle.gll(p)[1])) < 1e-15 * (p + 1)
+        Do = oracle.dphi(p)
+        assert np.max(np.abs(D - Do)) < 1e-14 * np.max(np.abs(Do)) * (p + 1)
+        Ko = oracle.k1(p)
+        assert np.max(np.abs(K - Ko)) < 1e-14 * np.max(np.abs(Ko)) * (p + 1)
