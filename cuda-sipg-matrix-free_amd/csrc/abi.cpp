@@ -15,15 +15,9 @@ using namespace sipg_internal;
 
 namespace {
 thread_local hipError_t g_last_hip = hipSuccess;
+}  // namespace
 
-size_t dtype_size(int dtype) { return dtype == SIPG_F64 ? 8 : 4; }
-
-bool overlaps(const void *a, const void *b, size_t bytes) {
-    const char *pa = (const char *)a, *pb = (const char *)b;
-    return pa < pb + bytes && pb < pa + bytes;
-}
-
-int ensure_ws(sipg_op *op, size_t bytes) {
+int sipg_internal::ensure_ws(sipg_op *op, size_t bytes) {
     if (op->ws_bytes >= bytes) return SIPG_OK;
     if (op->ws) {
         SIPG_HIP_TRY(hipStreamSynchronize(op->stream));
@@ -36,12 +30,7 @@ int ensure_ws(sipg_op *op, size_t bytes) {
     return SIPG_OK;
 }
 
-// Partial-buffer layout inside op->d_part
-struct parts {
-    double *pAp, *rr, *bb, *misc;
-    int napply;
-};
-parts part_layout(sipg_op *op) {
+sipg_internal::parts sipg_internal::part_layout(sipg_op *op) {
     parts p;
     p.napply = apply_num_blocks(op->dtype, op->p, op->dimx, op->dimy);
     p.pAp = op->d_part;
@@ -50,7 +39,6 @@ parts part_layout(sipg_op *op) {
     p.misc = p.bb + kMaxPartials;
     return p;
 }
-}  // namespace
 
 void sipg_internal::set_last_hip_error(hipError_t e) { g_last_hip = e; }
 
@@ -75,6 +63,20 @@ const char *sipg_strerror(int s) {
 }
 
 int sipg_max_degree(void) { return kMaxDegree; }
+
+int sipg_basis_tables(int p, double *nodes, double *weights, double *D, double *K1) {
+    if (p < 1 || p > 63) return SIPG_ERR_INVALID;
+    const int n1 = p + 1;
+    std::vector<double> z(n1), w(n1), Dm(n1 * n1), K(n1 * n1);
+    gll(p, z.data(), w.data());
+    dphi(p, z.data(), Dm.data());
+    stiffness_1d(p, w.data(), Dm.data(), K.data());
+    if (nodes) std::memcpy(nodes, z.data(), n1 * sizeof(double));
+    if (weights) std::memcpy(weights, w.data(), n1 * sizeof(double));
+    if (D) std::memcpy(D, Dm.data(), n1 * n1 * sizeof(double));
+    if (K1) std::memcpy(K1, K.data(), n1 * n1 * sizeof(double));
+    return SIPG_OK;
+}
 
 int sipg_device_count(int *count) {
     if (!count) return SIPG_ERR_INVALID;
@@ -175,6 +177,7 @@ int sipg_op_destroy(sipg_op *op) {
     if (op->d_state) (void)hipFree(op->d_state);
     if (op->h_state) (void)hipHostFree(op->h_state);
     if (op->ws) (void)hipFree(op->ws);
+    if (op->twin) sipg_op_destroy(op->twin);
     delete op;
     return SIPG_OK;
 }
@@ -193,23 +196,14 @@ int sipg_op_set_stream(sipg_op *op, void *stream) {
 
 int sipg_op_tables(const sipg_op *op, double *nodes, double *weights, double *D, double *K1) {
     if (!op) return SIPG_ERR_INVALID;
-    const int n1 = op->p + 1;
-    std::vector<double> z(n1), w(n1), Dm(n1 * n1), K(n1 * n1);
-    gll(op->p, z.data(), w.data());
-    dphi(op->p, z.data(), Dm.data());
-    stiffness_1d(op->p, w.data(), Dm.data(), K.data());
-    if (nodes) std::memcpy(nodes, z.data(), n1 * sizeof(double));
-    if (weights) std::memcpy(weights, w.data(), n1 * sizeof(double));
-    if (D) std::memcpy(D, Dm.data(), n1 * n1 * sizeof(double));
-    if (K1) std::memcpy(K1, K.data(), n1 * n1 * sizeof(double));
-    return SIPG_OK;
+    return sipg_basis_tables(op->p, nodes, weights, D, K1);
 }
 
 int sipg_apply(sipg_op *op, const void *x, void *y) {
     if (!op || !x || !y) return SIPG_ERR_INVALID;
     if (overlaps(x, y, (size_t)op->n * dtype_size(op->dtype))) return SIPG_ERR_ALIAS;
     return launch_apply(op->dtype, op->p, x, y, op->d_tab, op->dimx, op->dimy, op->pen, nullptr,
-                        nullptr, op->stream);
+                        nullptr, 0, op->stream);
 }
 
 int sipg_dot(sipg_op *op, const void *a, const void *b, double *result) {
@@ -240,117 +234,37 @@ int sipg_alpha_update(sipg_op *op, double alpha, void *x, const void *p, void *r
     return launch_alpha_update(op->dtype, alpha, x, p, r, Ap, op->n, op->stream);
 }
 
-int sipg_cg(sipg_op *op, void *x, const void *b, double tol, int tol_mode, int maxit, int *iters,
-            double *res_norm) {
-    if (!op || !x || !b) return SIPG_ERR_INVALID;
-    if (!(tol >= 0.0) || (tol_mode != SIPG_TOL_ABS && tol_mode != SIPG_TOL_REL)) return SIPG_ERR_INVALID;
-    const size_t sz = dtype_size(op->dtype);
-    const size_t vb = (size_t)op->n * sz;
-    if (overlaps(x, b, vb)) return SIPG_ERR_ALIAS;
-    int rc = ensure_ws(op, 3 * vb);
-    if (rc) return rc;
-    char *ws = (char *)op->ws;
-    void *r = ws, *p = ws + vb, *Ap = ws + 2 * vb;
-    const int64_t cap = maxit > 0 ? (int64_t)maxit : op->n;
-    const int imax = (int)std::min<int64_t>(cap, INT_MAX);
-    parts P = part_layout(op);
-    const int nb = vec_num_blocks(op->n);
-    hipStream_t s = op->stream;
-    cg_state *st = op->d_state;
-
-    // r = b - A x ; p = r ; r.r, b.b   (conjugate_gradient_gpu.hpp:40-60)
-    if ((rc = launch_apply(op->dtype, op->p, x, r, op->d_tab, op->dimx, op->dimy, op->pen, nullptr,
-                           nullptr, s)))
-        return rc;
-    if ((rc = launch_cg_init_residual(op->dtype, b, r, p, op->n, P.rr, P.bb, nb, s))) return rc;
-    if ((rc = launch_cg_init_finalize(P.rr, P.bb, nb, st, tol, tol_mode, imax, s))) return rc;
-
-    // p.Ap block sums: re-reduced in every consumer block when few, else
-    // folded to one value by a one-block kernel first.
-    const bool fold = P.napply > kMaxPartials;
-    const double *pAp_src = fold ? P.misc : P.pAp;
-    const int np_pAp = fold ? 1 : P.napply;
-
-    auto launch_iter = [&](int64_t k) -> int {
-        const int slot = (int)(k & 1);
-        int r2;
-        if ((r2 = launch_apply(op->dtype, op->p, p, Ap, op->d_tab, op->dimx, op->dimy, op->pen,
-                               P.pAp, st, s)))
-            return r2;
-        if (fold && (r2 = launch_reduce_partials(P.pAp, P.napply, P.misc, s))) return r2;
-        if ((r2 = launch_cg_update_xr(op->dtype, x, p, r, Ap, op->n, pAp_src, np_pAp, P.rr, nb, st,
-                                      slot, s)))
-            return r2;
-        return launch_cg_update_p(op->dtype, r, p, op->n, P.rr, nb, nb, st, slot, s);
-    };
-
-    // Host loop: keep two chunks of iterations queued; poll the state copy
-    // of the older chunk.  Kernels become no-ops once the state says done.
-    hipEvent_t ev[2];
-    SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    int64_t launched = 0;
-    int chunk = 4;
-    int inflight = 0;
-    int buf = 0;
-    rc = SIPG_OK;
-    // initial state check
-    SIPG_HIP_TRY(hipMemcpyAsync(&op->h_state[0], st, sizeof(cg_state), hipMemcpyDeviceToHost, s));
-    SIPG_HIP_TRY(hipStreamSynchronize(s));
-    bool finished = op->h_state[0].done != 0;
-    while (!finished) {
-        // enqueue one chunk
-        for (int c = 0; c < chunk && launched < imax; ++c, ++launched)
-            if ((rc = launch_iter(launched))) break;
-        if (rc) break;
-        SIPG_HIP_TRY(hipMemcpyAsync(&op->h_state[buf], st, sizeof(cg_state), hipMemcpyDeviceToHost, s));
-        SIPG_HIP_TRY(hipEventRecord(ev[buf], s));
-        ++inflight;
-        buf ^= 1;
-        if (chunk < 64) chunk *= 2;
-        if (inflight == 2 || launched >= imax) {
-            // wait for the older chunk
-            SIPG_HIP_TRY(hipEventSynchronize(ev[buf]));
-            --inflight;
-            if (op->h_state[buf].done) finished = true;
-            if (launched >= imax) {
-                SIPG_HIP_TRY(hipStreamSynchronize(s));
-                finished = true;
-            }
-        }
+int sipg_vec_dot(int dtype, int64_t n, const void *a, const void *b, double *result, void *stream) {
+    if (!a || !b || !result || n < 0 || (dtype != SIPG_F64 && dtype != SIPG_F32)) return SIPG_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = vec_num_blocks(n);
+    double *tmp = nullptr;
+    SIPG_HIP_TRY(hipMallocAsync((void **)&tmp, (nb + 1) * sizeof(double), s));
+    int rc = launch_dot_partials(dtype, a, b, n, tmp + 1, nb, s);
+    if (!rc) rc = launch_reduce_partials(tmp + 1, nb, tmp, s);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(result, tmp, sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { set_last_hip_error(e); rc = SIPG_ERR_HIP; }
     }
-    SIPG_HIP_TRY(hipStreamSynchronize(s));
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
-    if (rc) return rc;
-    SIPG_HIP_TRY(hipMemcpy(&op->h_state[0], st, sizeof(cg_state), hipMemcpyDeviceToHost));
-    const cg_state &hs = op->h_state[0];
-    if (iters) *iters = hs.it;
-    if (res_norm) *res_norm = std::sqrt(hs.s[hs.it & 1].rr);
-    if (hs.done == 2) return SIPG_ERR_BREAKDOWN;
-    if (hs.done == 3) return SIPG_ERR_NONFINITE;
-    if (!(std::sqrt(hs.s[hs.it & 1].rr) <= hs.stop)) return SIPG_NOT_CONVERGED;
-    return SIPG_OK;
+    (void)hipFreeAsync(tmp, s);
+    return rc;
 }
 
-int sipg_true_relres(sipg_op *op, const void *x, const void *b, double *relres) {
-    if (!op || !x || !b || !relres) return SIPG_ERR_INVALID;
-    if (op->dtype != SIPG_F64) return SIPG_ERR_INVALID;
-    const size_t vb = (size_t)op->n * 8;
-    int rc = ensure_ws(op, 3 * vb);
-    if (rc) return rc;
-    char *ws = (char *)op->ws;
-    void *Ax = ws + 2 * vb;   // reuse the Ap slot
-    void *r = ws;
-    if ((rc = launch_apply(op->dtype, op->p, x, Ax, op->d_tab, op->dimx, op->dimy, op->pen, nullptr,
-                           nullptr, op->stream)))
-        return rc;
-    if ((rc = launch_residual(op->dtype, b, Ax, r, op->n, op->stream))) return rc;
-    double rr = 0, bb = 0;
-    if ((rc = sipg_dot(op, r, r, &rr))) return rc;
-    if ((rc = sipg_dot(op, b, b, &bb))) return rc;
-    *relres = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
-    return SIPG_OK;
+int sipg_vec_axpy(int dtype, int64_t n, double alpha, const void *x, void *y, void *stream) {
+    if (!x || !y || n < 0 || (dtype != SIPG_F64 && dtype != SIPG_F32)) return SIPG_ERR_INVALID;
+    return launch_axpy(dtype, alpha, x, y, n, (hipStream_t)stream);
+}
+
+int sipg_vec_xpby(int dtype, int64_t n, const void *r, double beta, void *p, void *stream) {
+    if (!r || !p || n < 0 || (dtype != SIPG_F64 && dtype != SIPG_F32)) return SIPG_ERR_INVALID;
+    return launch_xpby(dtype, r, beta, p, n, (hipStream_t)stream);
+}
+
+int sipg_vec_alpha_update(int dtype, int64_t n, double alpha, void *x, const void *p, void *r,
+                          const void *Ap, void *stream) {
+    if (!x || !p || !r || !Ap || n < 0 || (dtype != SIPG_F64 && dtype != SIPG_F32)) return SIPG_ERR_INVALID;
+    return launch_alpha_update(dtype, alpha, x, p, r, Ap, n, (hipStream_t)stream);
 }
 
 int sipg_forcing_draw(uint64_t seed, int kmin, int kmax, int mmax, int *nmodes, double *a, int *m,
@@ -396,7 +310,7 @@ int sipg_time_apply(sipg_op *op, const void *x, void *y, int reps, double *ms_pe
     SIPG_HIP_TRY(hipEventRecord(a, op->stream));
     for (int i = 0; i < reps && rc == SIPG_OK; ++i)
         rc = launch_apply(op->dtype, op->p, x, y, op->d_tab, op->dimx, op->dimy, op->pen, nullptr,
-                          nullptr, op->stream);
+                          nullptr, 0, op->stream);
     SIPG_HIP_TRY(hipEventRecord(b, op->stream));
     SIPG_HIP_TRY(hipEventSynchronize(b));
     float ms = 0.f;

@@ -75,7 +75,7 @@ template <typename T, int P, int BX, int BY, bool DOT>
 __global__ __launch_bounds__(BX * BY * (P + 1))
 void apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ tab,
                   int dimx, int dimy, int nbx, int nblk, T pen,
-                  double *__restrict__ partials, const cg_state *__restrict__ st)
+                  double *__restrict__ partials, const cg_state *__restrict__ st, int slot)
 {
     constexpr int N1 = P + 1;
     constexpr int NQ = N1 * N1;
@@ -90,7 +90,7 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restric
     __shared__ double red[(NT + 63) / 64];
 
     if (DOT) {
-        if (st->done) return;
+        if (st->s[slot].done | st->err) return;
     }
 
     const T *w = tab;
@@ -252,7 +252,7 @@ namespace {
 
 template <typename T, int P>
 int launch_one(const void *x, void *y, const void *tab, int dimx, int dimy, double pen,
-               double *partials, const cg_state *st, hipStream_t s)
+               double *partials, const cg_state *st, int slot, hipStream_t s)
 {
     using S = ApplyShape<T, P>;
     constexpr int BX = S::BX, BY = S::BY;
@@ -262,11 +262,11 @@ int launch_one(const void *x, void *y, const void *tab, int dimx, int dimy, doub
     if (partials) {
         hipLaunchKernelGGL((apply_kernel<T, P, BX, BY, true>), grid, block, 0, s,
                            (const T *)x, (T *)y, (const T *)tab, dimx, dimy, nbx, nblk, (T)pen,
-                           partials, st);
+                           partials, st, slot);
     } else {
         hipLaunchKernelGGL((apply_kernel<T, P, BX, BY, false>), grid, block, 0, s,
                            (const T *)x, (T *)y, (const T *)tab, dimx, dimy, nbx, nblk, (T)pen,
-                           partials, st);
+                           partials, st, slot);
     }
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
@@ -284,16 +284,16 @@ struct Dispatch;
 template <typename T>
 struct Dispatch<T> {
     static int launch(int, const void *, void *, const void *, int, int, double, double *,
-                      const cg_state *, hipStream_t) { return SIPG_ERR_UNSUPPORTED; }
+                      const cg_state *, int, hipStream_t) { return SIPG_ERR_UNSUPPORTED; }
     static int nblocks(int, int, int) { return -1; }
 };
 
 template <typename T, int P0, int... Ps>
 struct Dispatch<T, P0, Ps...> {
     static int launch(int p, const void *x, void *y, const void *tab, int dimx, int dimy,
-                      double pen, double *partials, const cg_state *st, hipStream_t s) {
-        if (p == P0) return launch_one<T, P0>(x, y, tab, dimx, dimy, pen, partials, st, s);
-        return Dispatch<T, Ps...>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, s);
+                      double pen, double *partials, const cg_state *st, int slot, hipStream_t s) {
+        if (p == P0) return launch_one<T, P0>(x, y, tab, dimx, dimy, pen, partials, st, slot, s);
+        return Dispatch<T, Ps...>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, slot, s);
     }
     static int nblocks(int p, int dimx, int dimy) {
         if (p == P0) return nblocks_one<T, P0>(dimx, dimy);
@@ -312,11 +312,11 @@ int apply_num_blocks(int dtype, int p, int dimx, int dimy) {
 }
 
 int launch_apply(int dtype, int p, const void *x, void *y, const void *tab, int dimx, int dimy,
-                 double pen, double *partials, const cg_state *st, hipStream_t s)
+                 double pen, double *partials, const cg_state *st, int slot, hipStream_t s)
 {
     if (dtype == SIPG_F64)
-        return AllDegrees<double>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, s);
-    return AllDegrees<float>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, s);
+        return AllDegrees<double>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, slot, s);
+    return AllDegrees<float>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, slot, s);
 }
 
 }  // namespace sipg_internal

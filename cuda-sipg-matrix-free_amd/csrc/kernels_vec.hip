@@ -97,30 +97,8 @@ __global__ __launch_bounds__(NT) void alpha_kernel(T alpha, T *__restrict__ x, c
     }
 }
 
-// r = b - r (r holds A x on entry), p = r, block sums of r.r and b.b
-// (conjugate_gradient_gpu.hpp:42-60).
-template <typename T>
-__global__ __launch_bounds__(NT) void cg_init_kernel(const T *__restrict__ b, T *__restrict__ r,
-                                                     T *__restrict__ p, int64_t n,
-                                                     double *__restrict__ prr,
-                                                     double *__restrict__ pbb) {
-    double srr = 0.0, sbb = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-        const T bi = b[i];
-        const T ri = bi + T(-1) * r[i];
-        r[i] = ri;
-        p[i] = ri;
-        srr += (double)ri * (double)ri;
-        sbb += (double)bi * (double)bi;
-    }
-    srr = block_sum_all(srr);
-    sbb = block_sum_all(sbb);
-    if (threadIdx.x == 0) {
-        prr[blockIdx.x] = srr;
-        pbb[blockIdx.x] = sbb;
-    }
-}
-
+// Reads the initial r.r and b.b block sums (from true_residual_kernel, the
+// r = b - A x, p = r step of conjugate_gradient_gpu.hpp:40-60) into slot 0.
 __global__ __launch_bounds__(NT) void cg_init_finalize_kernel(const double *__restrict__ prr,
                                                               const double *__restrict__ pbb,
                                                               int np, cg_state *st, double tol,
@@ -129,37 +107,43 @@ __global__ __launch_bounds__(NT) void cg_init_finalize_kernel(const double *__re
     const double bb = reduce_partials_dev(pbb, np);
     if (threadIdx.x == 0) {
         const double stop = tol_mode == SIPG_TOL_REL ? tol * sqrt(bb) : tol;
-        st->s[0].rr = rr;
-        st->s[0].alpha = st->s[0].beta = st->s[0].pAp = 0.0;
-        st->s[1] = st->s[0];
+        cg_slot s0;
+        s0.rr = rr;
+        s0.alpha = s0.beta = s0.pAp = 0.0;
+        s0.it = 0;
+        s0.done = (sqrt(rr) > stop && maxit > 0) ? 0 : 1;   // while-condition of :69
+        s0.checked = 1;                                     // r is the true residual here
+        s0.pad = 0;
+        st->s[0] = s0;
+        st->s[1] = s0;
         st->rr0 = rr;
+        st->rr_ref = rr;
         st->bb = bb;
         st->stop = stop;
-        st->it = 0;
         st->maxit = maxit;
-        int done = (sqrt(rr) > stop && maxit > 0) ? 0 : 1;   // while-condition of :69
-        if (!isfinite(rr) || !isfinite(bb)) done = 3;
-        st->done = done;
+        st->nrep = 0;
+        st->err = (!isfinite(rr) || !isfinite(bb)) ? 3 : 0;
+        st->restart_p = 0;
     }
 }
 
 // x += alpha p; r -= alpha Ap with alpha = r.r / p.Ap from the apply's
-// block sums; writes block sums of the new r.r.
+// block sums (alpha_kernel + the host-side alpha of
+// conjugate_gradient_gpu.hpp:75-83); writes block sums of the new r.r.
 template <typename T>
 __global__ __launch_bounds__(NT) void cg_update_xr_kernel(T *__restrict__ x, const T *__restrict__ p,
                                                           T *__restrict__ r, const T *__restrict__ Ap,
                                                           int64_t n, const double *__restrict__ ppAp,
                                                           int np, double *__restrict__ prr,
                                                           cg_state *st, int slot) {
-    if (st->done) return;
+    if (st->s[slot].done | st->err) return;
     const double pAp = reduce_partials_dev(ppAp, np);
     const double rr = st->s[slot].rr;
     const double alpha_d = rr / pAp;
     if (!(pAp > 0.0) || !isfinite(alpha_d)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            st->s[slot].pAp = pAp;
-            st->done = isfinite(pAp) ? 2 : 3;
-        }
+        // every block takes this branch (same pAp); block 0 records it in a
+        // word no kernel of this iteration reads at entry
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->err = isfinite(pAp) ? 2 : 3;
         return;
     }
     const T alpha = (T)alpha_d;
@@ -171,37 +155,112 @@ __global__ __launch_bounds__(NT) void cg_update_xr_kernel(T *__restrict__ x, con
         s += (double)ri * (double)ri;
     }
     s = block_sum_all(s);
-    if (threadIdx.x == 0) {
-        prr[blockIdx.x] = s;
-        if (blockIdx.x == 0) {
-            st->s[slot].pAp = pAp;
-            st->s[slot].alpha = alpha_d;
-        }
-    }
+    if (threadIdx.x == 0) prr[blockIdx.x] = s;
 }
 
-// p = r + beta p, beta = r.r(new) / r.r(old); block 0 advances the state.
+// p = r + beta p, beta = r.r(new) / r.r(old) (beta_kernel and
+// conjugate_gradient_gpu.hpp:85-95); block 0 writes the next slot.
 template <typename T>
 __global__ __launch_bounds__(NT) void cg_update_p_kernel(const T *__restrict__ r, T *__restrict__ p,
                                                          int64_t n, const double *__restrict__ prr,
                                                          int np, cg_state *st, int slot) {
-    if (st->done) return;
+    const cg_slot cur = st->s[slot];
+    if (cur.done | st->err) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->s[slot ^ 1] = cur;   // carry forward
+        return;
+    }
     const double rr_new = reduce_partials_dev(prr, np);
-    const double rr = st->s[slot].rr;
-    const double beta_d = rr_new / rr;
+    const double beta_d = rr_new / cur.rr;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->s[slot].beta = beta_d;
-        st->s[slot ^ 1].rr = rr_new;
-        const int it = st->it + 1;
-        st->it = it;
-        int done = 0;
-        if (!(sqrt(rr_new) > st->stop) || it >= st->maxit) done = 1;
-        if (!isfinite(rr_new)) done = 3;
-        st->done = done;
+        cg_slot nx;
+        nx.rr = rr_new;
+        nx.alpha = 0.0;
+        nx.beta = beta_d;
+        nx.pAp = 0.0;
+        nx.it = cur.it + 1;
+        nx.done = (!(sqrt(rr_new) > st->stop) || nx.it >= st->maxit) ? 1 : 0;
+        nx.checked = 0;
+        nx.pad = 0;
+        st->s[slot ^ 1] = nx;
+        if (!isfinite(rr_new)) st->err = 3;
     }
     const T beta = (T)beta_d;
     for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
         p[i] = r[i] + beta * p[i];
+}
+
+// out = (TO)(b - Ax); optional p = out; block sums of (b-Ax)^2 and b^2.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(NT) void true_residual_kernel(const TI *__restrict__ b,
+                                                           const TI *__restrict__ Ax,
+                                                           TO *__restrict__ out, TO *__restrict__ pc,
+                                                           int64_t n, double *__restrict__ prr,
+                                                           double *__restrict__ pbb) {
+    double srr = 0.0, sbb = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        const TI bi_ = b[i];
+        const TI ri_ = bi_ - Ax[i];
+        const double bi = (double)bi_;
+        const double ri = (double)ri_;
+        const TO ro = (TO)ri;
+        out[i] = ro;
+        if (pc) pc[i] = ro;
+        srr += ri * ri;
+        sbb += bi * bi;
+    }
+    srr = block_sum_all(srr);
+    if (pbb) sbb = block_sum_all(sbb);
+    if (threadIdx.x == 0) {
+        prr[blockIdx.x] = srr;
+        if (pbb) pbb[blockIdx.x] = sbb;
+    }
+}
+
+template <typename TO, typename TL>
+__global__ __launch_bounds__(NT) void accumulate_kernel(TO *__restrict__ x, TL *__restrict__ xl,
+                                                        int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        x[i] += (TO)xl[i];
+        xl[i] = TL(0);
+    }
+}
+
+// After r has been replaced by the true residual b - A x: put its r.r in the
+// slot the next iteration reads, then either confirm the finish (tolerance
+// met, or out of iterations) or resume a recursion that stopped early.  The
+// old search direction is kept only while the true residual agrees with the
+// recursion (within 2x): a larger gap, or a recursion that had already
+// converged (its direction is rounding noise by then), continues from
+// p = r instead -- keeping p there makes the iteration diverge.
+__global__ __launch_bounds__(NT) void replace_finalize_kernel(const double *__restrict__ prr, int np,
+                                                              cg_state *st, int slot) {
+    if (st->err) return;
+    const double rr = reduce_partials_dev(prr, np);
+    if (threadIdx.x == 0) {
+        cg_slot c = st->s[slot];
+        st->restart_p = (rr > 4.0 * c.rr || c.done) ? 1 : 0;
+        c.rr = rr;
+        st->rr_ref = rr;
+        st->nrep += 1;
+        if (!isfinite(rr)) {
+            st->err = 3;
+        } else if (!(sqrt(rr) > st->stop) || c.it >= st->maxit) {
+            c.done = 1;
+            c.checked = 1;
+        } else {
+            c.done = 0;
+            c.checked = 0;
+        }
+        st->s[slot] = c;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void restart_p_kernel(const T *__restrict__ r, T *__restrict__ p,
+                                                       int64_t n, const cg_state *st) {
+    if (!st->restart_p || st->err) return;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        p[i] = r[i];
 }
 
 template <typename T>
@@ -307,15 +366,6 @@ int launch_alpha_update(int dtype, double alpha, void *x, const void *p, void *r
     return SIPG_OK;
 }
 
-int launch_cg_init_residual(int dtype, const void *b, void *r, void *p, int64_t n, double *part_rr,
-                            double *part_bb, int nblk, hipStream_t s) {
-    DISPATCH_T(dtype,
-        hipLaunchKernelGGL(cg_init_kernel<double>, dim3(nblk), dim3(NT), 0, s, (const double *)b, (double *)r, (double *)p, n, part_rr, part_bb),
-        hipLaunchKernelGGL(cg_init_kernel<float>, dim3(nblk), dim3(NT), 0, s, (const float *)b, (float *)r, (float *)p, n, part_rr, part_bb));
-    SIPG_HIP_TRY(hipGetLastError());
-    return SIPG_OK;
-}
-
 int launch_cg_init_finalize(const double *part_rr, const double *part_bb, int np, cg_state *st,
                             double tol, int tol_mode, int maxit, hipStream_t s) {
     hipLaunchKernelGGL(cg_init_finalize_kernel, dim3(1), dim3(NT), 0, s, part_rr, part_bb, np, st, tol,
@@ -348,6 +398,49 @@ int launch_residual(int dtype, const void *b, const void *Ax, void *r, int64_t n
     DISPATCH_T(dtype,
         hipLaunchKernelGGL(residual_kernel<double>, dim3(nb), dim3(NT), 0, s, (const double *)b, (const double *)Ax, (double *)r, n),
         hipLaunchKernelGGL(residual_kernel<float>, dim3(nb), dim3(NT), 0, s, (const float *)b, (const float *)Ax, (float *)r, n));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_true_residual(int in_dtype, int out_dtype, const void *b, const void *Ax, void *out,
+                         void *p_copy, int64_t n, double *prr, double *pbb, int nblk, hipStream_t s) {
+    if (in_dtype == SIPG_F64 && out_dtype == SIPG_F64)
+        hipLaunchKernelGGL((true_residual_kernel<double, double>), dim3(nblk), dim3(NT), 0, s, (const double *)b, (const double *)Ax, (double *)out, (double *)p_copy, n, prr, pbb);
+    else if (in_dtype == SIPG_F64 && out_dtype == SIPG_F32)
+        hipLaunchKernelGGL((true_residual_kernel<double, float>), dim3(nblk), dim3(NT), 0, s, (const double *)b, (const double *)Ax, (float *)out, (float *)p_copy, n, prr, pbb);
+    else if (in_dtype == SIPG_F32 && out_dtype == SIPG_F32)
+        hipLaunchKernelGGL((true_residual_kernel<float, float>), dim3(nblk), dim3(NT), 0, s, (const float *)b, (const float *)Ax, (float *)out, (float *)p_copy, n, prr, pbb);
+    else
+        return SIPG_ERR_INVALID;
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    if (x_dtype == SIPG_F64 && xl_dtype == SIPG_F64)
+        hipLaunchKernelGGL((accumulate_kernel<double, double>), dim3(nb), dim3(NT), 0, s, (double *)x, (double *)xl, n);
+    else if (x_dtype == SIPG_F64 && xl_dtype == SIPG_F32)
+        hipLaunchKernelGGL((accumulate_kernel<double, float>), dim3(nb), dim3(NT), 0, s, (double *)x, (float *)xl, n);
+    else if (x_dtype == SIPG_F32 && xl_dtype == SIPG_F32)
+        hipLaunchKernelGGL((accumulate_kernel<float, float>), dim3(nb), dim3(NT), 0, s, (float *)x, (float *)xl, n);
+    else
+        return SIPG_ERR_INVALID;
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_replace_finalize(const double *prr, int np, cg_state *st, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(replace_finalize_kernel, dim3(1), dim3(NT), 0, s, prr, np, st, slot);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_restart_p(int dtype, const void *r, void *p, int64_t n, const cg_state *st, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(restart_p_kernel<double>, dim3(nb), dim3(NT), 0, s, (const double *)r, (double *)p, n, st),
+        hipLaunchKernelGGL(restart_p_kernel<float>, dim3(nb), dim3(NT), 0, s, (const float *)r, (float *)p, n, st));
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

@@ -43,23 +43,31 @@ inline int table_len(int p) { return (p + 1) * (p + 1) + 4 * (p + 1); }
 void build_tables(int p, std::vector<double> &tab);
 
 // ---- CG scalar state, resident on the device ------------------------------
-// Two slots alternate between iterations so that a kernel never reads a
-// scalar that a block of the same kernel writes.
+// The iteration launched k-th (counted by the host) reads slot k%2 and its
+// last kernel writes slot (k+1)%2, so no kernel reads a word that a block of
+// the same launch writes.  An iteration that finds its slot finished is a
+// no-op that only carries the slot forward; host-side launch parity and the
+// slots therefore always agree.
 struct cg_slot {
-    double rr;     // r.r entering the iteration
-    double alpha;
+    double rr;      // r.r entering the iteration
+    double alpha;   // of the iteration that produced this slot
     double beta;
     double pAp;
+    int it;         // iterations completed
+    int done;       // 0 running, 1 finished (tolerance met or iteration cap)
+    int checked;    // 1 when `done` was confirmed by a true residual b - A x
+    int pad;
 };
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b
     double stop;     // stopping threshold on sqrt(r.r)
     double rr0;      // initial r.r
-    int it;          // iterations done
+    double rr_ref;   // r.r of the last true residual
     int maxit;
-    int done;        // 0 running, 1 finished (converged or cap), 2 breakdown, 3 non-finite
-    int pad;
+    int err;         // 0, 2 breakdown (p.Ap <= 0), 3 non-finite -- terminal
+    int nrep;        // true-residual recomputations so far
+    int restart_p;   // set by a true residual that disagrees with the recursion
 };
 
 // Reduction partial buffers hold at most this many block sums.
@@ -70,9 +78,9 @@ constexpr int kVecThreads = 256;
 // Number of thread blocks the fused apply uses for a mesh (= partial count).
 int apply_num_blocks(int dtype, int p, int dimx, int dimy);
 // y = A x; with partials != nullptr also writes per-block sums of x.y and
-// returns early when st->done is set.
+// returns early when st->s[slot].done is set.
 int launch_apply(int dtype, int p, const void *x, void *y, const void *tab, int dimx, int dimy,
-                 double pen, double *partials, const cg_state *st, hipStream_t s);
+                 double pen, double *partials, const cg_state *st, int slot, hipStream_t s);
 
 // ---- launchers (kernels_vec.hip) ------------------------------------------
 int vec_num_blocks(int64_t n);
@@ -84,8 +92,6 @@ int launch_xpby(int dtype, const void *r, double beta, void *p, int64_t n, hipSt
 int launch_alpha_update(int dtype, double alpha, void *x, const void *p, void *r, const void *Ap,
                         int64_t n, hipStream_t s);
 // CG pieces with device-resident scalars
-int launch_cg_init_residual(int dtype, const void *b, void *r, void *p, int64_t n,
-                            double *part_rr, double *part_bb, int nblk, hipStream_t s);
 int launch_cg_init_finalize(const double *part_rr, const double *part_bb, int np, cg_state *st,
                             double tol, int tol_mode, int maxit, hipStream_t s);
 int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *Ap, int64_t n,
@@ -94,6 +100,17 @@ int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *
 int launch_cg_update_p(int dtype, const void *r, void *p, int64_t n, const double *part_rr,
                        int np_rr, int nblk, cg_state *st, int slot, hipStream_t s);
 int launch_residual(int dtype, const void *b, const void *Ax, void *r, int64_t n, hipStream_t s);
+// out = (out dtype)(b - Ax) with b, Ax in in_dtype; optionally p = out;
+// block sums of (b-Ax).(b-Ax) and, when pbb != nullptr, of b.b.
+int launch_true_residual(int in_dtype, int out_dtype, const void *b, const void *Ax, void *out,
+                         void *p_copy, int64_t n, double *prr, double *pbb, int nblk, hipStream_t s);
+// x += xl; xl = 0   (x in x_dtype, xl in xl_dtype; f64+=f64, f64+=f32, f32+=f32)
+int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t s);
+// Folds the true-residual block sums into the state: records r.r in the
+// live slot, confirms convergence or resumes the iteration.
+int launch_replace_finalize(const double *prr, int np, cg_state *st, int slot, hipStream_t s);
+// p = r when the last true residual asked for a restart (st->restart_p)
+int launch_restart_p(int dtype, const void *r, void *p, int64_t n, const cg_state *st, hipStream_t s);
 int launch_convert(int from_dtype, int to_dtype, const void *src, void *dst, int64_t n,
                    hipStream_t s);
 int launch_rhs_fourier(int dtype, int p, int dim, const void *tab_f64, int nmodes,
@@ -115,7 +132,26 @@ struct sipg_op {
     double *d_scalar = nullptr;       // small device scalars
     sipg_internal::cg_state *d_state = nullptr;
     sipg_internal::cg_state *h_state = nullptr;   // pinned mirror
-    // cached CG workspace (r, p, Ap) in the op's dtype, allocated on first use
+    // cached CG workspace, allocated on first use
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    // FP32 twin for the mixed-precision solver (owned)
+    sipg_op *twin = nullptr;
 };
+
+namespace sipg_internal {
+// ---- shared host helpers (abi.cpp) ----------------------------------------
+inline size_t dtype_size(int dtype) { return dtype == SIPG_F64 ? 8 : 4; }
+inline bool overlaps(const void *a, const void *b, size_t bytes) {
+    const char *pa = (const char *)a, *pb = (const char *)b;
+    return pa < pb + bytes && pb < pa + bytes;
+}
+// grows op->ws to at least `bytes` (contents not kept)
+int ensure_ws(sipg_op *op, size_t bytes);
+// Partial-buffer layout inside op->d_part
+struct parts {
+    double *pAp, *rr, *bb, *misc;
+    int napply;
+};
+parts part_layout(sipg_op *op);
+}  // namespace sipg_internal

@@ -1,0 +1,309 @@
+// solvers.cpp -- device-resident conjugate gradient drivers.
+//
+// Reference: conjugate_gradient (conjugate_gradient_gpu.hpp:18-109) runs the
+// loop on the host: two blocking cuBLAS dots per iteration, alpha and beta
+// computed on the host, five launches.  Here the scalars live in a
+// cg_state on the device (kernels_vec.hip), an iteration is three
+// launches (fused apply + p.Ap, x/r update + r.r, p update), and the host
+// only enqueues chunks of iterations and polls a copy of the state one chunk
+// behind, so the GPU queue never drains while the host decides.
+//
+// One engine serves three entry points:
+//   sipg_cg        the reference's semantics: recursive residual only;
+//   sipg_cg_ex     + residual replacement / true-residual finish (FP64 or
+//                  FP32 op);
+//   sipg_cg_mixed  FP32 iteration with reliable updates into an FP64
+//                  solution and an FP64 true residual.
+#include "sipg_internal.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+using namespace sipg_internal;
+
+namespace {
+
+struct Engine {
+    // inner iteration
+    sipg_op *opI = nullptr;          // op of the iteration precision
+    void *xI = nullptr;              // iterate (x itself, or the FP32 correction x^)
+    void *r = nullptr, *p = nullptr, *Ap = nullptr;
+    // true residual (outer precision)
+    sipg_op *opO = nullptr;          // op used for b - A x (FP64 op in mixed mode)
+    void *xO = nullptr;              // solution vector in outer precision
+    const void *b = nullptr;         // right-hand side in outer precision
+    void *AxO = nullptr;             // scratch for A x in outer precision
+    bool mixed = false;              // fold xI into xO at each true residual
+    bool restart = false;            // p = r at each true residual
+    double replace_factor = 0.0;
+    bool true_stop = false;
+    int maxit = 0;
+
+    hipStream_t s = nullptr;
+    cg_state *st = nullptr;
+    cg_state *hst = nullptr;         // pinned, 2 entries
+    parts P{};
+    int nb = 0;
+    bool fold = false;
+    int64_t launched = 0;
+
+    int init(double tol, int tol_mode) {
+        int rc;
+        // r = b - A x ; p = r ; r.r ; b.b    (conjugate_gradient_gpu.hpp:40-60)
+        if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
+                               opO->pen, nullptr, nullptr, 0, s)))
+            return rc;
+        if (mixed) SIPG_HIP_TRY(hipMemsetAsync(xI, 0, (size_t)opI->n * dtype_size(opI->dtype), s));
+        if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, p, opI->n, P.rr, P.bb, nb, s)))
+            return rc;
+        return launch_cg_init_finalize(P.rr, P.bb, nb, st, tol, tol_mode, maxit, s);
+    }
+
+    int iteration() {
+        const int slot = (int)(launched & 1);
+        int rc;
+        if ((rc = launch_apply(opI->dtype, opI->p, p, Ap, opI->d_tab, opI->dimx, opI->dimy, opI->pen,
+                               P.pAp, st, slot, s)))
+            return rc;
+        const double *src = P.pAp;
+        int np = P.napply;
+        if (fold) {
+            if ((rc = launch_reduce_partials(P.pAp, P.napply, P.misc, s))) return rc;
+            src = P.misc;
+            np = 1;
+        }
+        if ((rc = launch_cg_update_xr(opI->dtype, xI, p, r, Ap, opI->n, src, np, P.rr, nb, st, slot, s)))
+            return rc;
+        if ((rc = launch_cg_update_p(opI->dtype, r, p, opI->n, P.rr, nb, nb, st, slot, s))) return rc;
+        ++launched;
+        return SIPG_OK;
+    }
+
+    // r <- b - A x (true residual) for the slot the next iteration reads
+    int replacement() {
+        const int slot = (int)(launched & 1);
+        int rc;
+        if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s))) return rc;
+        if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
+                               opO->pen, nullptr, nullptr, 0, s)))
+            return rc;
+        if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, restart ? p : nullptr,
+                                       opI->n, P.rr, nullptr, nb, s)))
+            return rc;
+        if ((rc = launch_replace_finalize(P.rr, nb, st, slot, s))) return rc;
+        return restart ? SIPG_OK : launch_restart_p(opI->dtype, r, p, opI->n, st, s);
+    }
+
+    int run(sipg_cg_info *info) {
+        hipEvent_t ev[2];
+        SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        int rc = SIPG_OK;
+        // state after init
+        SIPG_HIP_TRY(hipMemcpyAsync(&hst[0], st, sizeof(cg_state), hipMemcpyDeviceToHost, s));
+        SIPG_HIP_TRY(hipStreamSynchronize(s));
+        cg_state cur = hst[0];
+        int64_t cur_launched = 0;          // launches reflected in `cur`
+        int nrep_enq = 0;
+        int chunk = 4;
+        int inflight = 0, buf = 0;
+        int64_t pending_launched[2] = {0, 0};
+        // Each pass: decide on the newest polled state `cur`, enqueue one
+        // chunk of iterations and a state copy, and block on the older of
+        // two outstanding copies.  Iterations past the finish are no-ops on
+        // the device, so running one or two chunks ahead costs little.
+        for (;;) {
+            const cg_slot cs = cur.s[cur_launched & 1];
+            if (cur.err) break;
+            const bool rep_visible = cur.nrep == nrep_enq;
+            if (rep_visible && cs.done && (cs.checked || !true_stop)) break;   // final
+            bool want_rep = false;
+            if (rep_visible) {
+                if (true_stop && cs.done && !cs.checked) want_rep = true;
+                if (!cs.done && replace_factor > 0.0 &&
+                    cs.rr <= replace_factor * replace_factor * cur.rr_ref)
+                    want_rep = true;
+                // divergence guard: a recursion that grew 100x past the last
+                // true residual is recomputed (and restarted, see finalize)
+                if (!cs.done && (replace_factor > 0.0 || true_stop) && cs.rr > 1e4 * cur.rr_ref)
+                    want_rep = true;
+            }
+            if (want_rep) {
+                if ((rc = replacement())) break;
+                ++nrep_enq;
+            }
+            for (int c = 0; c < chunk; ++c)
+                if ((rc = iteration())) break;
+            if (rc) break;
+            if (chunk < 64) chunk *= 2;
+            SIPG_HIP_TRY(hipMemcpyAsync(&hst[buf], st, sizeof(cg_state), hipMemcpyDeviceToHost, s));
+            SIPG_HIP_TRY(hipEventRecord(ev[buf], s));
+            pending_launched[buf] = launched;
+            ++inflight;
+            buf ^= 1;
+            if (inflight == 2) {
+                // the older copy sits in `buf` after the flip
+                SIPG_HIP_TRY(hipEventSynchronize(ev[buf]));
+                cur = hst[buf];
+                cur_launched = pending_launched[buf];
+                --inflight;
+            }
+        }
+        // fold whatever the last cycle accumulated (a no-op after a final
+        // true residual, which already folded it)
+        if (!rc && mixed) rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s);
+        SIPG_HIP_TRY(hipStreamSynchronize(s));
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        if (rc) return rc;
+        SIPG_HIP_TRY(hipMemcpy(&hst[0], st, sizeof(cg_state), hipMemcpyDeviceToHost));
+        const cg_state &f = hst[0];
+        const cg_slot &fs = f.s[launched & 1];
+        int status = SIPG_OK;
+        if (f.err == 2) status = SIPG_ERR_BREAKDOWN;
+        else if (f.err == 3) status = SIPG_ERR_NONFINITE;
+        else if (!(std::sqrt(fs.rr) <= f.stop)) status = SIPG_NOT_CONVERGED;
+        if (info) {
+            info->status = status;
+            info->iters = fs.it;
+            info->replacements = f.nrep;
+            info->res_norm = std::sqrt(fs.rr);
+            info->true_res_norm = f.nrep > 0 || fs.it == 0 ? std::sqrt(f.rr_ref) : -1.0;
+            info->b_norm = std::sqrt(f.bb);
+        }
+        return status;
+    }
+};
+
+int check_opts(const sipg_cg_opts *o) {
+    if (!o) return SIPG_ERR_INVALID;
+    if (!(o->tol >= 0.0) || (o->tol_mode != SIPG_TOL_ABS && o->tol_mode != SIPG_TOL_REL))
+        return SIPG_ERR_INVALID;
+    if (!(o->replace_factor >= 0.0 && o->replace_factor < 1.0)) return SIPG_ERR_INVALID;
+    return SIPG_OK;
+}
+
+int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg_cg_info *info) {
+    int rc = check_opts(o);
+    if (rc) return rc;
+    const size_t vb = (size_t)op->n * dtype_size(op->dtype);
+    if (overlaps(x, b, vb)) return SIPG_ERR_ALIAS;
+    // With true residuals the iterate is group-updated (van der Vorst & Ye
+    // 2000): the CG increments go to a separate vector z that is folded
+    // into x only when r is recomputed, so the rounding of x += alpha p
+    // does not accumulate at the scale of |x| between recomputations.
+    const bool group = o->replace_factor > 0.0 || o->true_residual_stop != 0;
+    if ((rc = ensure_ws(op, (group ? 4 : 3) * vb))) return rc;
+    char *ws = (char *)op->ws;
+    Engine E;
+    E.opI = E.opO = op;
+    E.xO = x;
+    E.xI = group ? (void *)(ws + 3 * vb) : x;
+    E.mixed = group;
+    E.r = ws;
+    E.p = ws + vb;
+    E.Ap = ws + 2 * vb;
+    E.AxO = E.Ap;        // the initial and the replacement A x use the Ap slot
+    E.b = b;
+    E.replace_factor = o->replace_factor;
+    E.true_stop = o->true_residual_stop != 0;
+    E.restart = o->restart != 0;
+    E.maxit = (int)std::min<int64_t>(o->maxit > 0 ? (int64_t)o->maxit : op->n, INT_MAX);
+    E.s = op->stream;
+    E.st = op->d_state;
+    E.hst = op->h_state;
+    E.P = part_layout(op);
+    E.nb = vec_num_blocks(op->n);
+    E.fold = E.P.napply > kMaxPartials;
+    if ((rc = E.init(o->tol, o->tol_mode))) return rc;
+    return E.run(info);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sipg_cg(sipg_op *op, void *x, const void *b, double tol, int tol_mode, int maxit, int *iters,
+            double *res_norm) {
+    if (!op || !x || !b) return SIPG_ERR_INVALID;
+    sipg_cg_opts o{};
+    o.tol = tol;
+    o.tol_mode = tol_mode;
+    o.maxit = maxit;
+    o.replace_factor = 0.0;
+    o.true_residual_stop = 0;
+    sipg_cg_info info{};
+    const int rc = solve_plain(op, x, b, &o, &info);
+    if (iters) *iters = info.iters;
+    if (res_norm) *res_norm = info.res_norm;
+    return rc;
+}
+
+int sipg_cg_ex(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {
+    if (!op || !x || !b) return SIPG_ERR_INVALID;
+    return solve_plain(op, x, b, opts, info);
+}
+
+int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {
+    if (!op || !x || !b || op->dtype != SIPG_F64) return SIPG_ERR_INVALID;
+    int rc = check_opts(opts);
+    if (rc) return rc;
+    if (overlaps(x, b, (size_t)op->n * 8)) return SIPG_ERR_ALIAS;
+    if (!op->twin) {
+        if ((rc = sipg_op_create(&op->twin, op->dimx, op->dimy, op->p, op->pen, SIPG_F32, op->stream)))
+            return rc;
+    }
+    sipg_op *tw = op->twin;
+    tw->stream = op->stream;
+    const size_t v32 = (size_t)op->n * 4, v64 = (size_t)op->n * 8;
+    if ((rc = ensure_ws(tw, 4 * v32))) return rc;       // r^, p^, A p^, x^
+    if ((rc = ensure_ws(op, v64))) return rc;           // A x in FP64
+    char *w = (char *)tw->ws;
+    Engine E;
+    E.opI = tw;
+    E.opO = op;
+    E.r = w;
+    E.p = w + v32;
+    E.Ap = w + 2 * v32;
+    E.xI = w + 3 * v32;
+    E.xO = x;
+    E.b = b;
+    E.AxO = op->ws;
+    E.mixed = true;
+    E.restart = opts->restart != 0;
+    E.replace_factor = opts->replace_factor > 0.0 ? opts->replace_factor : 0.01;
+    E.true_stop = true;
+    E.maxit = (int)std::min<int64_t>(opts->maxit > 0 ? (int64_t)opts->maxit : op->n, INT_MAX);
+    E.s = op->stream;
+    E.st = tw->d_state;
+    E.hst = tw->h_state;
+    E.P = part_layout(tw);
+    E.nb = vec_num_blocks(op->n);
+    E.fold = E.P.napply > kMaxPartials;
+    if ((rc = E.init(opts->tol, opts->tol_mode))) return rc;
+    return E.run(info);
+}
+
+int sipg_true_relres(sipg_op *op, const void *x, const void *b, double *relres) {
+    if (!op || !x || !b || !relres) return SIPG_ERR_INVALID;
+    if (op->dtype != SIPG_F64) return SIPG_ERR_INVALID;
+    const size_t vb = (size_t)op->n * 8;
+    int rc = ensure_ws(op, 3 * vb);
+    if (rc) return rc;
+    char *ws = (char *)op->ws;
+    void *Ax = ws + 2 * vb;
+    void *r = ws;
+    if ((rc = launch_apply(op->dtype, op->p, x, Ax, op->d_tab, op->dimx, op->dimy, op->pen, nullptr,
+                           nullptr, 0, op->stream)))
+        return rc;
+    if ((rc = launch_residual(op->dtype, b, Ax, r, op->n, op->stream))) return rc;
+    double rr = 0, bb = 0;
+    if ((rc = sipg_dot(op, r, r, &rr))) return rc;
+    if ((rc = sipg_dot(op, b, b, &bb))) return rc;
+    *relres = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    return SIPG_OK;
+}
+
+}  // extern "C"

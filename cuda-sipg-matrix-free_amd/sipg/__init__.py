@@ -40,6 +40,21 @@ class SipgError(RuntimeError):
         super().__init__(f"{what}: {strerror(status)} (status {status})")
 
 
+class CgOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("tol_mode", ctypes.c_int), ("maxit", ctypes.c_int),
+                ("replace_factor", ctypes.c_double), ("true_residual_stop", ctypes.c_int),
+                ("restart", ctypes.c_int)]
+
+
+class CgInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_int), ("replacements", ctypes.c_int),
+                ("pad", ctypes.c_int), ("res_norm", ctypes.c_double),
+                ("true_res_norm", ctypes.c_double), ("b_norm", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -54,6 +69,7 @@ def _declare(L):
     sig = {
         "sipg_strerror": (ctypes.c_char_p, [i]),
         "sipg_max_degree": (i, []),
+        "sipg_basis_tables": (i, [i, _dp, _dp, _dp, _dp]),
         "sipg_device_count": (i, [_ip]),
         "sipg_set_device": (i, [i]),
         "sipg_device_synchronize": (i, []),
@@ -75,6 +91,12 @@ def _declare(L):
         "sipg_alpha_update": (i, [_vp, d, _vp, _vp, _vp, _vp]),
         "sipg_cg": (i, [_vp, _vp, _vp, d, i, i, _ip, _dp]),
         "sipg_true_relres": (i, [_vp, _vp, _vp, _dp]),
+        "sipg_cg_ex": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
+        "sipg_cg_mixed": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
+        "sipg_vec_dot": (i, [i, i64, _vp, _vp, _dp, _vp]),
+        "sipg_vec_axpy": (i, [i, i64, d, _vp, _vp, _vp]),
+        "sipg_vec_xpby": (i, [i, i64, _vp, d, _vp, _vp]),
+        "sipg_vec_alpha_update": (i, [i, i64, d, _vp, _vp, _vp, _vp, _vp]),
         "sipg_forcing_draw": (i, [ctypes.c_uint64, i, i, i, _ip, _dp, _ip, _ip]),
         "sipg_rhs_fourier": (i, [_vp, i, _dp, _ip, _ip, _vp]),
         "sipg_time_apply": (i, [_vp, _vp, _vp, i, _dp]),
@@ -227,6 +249,19 @@ class Operator:
         st = lib().sipg_cg(self.h, x.ptr, b.ptr, tol, tol_mode, maxit, ctypes.byref(it), ctypes.byref(res))
         return st, it.value, res.value
 
+    def cg_ex(self, x, b, tol, tol_mode=TOL_REL, maxit=0, replace_factor=0.0, true_stop=True,
+              restart=False):
+        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1 if true_stop else 0, 1 if restart else 0)
+        info = CgInfo()
+        st = lib().sipg_cg_ex(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
+        return st, info
+
+    def cg_mixed(self, x, b, tol, tol_mode=TOL_REL, maxit=0, replace_factor=0.0, restart=True):
+        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1, 1 if restart else 0)
+        info = CgInfo()
+        st = lib().sipg_cg_mixed(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
+        return st, info
+
     def true_relres(self, x, b):
         r = ctypes.c_double()
         check(lib().sipg_true_relres(self.h, x.ptr, b.ptr, ctypes.byref(r)), "true_relres")
@@ -276,6 +311,16 @@ def forcing_draw(seed, kmin=4, kmax=4, mmax=4):
           "forcing_draw")
     k = nm.value
     return a[:k].copy(), m[:k].copy(), n[:k].copy()
+
+
+def basis_tables(p):
+    """Host-only LGL nodes/weights, D[mode, node] and K1 for degree p."""
+    n1 = p + 1
+    z, w = np.zeros(n1), np.zeros(n1)
+    D, K = np.zeros(n1 * n1), np.zeros(n1 * n1)
+    check(lib().sipg_basis_tables(p, z.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
+                                  D.ctypes.data_as(_dp), K.ctypes.data_as(_dp)), "basis_tables")
+    return z, w, D.reshape(n1, n1), K.reshape(n1, n1)
 
 
 def pen_sigma(p):

@@ -57,6 +57,13 @@ int sipg_device_count(int *count);
 int sipg_set_device(int device);
 int sipg_device_synchronize(void);
 
+/* Host-only basis tables for degree p (no device work): LGL nodes and
+ * weights (p+1 each, LGL_quadrature_table.hpp:84-146), D[mode*(p+1)+node]
+ * = phi'_mode(z_node) (sem_function.hpp:86-110) and the 1-D stiffness K1
+ * (laplacian_operator.hpp:31-43), (p+1)^2 each.  Any pointer may be NULL.
+ * Valid for 1 <= p <= 63. */
+int sipg_basis_tables(int p, double *nodes, double *weights, double *D, double *K1);
+
 /* ---- device memory (library allocator; plain HIP device memory) -------- */
 int sipg_malloc(void **ptr, size_t bytes);
 int sipg_free(void *ptr);
@@ -98,6 +105,15 @@ int sipg_xpby(sipg_op *op, const void *r, double beta, void *p);
 /* x += alpha p; r -= alpha Ap   (alpha_kernel, conjugate_gradient_kernels.hpp:5-23) */
 int sipg_alpha_update(sipg_op *op, double alpha, void *x, const void *p, void *r, const void *Ap);
 
+/* Operator-free forms of the same vector kernels, for callers that hold
+ * only raw vectors (e.g. a CG over a user-defined abs_mvm).  n elements of
+ * dtype; stream as in sipg_op_create. */
+int sipg_vec_dot(int dtype, int64_t n, const void *a, const void *b, double *result, void *stream);
+int sipg_vec_axpy(int dtype, int64_t n, double alpha, const void *x, void *y, void *stream);
+int sipg_vec_xpby(int dtype, int64_t n, const void *r, double beta, void *p, void *stream);
+int sipg_vec_alpha_update(int dtype, int64_t n, double alpha, void *x, const void *p, void *r,
+                          const void *Ap, void *stream);
+
 /* ---- CG solve ---------------------------------------------------------- */
 /* Unpreconditioned CG on A x = b with device-resident scalars.  Replaces
  * conjugate_gradient (conjugate_gradient_gpu.hpp:18-109).  x: initial
@@ -109,6 +125,51 @@ int sipg_alpha_update(sipg_op *op, double alpha, void *x, const void *p, void *r
  * SIPG_ERR_NONFINITE (x then holds the last iterate). */
 int sipg_cg(sipg_op *op, void *x, const void *b, double tol, int tol_mode, int maxit,
             int *iters, double *res_norm);
+
+/* Extended CG.  Same iteration as sipg_cg plus, optionally, residual
+ * replacement (van der Vorst & Ye 2000): whenever the recursive sqrt(r.r)
+ * has fallen by `replace_factor` since the last true residual, r is
+ * recomputed as b - A x and the iteration continues with the same search
+ * direction; with `true_residual_stop` the finish is confirmed on the true
+ * residual (a recursion that stopped early resumes).  This has no
+ * counterpart in the reference, whose CG stops on the recursive residual
+ * only; it is what makes a 1e-10 relative residual reachable at kappa ~ 1e8. */
+typedef struct {
+    double tol;              /* stopping tolerance                              */
+    int tol_mode;            /* SIPG_TOL_ABS or SIPG_TOL_REL                    */
+    int maxit;               /* <= 0: n                                         */
+    double replace_factor;   /* in (0,1): recompute r at each such drop; 0: off */
+    int true_residual_stop;  /* 1: confirm convergence on b - A x               */
+    int restart;             /* 1: also reset the search direction p = r at each
+                                true residual (iterative refinement: a fresh
+                                Krylov space per cycle); 0: keep p            */
+} sipg_cg_opts;
+
+typedef struct {
+    int status;              /* same codes as the return value                  */
+    int iters;               /* CG iterations                                   */
+    int replacements;        /* true-residual recomputations                    */
+    int pad;
+    double res_norm;         /* final sqrt(r.r) of the iterated residual        */
+    double true_res_norm;    /* ||b - A x|| at the last recomputation, or -1    */
+    double b_norm;           /* ||b||                                           */
+} sipg_cg_info;
+
+int sipg_cg_ex(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
+
+/* Mixed-precision solve (no counterpart in the reference): the CG
+ * iteration runs in FP32 (operator apply, vector updates, FP64-accumulated
+ * dots) on an FP32 twin of `op`; each time the FP32 residual has fallen by
+ * `replace_factor` (default 0.01 when 0 is passed) the FP32 iterate is
+ * folded into the FP64 solution and the true residual b - A x is
+ * recomputed in FP64 (defect correction).  With opts->restart = 1 each
+ * cycle restarts the FP32 CG from that residual (iterative refinement);
+ * with 0 the search direction is kept (a reliable update, Sleijpen & van der
+ * Vorst 1996; Clark et al. 2010), which is cheaper while eps32 * kappa(A)
+ * is small but loses convergence once it is not.  The final convergence
+ * test is on the FP64 true residual.  op must be SIPG_F64; x (in/out) and b
+ * are FP64 device vectors. */
+int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
 
 /* ||b - A x||_2 / ||b||_2 in FP64 (op must be SIPG_F64). */
 int sipg_true_relres(sipg_op *op, const void *x, const void *b, double *relres);

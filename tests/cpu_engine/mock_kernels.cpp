@@ -1,0 +1,254 @@
+// CPU stand-ins for libsipg_hip's kernel launchers, for the CPU test build
+// of the host orchestration (abi.cpp, solvers.cpp).  Each function mirrors
+// the semantics of the kernel it replaces (kernels_apply.hip,
+// kernels_vec.hip) as one "block": the state-machine rules (slots,
+// carry-forward, err, replacement finalize) are restated line for line so
+// the engine's protocol can be exercised without a GPU.  The operator apply
+// itself is the oracle's.  Test infrastructure only.
+#include "../../cuda-sipg-matrix-free_amd/csrc/sipg_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+extern "C" {
+void orc_apply(int dimx, int dimy, int p, double pen, const double *x, double *y);
+void orc_apply_f32(int dimx, int dimy, int p, float pen, const float *x, float *y);
+}
+
+namespace sipg_internal {
+
+static const int kMockApplyBlocks = 3;
+static const int kMockVecBlocks = 2;
+
+int apply_num_blocks(int, int p, int, int) { return p <= kMaxDegree ? kMockApplyBlocks : -1; }
+int vec_num_blocks(int64_t) { return kMockVecBlocks; }
+
+template <typename T> static T *tp(void *p) { return (T *)p; }
+template <typename T> static const T *tp(const void *p) { return (const T *)p; }
+
+static double dotv(int dtype, const void *a, const void *b, int64_t n) {
+    double s = 0;
+    for (int64_t i = 0; i < n; ++i)
+        s += dtype == SIPG_F64 ? tp<double>(a)[i] * tp<double>(b)[i]
+                               : (double)tp<float>(a)[i] * (double)tp<float>(b)[i];
+    return s;
+}
+
+static void put_partials(double *part, int np, double v) {
+    part[0] = v;
+    for (int i = 1; i < np; ++i) part[i] = 0.0;
+}
+
+static double sum_partials(const double *part, int np) {
+    double s = 0;
+    for (int i = 0; i < np; ++i) s += part[i];
+    return s;
+}
+
+int launch_apply(int dtype, int p, const void *x, void *y, const void *, int dimx, int dimy,
+                 double pen, double *partials, const cg_state *st, int slot, hipStream_t) {
+    if (partials && (st->s[slot].done | st->err)) return SIPG_OK;
+    const int64_t n = (int64_t)dimx * dimy * (p + 1) * (p + 1);
+    if (dtype == SIPG_F64) orc_apply(dimx, dimy, p, pen, tp<double>(x), tp<double>(y));
+    else orc_apply_f32(dimx, dimy, p, (float)pen, tp<float>(x), tp<float>(y));
+    if (partials) put_partials(partials, kMockApplyBlocks, dotv(dtype, x, y, n));
+    return SIPG_OK;
+}
+
+int launch_dot_partials(int dtype, const void *a, const void *b, int64_t n, double *partials, int nblk,
+                        hipStream_t) {
+    put_partials(partials, nblk, dotv(dtype, a, b, n));
+    return SIPG_OK;
+}
+
+int launch_reduce_partials(const double *partials, int np, double *out, hipStream_t) {
+    out[0] = sum_partials(partials, np);
+    return SIPG_OK;
+}
+
+template <typename T> static void axpy_t(T a, const T *x, T *y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) y[i] = y[i] + a * x[i];
+}
+int launch_axpy(int dtype, double alpha, const void *x, void *y, int64_t n, hipStream_t) {
+    if (dtype == SIPG_F64) axpy_t<double>(alpha, tp<double>(x), tp<double>(y), n);
+    else axpy_t<float>((float)alpha, tp<float>(x), tp<float>(y), n);
+    return SIPG_OK;
+}
+
+template <typename T> static void xpby_t(const T *r, T b, T *p, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) p[i] = r[i] + b * p[i];
+}
+int launch_xpby(int dtype, const void *r, double beta, void *p, int64_t n, hipStream_t) {
+    if (dtype == SIPG_F64) xpby_t<double>(tp<double>(r), beta, tp<double>(p), n);
+    else xpby_t<float>(tp<float>(r), (float)beta, tp<float>(p), n);
+    return SIPG_OK;
+}
+
+template <typename T> static void alpha_t(T a, T *x, const T *p, T *r, const T *Ap, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) { x[i] = x[i] + a * p[i]; r[i] = r[i] - a * Ap[i]; }
+}
+int launch_alpha_update(int dtype, double alpha, void *x, const void *p, void *r, const void *Ap,
+                        int64_t n, hipStream_t) {
+    if (dtype == SIPG_F64) alpha_t<double>(alpha, tp<double>(x), tp<double>(p), tp<double>(r), tp<double>(Ap), n);
+    else alpha_t<float>((float)alpha, tp<float>(x), tp<float>(p), tp<float>(r), tp<float>(Ap), n);
+    return SIPG_OK;
+}
+
+int launch_cg_init_finalize(const double *prr, const double *pbb, int np, cg_state *st, double tol,
+                            int tol_mode, int maxit, hipStream_t) {
+    const double rr = sum_partials(prr, np), bb = sum_partials(pbb, np);
+    const double stop = tol_mode == SIPG_TOL_REL ? tol * std::sqrt(bb) : tol;
+    cg_slot s0{};
+    s0.rr = rr;
+    s0.done = (std::sqrt(rr) > stop && maxit > 0) ? 0 : 1;
+    s0.checked = 1;
+    st->s[0] = s0;
+    st->s[1] = s0;
+    st->rr0 = rr;
+    st->rr_ref = rr;
+    st->bb = bb;
+    st->stop = stop;
+    st->maxit = maxit;
+    st->nrep = 0;
+    st->err = (!std::isfinite(rr) || !std::isfinite(bb)) ? 3 : 0;
+    st->restart_p = 0;
+    return SIPG_OK;
+}
+
+template <typename T>
+static void update_xr_t(T *x, const T *p, T *r, const T *Ap, int64_t n, const double *ppAp, int np,
+                        double *prr, int nblk, cg_state *st, int slot) {
+    if (st->s[slot].done | st->err) return;
+    const double pAp = sum_partials(ppAp, np);
+    const double rr = st->s[slot].rr;
+    const double alpha_d = rr / pAp;
+    if (!(pAp > 0.0) || !std::isfinite(alpha_d)) { st->err = std::isfinite(pAp) ? 2 : 3; return; }
+    const T a = (T)alpha_d;
+    double s = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = x[i] + a * p[i];
+        const T ri = r[i] - a * Ap[i];
+        r[i] = ri;
+        s += (double)ri * (double)ri;
+    }
+    put_partials(prr, nblk, s);
+}
+int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *Ap, int64_t n,
+                        const double *part_pAp, int np_pAp, double *part_rr, int nblk, cg_state *st,
+                        int slot, hipStream_t) {
+    if (dtype == SIPG_F64)
+        update_xr_t<double>(tp<double>(x), tp<double>(p), tp<double>(r), tp<double>(Ap), n, part_pAp, np_pAp, part_rr, nblk, st, slot);
+    else
+        update_xr_t<float>(tp<float>(x), tp<float>(p), tp<float>(r), tp<float>(Ap), n, part_pAp, np_pAp, part_rr, nblk, st, slot);
+    return SIPG_OK;
+}
+
+template <typename T>
+static void update_p_t(const T *r, T *p, int64_t n, const double *prr, int np, cg_state *st, int slot) {
+    const cg_slot cur = st->s[slot];
+    if (cur.done | st->err) { st->s[slot ^ 1] = cur; return; }
+    const double rr_new = sum_partials(prr, np);
+    const double beta_d = rr_new / cur.rr;
+    cg_slot nx{};
+    nx.rr = rr_new;
+    nx.beta = beta_d;
+    nx.it = cur.it + 1;
+    nx.done = (!(std::sqrt(rr_new) > st->stop) || nx.it >= st->maxit) ? 1 : 0;
+    st->s[slot ^ 1] = nx;
+    if (!std::isfinite(rr_new)) st->err = 3;
+    const T b = (T)beta_d;
+    for (int64_t i = 0; i < n; ++i) p[i] = r[i] + b * p[i];
+}
+int launch_cg_update_p(int dtype, const void *r, void *p, int64_t n, const double *part_rr, int np_rr,
+                       int, cg_state *st, int slot, hipStream_t) {
+    if (dtype == SIPG_F64) update_p_t<double>(tp<double>(r), tp<double>(p), n, part_rr, np_rr, st, slot);
+    else update_p_t<float>(tp<float>(r), tp<float>(p), n, part_rr, np_rr, st, slot);
+    return SIPG_OK;
+}
+
+int launch_residual(int dtype, const void *b, const void *Ax, void *r, int64_t n, hipStream_t) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (dtype == SIPG_F64) tp<double>(r)[i] = tp<double>(b)[i] - tp<double>(Ax)[i];
+        else tp<float>(r)[i] = tp<float>(b)[i] - tp<float>(Ax)[i];
+    }
+    return SIPG_OK;
+}
+
+template <typename TI, typename TO>
+static void true_res_t(const TI *b, const TI *Ax, TO *out, TO *pc, int64_t n, double *prr, double *pbb,
+                       int nblk) {
+    double srr = 0, sbb = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const TI ri_ = b[i] - Ax[i];
+        const double ri = (double)ri_, bi = (double)b[i];
+        const TO ro = (TO)ri;
+        out[i] = ro;
+        if (pc) pc[i] = ro;
+        srr += ri * ri;
+        sbb += bi * bi;
+    }
+    put_partials(prr, nblk, srr);
+    if (pbb) put_partials(pbb, nblk, sbb);
+}
+int launch_true_residual(int in_dtype, int out_dtype, const void *b, const void *Ax, void *out,
+                         void *p_copy, int64_t n, double *prr, double *pbb, int nblk, hipStream_t) {
+    if (in_dtype == SIPG_F64 && out_dtype == SIPG_F64)
+        true_res_t<double, double>(tp<double>(b), tp<double>(Ax), tp<double>(out), tp<double>(p_copy), n, prr, pbb, nblk);
+    else if (in_dtype == SIPG_F64 && out_dtype == SIPG_F32)
+        true_res_t<double, float>(tp<double>(b), tp<double>(Ax), tp<float>(out), tp<float>(p_copy), n, prr, pbb, nblk);
+    else if (in_dtype == SIPG_F32 && out_dtype == SIPG_F32)
+        true_res_t<float, float>(tp<float>(b), tp<float>(Ax), tp<float>(out), tp<float>(p_copy), n, prr, pbb, nblk);
+    else
+        return SIPG_ERR_INVALID;
+    return SIPG_OK;
+}
+
+int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t) {
+    if (x_dtype == SIPG_F32 && xl_dtype == SIPG_F64) return SIPG_ERR_INVALID;
+    for (int64_t i = 0; i < n; ++i) {
+        const double v = xl_dtype == SIPG_F64 ? tp<double>(xl)[i] : (double)tp<float>(xl)[i];
+        if (x_dtype == SIPG_F64) tp<double>(x)[i] += xl_dtype == SIPG_F64 ? v : (double)tp<float>(xl)[i];
+        else tp<float>(x)[i] += tp<float>(xl)[i];
+        if (xl_dtype == SIPG_F64) tp<double>(xl)[i] = 0;
+        else tp<float>(xl)[i] = 0;
+    }
+    return SIPG_OK;
+}
+
+int launch_replace_finalize(const double *prr, int np, cg_state *st, int slot, hipStream_t) {
+    if (st->err) return SIPG_OK;
+    const double rr = sum_partials(prr, np);
+    cg_slot c = st->s[slot];
+    st->restart_p = (rr > 4.0 * c.rr || c.done) ? 1 : 0;
+    c.rr = rr;
+    st->rr_ref = rr;
+    st->nrep += 1;
+    if (!std::isfinite(rr)) st->err = 3;
+    else if (!(std::sqrt(rr) > st->stop) || c.it >= st->maxit) { c.done = 1; c.checked = 1; }
+    else { c.done = 0; c.checked = 0; }
+    st->s[slot] = c;
+    return SIPG_OK;
+}
+
+int launch_restart_p(int dtype, const void *r, void *p, int64_t n, const cg_state *st, hipStream_t) {
+    if (!st->restart_p || st->err) return SIPG_OK;
+    std::memcpy(p, r, (size_t)n * (dtype == SIPG_F64 ? 8 : 4));
+    return SIPG_OK;
+}
+
+int launch_convert(int from, int to, const void *src, void *dst, int64_t n, hipStream_t) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double v = from == SIPG_F64 ? tp<double>(src)[i] : (double)tp<float>(src)[i];
+        if (to == SIPG_F64) tp<double>(dst)[i] = v;
+        else tp<float>(dst)[i] = (float)v;
+    }
+    return SIPG_OK;
+}
+
+int launch_rhs_fourier(int, int, int, const void *, int, const double *, const int *, const int *, void *,
+                       hipStream_t) {
+    return SIPG_ERR_UNSUPPORTED;
+}
+
+}  // namespace sipg_internal

@@ -1,0 +1,59 @@
+"""The solver engine's host orchestration (abi.cpp, solvers.cpp, basis.cpp
+-- the shipped sources) compiled for the CPU against a mock HIP runtime
+and CPU stand-ins of the kernel launchers, under AddressSanitizer and
+UBSan.  Exercises the device-state protocol (parity slots, carry-forward,
+residual replacement, iteration cap, mixed-precision folding) without a
+GPU; the numerics are the oracle's."""
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ENG = os.path.join(HERE, "cpu_engine")
+
+
+@pytest.fixture(scope="module")
+def results():
+    if shutil.which("g++") is None:
+        pytest.skip("no host compiler")
+    subprocess.run(["make", "-s", "-C", ENG], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="halt_on_error=1")
+    out = subprocess.run([os.path.join(ENG, "_build", "engine_driver")], check=True,
+                         capture_output=True, text=True, env=env, timeout=600)
+    assert "runtime error" not in out.stderr and "AddressSanitizer" not in out.stderr, out.stderr
+    return {r["case"]: r for r in map(json.loads, out.stdout.strip().splitlines())}
+
+
+def test_plain_cg_is_the_reference_iteration(results):
+    r = results["plain"]
+    assert r["rc"] == 0 and r["iters"] == r["oracle_iters"] and r["diff"] == 0.0
+
+
+def test_iteration_cap(results):
+    r = results["plain_cap"]
+    assert r["rc"] == 6 and r["iters"] == 5 and r["diff"] == 0.0
+
+
+def test_replacement_reaches_true_residual(results):
+    r = results["ex_replace"]
+    assert r["rc"] == 0 and r["reps"] >= 2 and r["relres"] <= 1e-10
+    assert abs(r["iters"] - r["oracle_iters"]) <= 0.05 * r["oracle_iters"]
+
+
+def test_no_replacement_matches_plain(results):
+    r = results["ex_noreplace"]
+    assert r["rc"] == 0 and r["diff"] == 0.0 and r["iters"] == r["oracle_iters"]
+
+
+@pytest.mark.parametrize("case", ["mixed", "mixed_keep_p"])
+def test_mixed_precision_converges_in_fp64(results, case):
+    r = results[case]
+    assert r["rc"] == 0 and r["relres"] <= 1e-10 and r["diff"] <= 1e-8
+
+
+def test_mixed_cap_reports_not_converged(results):
+    r = results["mixed_cap"]
+    assert r["rc"] == 6 and r["iters"] == 30

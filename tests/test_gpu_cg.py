@@ -45,7 +45,12 @@ def test_rhs_fourier_matches_oracle(sipg, oracle):
         op.rhs_fourier(modes, b)
         got = op.download(b)
         ref = oracle.rhs(dim, p, sipg.pen_sigma(p), oracle.problem(0, modes))
-        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+        # The reference evaluates the Dirichlet data g = u_ex at mesh
+        # coordinates accumulated by repeated addition
+        # (build_square_mesh.hpp:256-284); for h = 1/13 the bottom edge sits at
+        # y = 2.8e-16, so its g = 0 boundary terms come out O(1e-13) instead of
+        # 0.  The device rhs uses g = 0 exactly.
+        assert np.max(np.abs(got - ref)) <= 1e-11 * np.max(np.abs(ref))
 
 
 def test_c1_solve_matches_dense_lapack(sipg):
@@ -155,3 +160,85 @@ def test_cg_f32_converges_to_f32_level(sipg, oracle):
     assert st == sipg.OK
     xo, _, _ = oracle.cg(dim, dim, p, pen, bh, tol=1e-12, tol_mode=1)
     assert np.linalg.norm(op.download(x) - xo) <= 1e-3 * np.linalg.norm(xo)
+
+
+def test_cg_ex_reaches_true_relative_residual(sipg, oracle):
+    modes = sipg.forcing_draw(0, 4, 4, 4)
+    dim, p = 32, 6
+    pen = sipg.pen_sigma(p)
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    b = op.vector()
+    op.rhs_fourier(modes, b)
+    x = op.vector()
+    x.zero()
+    st, info = op.cg_ex(x, b, 1e-10, sipg.TOL_REL, replace_factor=0.1, true_stop=True)
+    assert st == sipg.OK, sipg.strerror(st)
+    assert info.replacements >= 2
+    rel = op.true_relres(x, b)
+    assert rel <= 1e-10, rel
+    assert info.true_res_norm <= 1e-10 * info.b_norm * 1.0000001
+
+
+def test_cg_ex_without_replacement_equals_cg(sipg):
+    op = sipg.Operator(12, 12, 4, 125.0, sipg.F64)
+    bh = np.random.default_rng(9).uniform(-1, 1, op.n)
+    b = op.to_device(bh)
+    x1, x2 = op.vector(), op.vector()
+    x1.zero()
+    x2.zero()
+    st1, it1, res1 = op.cg(x1, b, 1e-9, sipg.TOL_REL)
+    st2, info = op.cg_ex(x2, b, 1e-9, sipg.TOL_REL, replace_factor=0.0, true_stop=False)
+    assert st1 == st2 == sipg.OK and it1 == info.iters
+    assert np.array_equal(op.download(x1), op.download(x2))
+
+
+def test_mixed_precision_matches_fp64_oracle(sipg, oracle):
+    modes = sipg.forcing_draw(0, 4, 4, 4)
+    dim, p = 24, 7
+    pen = sipg.pen_sigma(p)
+    bh = oracle.rhs(dim, p, pen, oracle.problem(0, modes))
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    b = op.to_device(bh)
+    x = op.vector()
+    x.zero()
+    st, info = op.cg_mixed(x, b, 1e-10, sipg.TOL_REL)
+    assert st == sipg.OK, sipg.strerror(st)
+    rel = op.true_relres(x, b)
+    assert rel <= 1e-10, rel
+    # error against the FP64 oracle solve at the same tolerance
+    xo, ito, _ = oracle.cg(dim, dim, p, pen, bh, tol=1e-12, tol_mode=1)
+    err = np.linalg.norm(op.download(x) - xo) / np.linalg.norm(xo)
+    assert err <= 1e-8, err
+    assert info.replacements >= 2
+
+
+def test_mixed_precision_keep_direction_variant_finishes(sipg, oracle):
+    # the reliable-update variant (search direction kept across true
+    # residuals) on a small, well-conditioned case
+    modes = sipg.forcing_draw(0, 4, 4, 4)
+    dim, p = 8, 3
+    pen = sipg.pen_sigma(p)
+    bh = oracle.rhs(dim, p, pen, oracle.problem(0, modes))
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    b = op.to_device(bh)
+    x = op.vector()
+    x.zero()
+    st, info = op.cg_mixed(x, b, 1e-10, sipg.TOL_REL, replace_factor=0.1, restart=False)
+    assert st == sipg.OK, sipg.strerror(st)
+    assert op.true_relres(x, b) <= 1e-10
+
+
+def test_operator_free_vector_kernels(sipg):
+    import ctypes
+    n = 10007
+    a = np.random.default_rng(5).uniform(-1, 1, n)
+    b = np.random.default_rng(6).uniform(-1, 1, n)
+    A, B = sipg.DeviceBuffer(8 * n), sipg.DeviceBuffer(8 * n)
+    A.upload(a)
+    B.upload(b)
+    r = ctypes.c_double()
+    sipg.check(sipg.lib().sipg_vec_dot(sipg.F64, n, A.ptr, B.ptr, ctypes.byref(r), None), "vec_dot")
+    assert abs(r.value - a @ b) <= 1e-13 * np.abs(a * b).sum()
+    sipg.check(sipg.lib().sipg_vec_axpy(sipg.F64, n, 2.0, A.ptr, B.ptr, None), "vec_axpy")
+    sipg.synchronize()
+    assert np.array_equal(B.download(np.float64), b + 2.0 * a)
