@@ -132,6 +132,9 @@ int sipg_op_create(sipg_op **out, int dimx, int dimy, int p, double pen, int dty
     if (p > kMaxDegree) return SIPG_ERR_UNSUPPORTED;
     const int64_t noe = (int64_t)dimx * dimy;
     if (noe > (int64_t)INT_MAX) return SIPG_ERR_INVALID;
+    // the operator kernel addresses a block's footprint (up to 6 element
+    // rows) with 32-bit offsets
+    if ((int64_t)6 * dimx * (p + 1) * (p + 1) > (int64_t)INT_MAX) return SIPG_ERR_UNSUPPORTED;
     sipg_op *op = new sipg_op();
     op->dimx = dimx;
     op->dimy = dimy;
@@ -321,12 +324,20 @@ int sipg_time_apply(sipg_op *op, const void *x, void *y, int reps, double *ms_pe
     return rc;
 }
 
+// Host data in the reference order ((i1*(p+1)+i2)*noe + e) <-> the
+// element-major device order (e*(p+1)^2 + i1*(p+1)+i2).
 int sipg_upload(sipg_op *op, const double *host, void *dev) {
     if (!op || !host || !dev) return SIPG_ERR_INVALID;
+    const int64_t noe = (int64_t)op->dimx * op->dimy, nq = (int64_t)(op->p + 1) * (op->p + 1);
     if (op->dtype == SIPG_F64) {
-        SIPG_HIP_TRY(hipMemcpy(dev, host, (size_t)op->n * 8, hipMemcpyHostToDevice));
+        std::vector<double> tmp(op->n);
+        for (int64_t e = 0; e < noe; ++e)
+            for (int64_t q = 0; q < nq; ++q) tmp[e * nq + q] = host[q * noe + e];
+        SIPG_HIP_TRY(hipMemcpy(dev, tmp.data(), (size_t)op->n * 8, hipMemcpyHostToDevice));
     } else {
-        std::vector<float> tmp(host, host + op->n);
+        std::vector<float> tmp(op->n);
+        for (int64_t e = 0; e < noe; ++e)
+            for (int64_t q = 0; q < nq; ++q) tmp[e * nq + q] = (float)host[q * noe + e];
         SIPG_HIP_TRY(hipMemcpy(dev, tmp.data(), (size_t)op->n * 4, hipMemcpyHostToDevice));
     }
     return SIPG_OK;
@@ -335,14 +346,33 @@ int sipg_upload(sipg_op *op, const double *host, void *dev) {
 int sipg_download(sipg_op *op, const void *dev, double *host) {
     if (!op || !host || !dev) return SIPG_ERR_INVALID;
     SIPG_HIP_TRY(hipStreamSynchronize(op->stream));
+    const int64_t noe = (int64_t)op->dimx * op->dimy, nq = (int64_t)(op->p + 1) * (op->p + 1);
     if (op->dtype == SIPG_F64) {
-        SIPG_HIP_TRY(hipMemcpy(host, dev, (size_t)op->n * 8, hipMemcpyDeviceToHost));
+        std::vector<double> tmp(op->n);
+        SIPG_HIP_TRY(hipMemcpy(tmp.data(), dev, (size_t)op->n * 8, hipMemcpyDeviceToHost));
+        for (int64_t e = 0; e < noe; ++e)
+            for (int64_t q = 0; q < nq; ++q) host[q * noe + e] = tmp[e * nq + q];
     } else {
         std::vector<float> tmp(op->n);
         SIPG_HIP_TRY(hipMemcpy(tmp.data(), dev, (size_t)op->n * 4, hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < op->n; ++i) host[i] = tmp[i];
+        for (int64_t e = 0; e < noe; ++e)
+            for (int64_t q = 0; q < nq; ++q) host[q * noe + e] = tmp[e * nq + q];
     }
     return SIPG_OK;
+}
+
+int sipg_to_device_layout(sipg_op *op, const void *ref, void *dev) {
+    if (!op || !ref || !dev) return SIPG_ERR_INVALID;
+    if (overlaps(ref, dev, (size_t)op->n * dtype_size(op->dtype))) return SIPG_ERR_ALIAS;
+    return launch_layout(op->dtype, 1, ref, dev, (int64_t)op->dimx * op->dimy, (op->p + 1) * (op->p + 1),
+                         op->stream);
+}
+
+int sipg_to_reference_layout(sipg_op *op, const void *dev, void *ref) {
+    if (!op || !ref || !dev) return SIPG_ERR_INVALID;
+    if (overlaps(ref, dev, (size_t)op->n * dtype_size(op->dtype))) return SIPG_ERR_ALIAS;
+    return launch_layout(op->dtype, 0, dev, ref, (int64_t)op->dimx * op->dimy, (op->p + 1) * (op->p + 1),
+                         op->stream);
 }
 
 }  // extern "C"

@@ -1,0 +1,139 @@
+// basis_constexpr.hpp -- the degree-p spectral-element tables evaluated at
+// compile time, so the fused kernels (templated on p) see their
+// coefficients as constants: the compiler rematerialises them instead of
+// keeping ~(p+1)^2 table values live in registers.
+//
+// Same mathematics as basis.cpp (reference: LGL_quadrature_table.hpp:84-146,
+// sem_function.hpp:86-110, laplacian_operator.hpp:31-43): LGL nodes are the
+// endpoints plus the roots of P_p', found by Newton from Chebyshev-Gauss-
+// Lobatto guesses; weights 2/(p(p+1) P_p(z)^2); D(mode,node) the Lagrange
+// derivative; K1 = D W D^T.  The tests compare the tables the kernels use
+// with the host ones and with the oracle.
+#pragma once
+
+namespace sipg_internal {
+namespace cx {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+constexpr double cabs(double x) { return x < 0 ? -x : x; }
+
+// cos by argument reduction to [-pi, pi] and a Taylor series (enough terms
+// for full double accuracy); only used for Newton starting points.
+constexpr double ccos(double x) {
+    while (x > kPi) x -= 2 * kPi;
+    while (x < -kPi) x += 2 * kPi;
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 30; ++k) {
+        term *= -x * x / ((2 * k - 1) * (2 * k));
+        sum += term;
+    }
+    return sum;
+}
+
+// P_n(x) and P_n'(x)
+constexpr void legendre(int n, double x, double &P, double &dP) {
+    double p0 = 1.0, p1 = x, d0 = 0.0, d1 = 1.0;
+    if (n == 0) { P = 1.0; dP = 0.0; return; }
+    for (int k = 1; k < n; ++k) {
+        const double p2 = ((2.0 * k + 1.0) * x * p1 - k * p0) / (k + 1.0);
+        const double d2 = d0 + (2.0 * k + 1.0) * p1;
+        p0 = p1; p1 = p2;
+        d0 = d1; d1 = d2;
+    }
+    P = p1;
+    dP = d1;
+}
+
+template <int P>
+struct Tables {
+    static constexpr int N1 = P + 1;
+    double z[N1] = {};
+    double w[N1] = {};
+    double D[N1][N1] = {};    // D[mode][node] = phi'_mode(z_node)
+    double K1[N1][N1] = {};   // sum_k w_k D[a][k] D[b][k]
+};
+
+template <int P>
+constexpr Tables<P> make_tables() {
+    Tables<P> t{};
+    constexpr int N = P;
+    t.z[0] = -1.0;
+    t.z[N] = 1.0;
+    for (int j = 1; 2 * j < N; ++j) {
+        double x = -ccos(kPi * j / N);
+        for (int it = 0; it < 100; ++it) {
+            double Pv = 0, dP = 0;
+            legendre(N, x, Pv, dP);
+            const double d2P = (2.0 * x * dP - N * (N + 1.0) * Pv) / (1.0 - x * x);
+            const double dx = dP / d2P;
+            x -= dx;
+            if (cabs(dx) <= 1e-17) break;
+        }
+        t.z[j] = x;
+        t.z[N - j] = -x;
+    }
+    if (N % 2 == 0) t.z[N / 2] = 0.0;
+    double Lz[P + 1] = {};
+    for (int i = 0; i <= N; ++i) {
+        double Pv = 0, dP = 0;
+        legendre(N, t.z[i], Pv, dP);
+        Lz[i] = Pv;
+        t.w[i] = 2.0 / (N * (N + 1.0) * Pv * Pv);
+    }
+    for (int m = 0; m <= N; ++m)
+        for (int q = 0; q <= N; ++q) {
+            double v = 0.0;
+            if (m != q) v = Lz[q] / (Lz[m] * (t.z[q] - t.z[m]));
+            else if (q == 0) v = -0.25 * N * (N + 1.0);
+            else if (q == N) v = 0.25 * N * (N + 1.0);
+            t.D[m][q] = v;
+        }
+    for (int a = 0; a <= N; ++a)
+        for (int b = 0; b <= N; ++b) {
+            double r = 0.0;
+            for (int k = 0; k <= N; ++k) r += t.w[k] * t.D[a][k] * t.D[b][k];
+            t.K1[a][b] = r;
+        }
+    return t;
+}
+
+// Even-odd split of the centrosymmetric K1 (K1[a][b] = K1[N-a][N-b] on the
+// symmetric LGL nodes): with ue[i] = u[i] + u[N-i], uo[i] = u[i] - u[N-i]
+// (i < H = (p+1)/2, and ue[H] = u[H] for odd p+1),
+//   (K1 u)[a]   = sum_i E[a][i] ue[i] + sum_i O[a][i] uo[i]
+//   (K1 u)[N-a] = sum_i E[a][i] ue[i] - sum_i O[a][i] uo[i]
+// with E[a][i] = (K1[a][i] + K1[a][N-i]) / 2, O[a][i] = (K1[a][i] - K1[a][N-i]) / 2
+// and E[a][H] = K1[a][H].  (p+1)^2 multiply-adds become about half.
+template <int P>
+struct EvenOdd {
+    static constexpr int N1 = P + 1;
+    static constexpr int H = N1 / 2;            // pairs
+    static constexpr int ME = (N1 + 1) / 2;     // even size (H, plus the middle node)
+    double E[ME][ME] = {};
+    double O[H > 0 ? H : 1][H > 0 ? H : 1] = {};
+};
+
+template <int P>
+constexpr EvenOdd<P> make_even_odd() {
+    const Tables<P> t = make_tables<P>();
+    EvenOdd<P> eo{};
+    constexpr int N = P, H = EvenOdd<P>::H, ME = EvenOdd<P>::ME;
+    for (int a = 0; a < ME; ++a) {
+        for (int i = 0; i < H; ++i) {
+            eo.E[a][i] = 0.5 * (t.K1[a][i] + t.K1[a][N - i]);
+            if (a < H) eo.O[a][i] = 0.5 * (t.K1[a][i] - t.K1[a][N - i]);
+        }
+        if (ME > H) eo.E[a][H] = t.K1[a][H];
+    }
+    return eo;
+}
+
+// One instance per degree, usable inside device code as constants.
+template <int P>
+struct Basis {
+    static constexpr Tables<P> T = make_tables<P>();
+};
+
+}  // namespace cx
+}  // namespace sipg_internal

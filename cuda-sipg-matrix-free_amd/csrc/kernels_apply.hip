@@ -7,17 +7,22 @@
 // element, read-modify-write of the output).  Here one kernel does all of
 // it in a single pass over the input and a single write of the output:
 //
+//   * vectors are element-major on the device: the (p+1)^2 values of element
+//     e = dimx*y + x are contiguous at e*(p+1)^2 + i1*(p+1) + i2;
 //   * a workgroup owns a BX x BY block of elements and stages the block's
-//     (p+1)^2 tiles plus a one-element halo ring (the four side neighbours'
-//     tiles) in LDS, node-plane major: lds[node][slot];
-//   * thread (slot, j) keeps column j of its element in registers and does
-//     the x-direction work for it: the sum-factorised contraction
-//     w_j * K1 * U[:,j] plus the left/right face terms; it then does the
-//     y-direction work for row j from LDS (K1 * U[j,:] plus down/up faces);
-//   * the row results go back through LDS (over the element's own tile) to
-//     be added to the column results, and each thread stores one column:
-//     for a fixed node the lanes of a wave write 16 consecutive elements.
-//
+//     tiles plus a one-element halo ring (the four side neighbours' tiles)
+//     in LDS, one tile per slot with an odd row stride.  Each block row and
+//     each halo row is one contiguous run of BX*(p+1)^2 values in memory and
+//     each halo-column tile a run of (p+1)^2, so the loads are full 64-lane
+//     runs of consecutive words;
+//   * thread (slot, j) reads column j of its element and does the
+//     x-direction work for it: the sum-factorised contraction
+//     w_j * K1 * U[:,j] plus the left/right face terms; it also reads row j
+//     and does the y-direction work (K1 * U[j,:] plus down/up faces);
+//   * the column results are written over the element's own LDS tile, the
+//     row results are added to them, and the block's rows are stored back
+//     as contiguous runs.
+
 // The discrete operator is exactly the reference's (see the oracle and the
 // tests): Y = K1 U W + W U K1 + face terms of flux_term6a/6b, with the
 // same interior (1/2-average, 2*pen) and Dirichlet (one-sided, pen) rules.
@@ -27,18 +32,16 @@
 // each XCD walks a contiguous band of block rows and the halo rows it reads
 // are mostly in its own L2.
 #include "sipg_internal.hpp"
+#include "basis_constexpr.hpp"
 
 namespace sipg_internal {
-
-template <typename T, int P>
-struct ApplyCfg;   // block shape per (type, degree); see select below
 
 // BX x BY elements per workgroup, NE = BX*BY, threads = NE*(P+1) <= 1024,
 // LDS = (P+1)^2 * (NE + 2BX + 2BY) * sizeof(T) <= 160 KiB.
 template <typename T, int P>
 struct ApplyShape {
     static constexpr int N1 = P + 1;
-    static constexpr bool big64 = sizeof(T) == 8 && P >= 14;
+    static constexpr bool big64 = sizeof(T) == 8 && P >= 13;
     static constexpr bool small = N1 > 16;
     static constexpr int BX = (big64 || small) ? 8 : 16;
     static constexpr int BY = 4;
@@ -71,95 +74,159 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
     return s;   // valid in thread 0
 }
 
+// Odd LDS row stride (in elements of T) for a (p+1)^2 tile: lanes that read
+// the same node of consecutive tiles then hit distinct banks.
+__host__ __device__ constexpr int tile_stride(int nq) { return (nq % 2) ? nq : nq + 1; }
+
+// LDS position of the idx-th value of consecutive tiles stored with row
+// stride TS >= NQ.
+template <int NQ, int TS>
+__device__ __forceinline__ int lds_index(int idx) {
+    if constexpr (TS == NQ) return idx;
+    else return idx + (TS - NQ) * (idx / NQ);
+}
+
 template <typename T, int P, int BX, int BY, bool DOT>
 __global__ __launch_bounds__(BX * BY * (P + 1))
-void apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ tab,
-                  int dimx, int dimy, int nbx, int nblk, T pen,
-                  double *__restrict__ partials, const cg_state *__restrict__ st, int slot)
+void apply_kernel(const T *__restrict__ x, T *__restrict__ y, int dimx, int dimy, int nbx,
+                  int nblk, T pen, double *__restrict__ partials,
+                  const cg_state *__restrict__ st, int slot)
 {
     constexpr int N1 = P + 1;
     constexpr int NQ = N1 * N1;
+    constexpr int TS = tile_stride(NQ);
     constexpr int NE = BX * BY;
-    constexpr int NH = 2 * BX + 2 * BY;
-    constexpr int NS = NE + NH;
+    // LDS slots: (BY+2) rows of BX tiles for the element rows y0-1 .. y0+BY
+    // (slot r*BX + t holds element (x0+t, y0-1+r)), then the left halo
+    // column (x0-1, y0+k) at SL_COL+k and the right one (x0+BX, y0+k) at
+    // SL_COL+BY+k.  A block's own element (ex, ey) sits in slot (ey+1)*BX+ex,
+    // so its up/down neighbours are always slot -/+ BX.
+    constexpr int RUN = BX * NQ;                 // values of one row run
+    constexpr int NROWV = (BY + 2) * RUN;
+    constexpr int SL_COL = (BY + 2) * BX;
+    constexpr int NCOLV = 2 * BY * NQ;
+    constexpr int NS = SL_COL + 2 * BY;
     constexpr int NT = NE * N1;
-    // halo slot groups
-    constexpr int H_UP = NE, H_DN = NE + BX, H_LF = NE + 2 * BX, H_RT = NE + 2 * BX + BY;
 
-    __shared__ T lds[NQ * NS];
+    __shared__ T lds[NS * TS];
     __shared__ double red[(NT + 63) / 64];
 
     if (DOT) {
         if (st->s[slot].done | st->err) return;
     }
 
-    const T *w = tab;
-    const T *K1 = tab + N1;
-    const T *D0 = tab + N1 + NQ;
-    const T *DP = tab + 2 * N1 + NQ;
+    // degree-p tables as compile-time constants (see basis_constexpr.hpp)
+    constexpr cx::Tables<P> CT = cx::make_tables<P>();
+    struct CW { __device__ T operator[](int i) const { return (T)CT.w[i]; } } w;
+    struct C0 { __device__ T operator[](int i) const { return (T)CT.D[i][0]; } } D0;
+    struct CP { __device__ T operator[](int i) const { return (T)CT.D[i][P]; } } DP;
 
     const int tid = threadIdx.x;
     const int lb = xcd_remap(blockIdx.x, nblk);
     const int bx = lb % nbx, by = lb / nbx;
     const int x0 = bx * BX, y0 = by * BY;
-    const size_t noe = (size_t)dimx * (size_t)dimy;
+    // offsets below are relative to the block's first own tile, in int:
+    // a block's footprint spans (BY+2) rows of dimx*(p+1)^2 values
+    const int RS = dimx * NQ;                    // row stride
+    const int GAP = RS - RUN;
+    const T *xb = x + ((size_t)y0 * dimx + x0) * NQ;
+    const bool interior = x0 >= 1 && x0 + BX < dimx && y0 >= 1 && y0 + BY < dimy;
 
+    // ---- stage: row runs as one linear index space, then halo columns.
+    // Row value idx lies in run r = idx / RUN at offset idx + r*GAP - RS
+    // from xb and in LDS at lidx(idx) (tiles are consecutive slots).
+    {
+        constexpr int KR = (NROWV + NT - 1) / NT;
+        T buf[KR];
+        if (interior) {
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int idx = tid + k * NT;
+                if ((k + 1) * NT <= NROWV || idx < NROWV) {
+                    const int r = idx / RUN;
+                    buf[k] = xb[idx + r * GAP - RS];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int idx = tid + k * NT;
+                if ((k + 1) * NT <= NROWV || idx < NROWV) lds[lds_index<NQ, TS>(idx)] = buf[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int idx = tid + k * NT;
+                const int r = idx / RUN;
+                const int t = (idx - r * RUN) / NQ;
+                const int gy = y0 - 1 + r;
+                const bool ok = idx < NROWV && gy >= 0 && gy < dimy && x0 + t < dimx;
+                buf[k] = ok ? xb[idx + r * GAP - RS] : T(0);
+            }
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int idx = tid + k * NT;
+                if (idx < NROWV) lds[lds_index<NQ, TS>(idx)] = buf[k];
+            }
+        }
+        constexpr int KC = (NCOLV + NT - 1) / NT;
+        T cbuf[KC];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int idx = tid + k * NT;
+            const int kk = idx / NQ, q = idx - kk * NQ;
+            const bool right = kk >= BY;
+            const int row = right ? kk - BY : kk;
+            const bool ok = idx < NCOLV && y0 + row < dimy && (right ? x0 + BX < dimx : x0 >= 1);
+            cbuf[k] = ok ? xb[row * RS + (right ? BX * NQ : -NQ) + q] : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int idx = tid + k * NT;
+            const int kk = idx / NQ, q = idx - kk * NQ;
+            if (idx < NCOLV) lds[(SL_COL + kk) * TS + q] = cbuf[k];
+        }
+    }
+    __syncthreads();
+
+    // ---- thread (s, j): element s of the block, line j ---------------------
     const int s = tid % NE;
     int j = tid / NE;
     if (NE % 64 == 0) j = __builtin_amdgcn_readfirstlane(j);
     const int ex = s % BX, ey = s / BX;
     const int gx = x0 + ex, gy = y0 + ey;
-    const bool valid = (gx < dimx) && (gy < dimy);
-    const size_t e = (size_t)gy * dimx + gx;
-
-    // ---- stage: own column j into registers and LDS ----------------------
-    T u[N1];
-#pragma unroll
-    for (int i = 0; i < N1; ++i) {
-        u[i] = valid ? x[(size_t)(i * N1 + j) * noe + e] : T(0);
-        lds[(i * N1 + j) * NS + s] = u[i];
-    }
-    // ---- stage: halo ring (full tiles of the side neighbours) ------------
-    for (int k = tid; k < NH * NQ; k += NT) {
-        const int h = k % NH;
-        const int q = k / NH;
-        int hx, hy;
-        if (h < BX)               { hx = x0 + h;              hy = y0 - 1;  }
-        else if (h < 2 * BX)      { hx = x0 + h - BX;         hy = y0 + BY; }
-        else if (h < 2 * BX + BY) { hx = x0 - 1;              hy = y0 + h - 2 * BX; }
-        else                      { hx = x0 + BX;             hy = y0 + h - 2 * BX - BY; }
-        T v = T(0);
-        if (hx >= 0 && hx < dimx && hy >= 0 && hy < dimy)
-            v = x[(size_t)q * noe + (size_t)hy * dimx + hx];
-        lds[q * NS + NE + h] = v;
-    }
-    __syncthreads();
-
     const bool Lb = (gx == 0), Rb = (gx == dimx - 1);
     const bool Ub = (gy == 0), Db = (gy == dimy - 1);   // y grows downwards
     const T wj = w[j];
     const T pen2 = pen + pen;
+    const int sl = (ey + 1) * BX + ex;                  // own LDS slot
+    const T *tile = lds + sl * TS;
 
-    // ---- x-direction, column j (flux_term6a + first half of volume) ------
-    T Yx[N1];
+    // column j (x-direction, flux_term6a) and row j (y-direction, flux_term6b)
+    T u[N1], v[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+        u[i] = tile[i * N1 + j];
+        v[i] = tile[j * N1 + i];
+    }
+    T ev, fv, Ax, Bx, g, h, Cy, Dy;
     {
         T nL = 0, tL = 0, nR = 0, tR = 0;
-        const int sL = ex > 0 ? s - 1 : H_LF + ey;
-        const int sR = ex < BX - 1 ? s + 1 : H_RT + ey;
+        const T *tlL = lds + (ex > 0 ? sl - 1 : SL_COL + ey) * TS;
+        const T *tlR = lds + (ex < BX - 1 ? sl + 1 : SL_COL + BY + ey) * TS;
         if (!Lb) {
 #pragma unroll
             for (int i = 0; i < N1; ++i) {
-                const T v = lds[(i * N1 + j) * NS + sL];
-                nL += DP[i] * v;
-                if (i == P) tL = v;
+                const T t = tlL[i * N1 + j];
+                nL += DP[i] * t;
+                if (i == P) tL = t;
             }
         }
         if (!Rb) {
 #pragma unroll
             for (int i = 0; i < N1; ++i) {
-                const T v = lds[(i * N1 + j) * NS + sR];
-                nR += D0[i] * v;
-                if (i == 0) tR = v;
+                const T t = tlR[i * N1 + j];
+                nR += D0[i] * t;
+                if (i == 0) tR = t;
             }
         }
         T dL = 0, dR = 0;
@@ -168,34 +235,19 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restric
             dL += D0[i] * u[i];
             dR += DP[i] * u[i];
         }
-        const T ev = wj * (Lb ? u[0] : T(0.5) * (u[0] - tL));
-        const T fv = wj * (Rb ? -u[P] : T(0.5) * (tR - u[P]));
-        const T A = Lb ? dL : T(0.5) * (dL + nL);
-        const T B = Rb ? -dR : T(-0.5) * (dR + nR);
-#pragma unroll
-        for (int a = 0; a < N1; ++a) {
-            T acc = 0;
-#pragma unroll
-            for (int i = 0; i < N1; ++i) acc += K1[a * N1 + i] * u[i];
-            Yx[a] = wj * acc + D0[a] * ev + DP[a] * fv;
-        }
-        Yx[0] += wj * A + (Lb ? pen : pen2) * ev;
-        Yx[P] += wj * B - (Rb ? pen : pen2) * fv;
+        ev = wj * (Lb ? u[0] : T(0.5) * (u[0] - tL));
+        fv = wj * (Rb ? -u[P] : T(0.5) * (tR - u[P]));
+        Ax = Lb ? dL : T(0.5) * (dL + nL);
+        Bx = Rb ? -dR : T(-0.5) * (dR + nR);
     }
-
-    // ---- y-direction, row j (flux_term6b + second half of volume) --------
-    T Yy[N1];
     {
-        T v[N1];
-#pragma unroll
-        for (int k = 0; k < N1; ++k) v[k] = lds[(j * N1 + k) * NS + s];
         T nD = 0, tD = 0, nU = 0, tU = 0;
-        const int sD = ey < BY - 1 ? s + BX : H_DN + ex;   // element below (y+1)
-        const int sU = ey > 0 ? s - BX : H_UP + ex;        // element above (y-1)
+        const T *tlD = lds + (sl + BX) * TS;   // element below (y+1)
+        const T *tlU = lds + (sl - BX) * TS;   // element above (y-1)
         if (!Db) {
 #pragma unroll
             for (int k = 0; k < N1; ++k) {
-                const T t = lds[(j * N1 + k) * NS + sD];
+                const T t = tlD[j * N1 + k];
                 nD += DP[k] * t;
                 if (k == P) tD = t;
             }
@@ -203,7 +255,7 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restric
         if (!Ub) {
 #pragma unroll
             for (int k = 0; k < N1; ++k) {
-                const T t = lds[(j * N1 + k) * NS + sU];
+                const T t = tlU[j * N1 + k];
                 nU += D0[k] * t;
                 if (k == 0) tU = t;
             }
@@ -214,35 +266,99 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, const T *__restric
             dD += D0[k] * v[k];
             dU += DP[k] * v[k];
         }
-        const T g = wj * (Db ? v[0] : T(0.5) * (v[0] - tD));
-        const T h = wj * (Ub ? -v[P] : T(0.5) * (tU - v[P]));
-        const T C = Db ? dD : T(0.5) * (dD + nD);
-        const T Dd = Ub ? -dU : T(-0.5) * (dU + nU);
-#pragma unroll
-        for (int b = 0; b < N1; ++b) {
-            T acc = 0;
-#pragma unroll
-            for (int k = 0; k < N1; ++k) acc += K1[b * N1 + k] * v[k];
-            Yy[b] = wj * acc + D0[b] * g + DP[b] * h;
-        }
-        Yy[0] += wj * C + (Db ? pen : pen2) * g;
-        Yy[P] += wj * Dd - (Ub ? pen : pen2) * h;
+        g = wj * (Db ? v[0] : T(0.5) * (v[0] - tD));
+        h = wj * (Ub ? -v[P] : T(0.5) * (tU - v[P]));
+        Cy = Db ? dD : T(0.5) * (dD + nD);
+        Dy = Ub ? -dU : T(-0.5) * (dU + nU);
     }
 
-    // ---- transpose the row results through the element's own tile --------
-    __syncthreads();
+    // ---- both sum-factorised contractions, Yx[a] = (K1 U)[a][j] and
+    // Yy[a] = (U K1)[j][a] (K1 symmetric), in even-odd form: each constant
+    // of the half-size E/O blocks serves both directions while it is live.
+    T Yx[N1], Yy[N1];
+    {
+        constexpr cx::EvenOdd<P> EO = cx::make_even_odd<P>();
+        constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
+        T ue[ME], uo[H > 0 ? H : 1], ve[ME], vo[H > 0 ? H : 1];
 #pragma unroll
-    for (int b = 0; b < N1; ++b) lds[(j * N1 + b) * NS + s] = Yy[b];
-    __syncthreads();
-
-    double dot = 0.0;
+        for (int i = 0; i < H; ++i) {
+            ue[i] = u[i] + u[P - i];
+            uo[i] = u[i] - u[P - i];
+            ve[i] = v[i] + v[P - i];
+            vo[i] = v[i] - v[P - i];
+        }
+        if (ME > H) { ue[ME - 1] = u[H]; ve[ME - 1] = v[H]; }
+#pragma unroll
+        for (int a = 0; a < ME; ++a) {
+            T ex_ = 0, ey_ = 0, ox_ = 0, oy_ = 0;
+#pragma unroll
+            for (int i = 0; i < ME; ++i) {
+                const T c = (T)EO.E[a][i];
+                ex_ += c * ue[i];
+                ey_ += c * ve[i];
+            }
+            if (a < H) {
+#pragma unroll
+                for (int i = 0; i < H; ++i) {
+                    const T c = (T)EO.O[a][i];
+                    ox_ += c * uo[i];
+                    oy_ += c * vo[i];
+                }
+                Yx[a] = ex_ + ox_;
+                Yx[P - a] = ex_ - ox_;
+                Yy[a] = ey_ + oy_;
+                Yy[P - a] = ey_ - oy_;
+            } else {
+                Yx[a] = ex_;
+                Yy[a] = ey_;
+            }
+        }
+    }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
-        const T ya = Yx[a] + lds[(a * N1 + j) * NS + s];
-        if (valid) y[(size_t)(a * N1 + j) * noe + e] = ya;
-        if (DOT) dot += (double)u[a] * (double)ya;
+        Yx[a] = wj * Yx[a] + D0[a] * ev + DP[a] * fv;
+        Yy[a] = wj * Yy[a] + D0[a] * g + DP[a] * h;
+    }
+    Yx[0] += wj * Ax + (Lb ? pen : pen2) * ev;
+    Yx[P] += wj * Bx - (Rb ? pen : pen2) * fv;
+    Yy[0] += wj * Cy + (Db ? pen : pen2) * g;
+    Yy[P] += wj * Dy - (Ub ? pen : pen2) * h;
+
+    // ---- Y[a][b] = Yx (column j) + Yy (row j), assembled in the element's
+    // own LDS tile, then copied out tile by tile.
+    __syncthreads();                                   // all tile reads done
+    T *otile = lds + sl * TS;
+#pragma unroll
+    for (int a = 0; a < N1; ++a) otile[a * N1 + j] = Yx[a];
+    __syncthreads();
+    double dot = 0.0;
+#pragma unroll
+    for (int b = 0; b < N1; ++b) {
+        const T yv = otile[j * N1 + b] + Yy[b];
+        otile[j * N1 + b] = yv;
+        if (DOT) dot += (double)v[b] * (double)yv;
+    }
+    __syncthreads();
+    {
+        // copy the block's own rows out: BY row runs, linear as in staging
+        constexpr int NOUT = BY * RUN;
+        constexpr int KO = (NOUT + NT - 1) / NT;
+        T *yb = y + ((size_t)y0 * dimx + x0) * NQ;
+        const bool full = x0 + BX <= dimx && y0 + BY <= dimy;
+#pragma unroll
+        for (int k = 0; k < KO; ++k) {
+            const int idx = tid + k * NT;
+            if ((k + 1) * NT <= NOUT || idx < NOUT) {
+                const int r = idx / RUN;
+                bool ok = true;
+                if (!full) ok = y0 + r < dimy && x0 + (idx - r * RUN) / NQ < dimx;
+                if (ok) yb[idx + r * GAP] = lds[lds_index<NQ, TS>(idx + RUN)];
+            }
+        }
     }
     if (DOT) {
+        const bool valid = (gx < dimx) && (gy < dimy);
+        if (!valid) dot = 0.0;
         const double bs = block_sum<NT>(dot, red);
         if (tid == 0) partials[blockIdx.x] = bs;
     }
@@ -261,13 +377,14 @@ int launch_one(const void *x, void *y, const void *tab, int dimx, int dimy, doub
     dim3 grid(nblk), block(BX * BY * (P + 1));
     if (partials) {
         hipLaunchKernelGGL((apply_kernel<T, P, BX, BY, true>), grid, block, 0, s,
-                           (const T *)x, (T *)y, (const T *)tab, dimx, dimy, nbx, nblk, (T)pen,
+                           (const T *)x, (T *)y, dimx, dimy, nbx, nblk, (T)pen,
                            partials, st, slot);
     } else {
         hipLaunchKernelGGL((apply_kernel<T, P, BX, BY, false>), grid, block, 0, s,
-                           (const T *)x, (T *)y, (const T *)tab, dimx, dimy, nbx, nblk, (T)pen,
+                           (const T *)x, (T *)y, dimx, dimy, nbx, nblk, (T)pen,
                            partials, st, slot);
     }
+    (void)tab;
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

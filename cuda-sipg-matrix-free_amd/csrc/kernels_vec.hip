@@ -277,6 +277,30 @@ __global__ __launch_bounds__(NT) void convert_kernel(const S *__restrict__ src, 
         dst[i] = (D)src[i];
 }
 
+// Layout conversion between the reference order ((i1*(p+1)+i2)*noe + e) and
+// the element-major device order (e*(p+1)^2 + i1*(p+1)+i2).
+template <typename T>
+__global__ __launch_bounds__(NT) void to_device_layout_kernel(const T *__restrict__ ref, T *__restrict__ dev,
+                                                              int64_t noe, int nq) {
+    const int64_t n = noe * nq;
+    for (int64_t idx = (int64_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * NT) {
+        const int64_t e = idx / nq;
+        const int q = (int)(idx - e * nq);
+        dev[idx] = ref[(int64_t)q * noe + e];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void to_reference_layout_kernel(const T *__restrict__ dev, T *__restrict__ ref,
+                                                                 int64_t noe, int nq) {
+    const int64_t n = noe * nq;
+    for (int64_t idx = (int64_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * NT) {
+        const int q = (int)(idx / noe);
+        const int64_t e = idx - (int64_t)q * noe;
+        ref[idx] = dev[e * nq + q];
+    }
+}
+
 // Load vector of the Fourier problem with g = 0 (sipg_sem_2d.hpp:248-262):
 // b(i1,i2,e) = (w_i1 h/2)(w_i2 h/2) f(x_e + h(z_i1+1)/2, y_e + h(z_i2+1)/2),
 // x_e = col*h, y_e = 1 - (row+1)*h.
@@ -291,9 +315,11 @@ __global__ __launch_bounds__(NT) void rhs_fourier_kernel(int p, int dim, const d
     const double *w = tab;
     const double *z = tab + 3 * n1 + n1 * n1;
     const double h = 1.0 / dim;
+    const int nq = n1 * n1;
     for (int64_t idx = (int64_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * NT) {
-        const int64_t e = idx % noe;
-        const int node = (int)(idx / noe);
+        // element-major device layout: idx = e * (p+1)^2 + (i1*(p+1) + i2)
+        const int64_t e = idx / nq;
+        const int node = (int)(idx - e * nq);
         const int i1 = node / n1, i2 = node % n1;
         const int col = (int)(e % dim), row = (int)(e / dim);
         const double xe = col * h, ye = 1.0 - (row + 1) * h;
@@ -455,6 +481,22 @@ int launch_convert(int from_dtype, int to_dtype, const void *src, void *dst, int
         hipLaunchKernelGGL((convert_kernel<double, double>), dim3(nb), dim3(NT), 0, s, (const double *)src, (double *)dst, n);
     else
         hipLaunchKernelGGL((convert_kernel<float, float>), dim3(nb), dim3(NT), 0, s, (const float *)src, (float *)dst, n);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t noe, int nq, hipStream_t s) {
+    int64_t nb = (noe * nq + NT - 1) / NT;
+    if (nb > 4096) nb = 4096;
+    if (to_device) {
+        DISPATCH_T(dtype,
+            hipLaunchKernelGGL(to_device_layout_kernel<double>, dim3((int)nb), dim3(NT), 0, s, (const double *)src, (double *)dst, noe, nq),
+            hipLaunchKernelGGL(to_device_layout_kernel<float>, dim3((int)nb), dim3(NT), 0, s, (const float *)src, (float *)dst, noe, nq));
+    } else {
+        DISPATCH_T(dtype,
+            hipLaunchKernelGGL(to_reference_layout_kernel<double>, dim3((int)nb), dim3(NT), 0, s, (const double *)src, (double *)dst, noe, nq),
+            hipLaunchKernelGGL(to_reference_layout_kernel<float>, dim3((int)nb), dim3(NT), 0, s, (const float *)src, (float *)dst, noe, nq));
+    }
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

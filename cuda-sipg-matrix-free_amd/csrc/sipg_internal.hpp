@@ -113,6 +113,9 @@ int launch_replace_finalize(const double *prr, int np, cg_state *st, int slot, h
 int launch_restart_p(int dtype, const void *r, void *p, int64_t n, const cg_state *st, hipStream_t s);
 int launch_convert(int from_dtype, int to_dtype, const void *src, void *dst, int64_t n,
                    hipStream_t s);
+// reference order <-> element-major device order (to_device = 1: ref -> dev)
+int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t noe, int nq,
+                  hipStream_t s);
 int launch_rhs_fourier(int dtype, int p, int dim, const void *tab_f64, int nmodes,
                        const double *a_dev, const int *m_dev, const int *n_dev, void *b,
                        hipStream_t s);

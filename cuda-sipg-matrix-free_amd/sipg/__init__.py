@@ -102,6 +102,8 @@ def _declare(L):
         "sipg_time_apply": (i, [_vp, _vp, _vp, i, _dp]),
         "sipg_upload": (i, [_vp, _dp, _vp]),
         "sipg_download": (i, [_vp, _vp, _dp]),
+        "sipg_to_device_layout": (i, [_vp, _vp, _vp]),
+        "sipg_to_reference_layout": (i, [_vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -216,6 +218,14 @@ class Operator:
         v = self.vector()
         self.upload(host, v)
         return v
+
+    def to_device_layout(self, ref, dev):
+        """Device array in the reference order -> element-major (async)."""
+        return lib().sipg_to_device_layout(self.h, ref.ptr, dev.ptr)
+
+    def to_reference_layout(self, dev, ref):
+        """Element-major device array -> the reference order (async)."""
+        return lib().sipg_to_reference_layout(self.h, dev.ptr, ref.ptr)
 
     def apply(self, x, y):
         return lib().sipg_apply(self.h, x.ptr, y.ptr)

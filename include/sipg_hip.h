@@ -7,11 +7,16 @@
  * function returns an int status (SIPG_OK = 0) and never exits the process;
  * sipg_strerror() turns a status into text.
  *
- * Vector layout ("mode_vector" order): the dof (i1,i2,e) of an element
- * e = dimx*y + x lives at index ((i1*(p+1)+i2)*noe + e), noe = dimx*dimy,
- * i1 the x-node, i2 the y-node (reference mode_vector.hpp:108-120,
- * mesh_info.hpp:231-235).  The device layout IS this order, so a reference
- * mode_vector's device array can be passed as is.
+ * Vector layouts.  The reference ("mode_vector" order) stores the dof
+ * (i1,i2,e) of element e = dimx*y + x at ((i1*(p+1)+i2)*noe + e), noe =
+ * dimx*dimy, i1 the x-node, i2 the y-node (mode_vector.hpp:108-120,
+ * mesh_info.hpp:231-235).  Device vectors of this library are element-major:
+ * the same dof lives at (e*(p+1)^2 + i1*(p+1)+i2), so each element's tile is
+ * contiguous and the operator's halo reads are whole tiles.  sipg_upload /
+ * sipg_download take and return host data in the reference order;
+ * sipg_to_device_layout / sipg_to_reference_layout convert device arrays
+ * (e.g. a reference mode_vector's device array).  Vector kernels (dot, axpy,
+ * ...) do not depend on the layout.
  *
  * Penalty: `pen` is the reference kernels' pen (sipg_sem_2d_gpu_kernels.hpp:
  * 243-244, 308-309); the physical SIPG sigma of sigma/h*int[u][v] is 2*pen.
@@ -193,10 +198,16 @@ int sipg_rhs_fourier(sipg_op *op, int nmodes, const double *a, const int *m, con
 int sipg_time_apply(sipg_op *op, const void *x, void *y, int reps, double *ms_per_apply);
 
 /* ---- layout helpers ---------------------------------------------------- */
-/* Device layout == host_mode_vector layout, so these are typed copies;
- * they also convert between FP64 host data and an FP32 op. */
+/* Host FP64 data in the reference order <-> device vector of the op's dtype
+ * in the element-major order (synchronous; converts FP64 <-> FP32). */
 int sipg_upload(sipg_op *op, const double *host, void *dev);
 int sipg_download(sipg_op *op, const void *dev, double *host);
+/* Device array in the reference order <-> element-major device order, in
+ * the op's dtype (asynchronous on the op's stream; arrays must not overlap).
+ * Replace copy() of mode_vector.hpp:314-359 when data is already on the
+ * device in the reference's layout. */
+int sipg_to_device_layout(sipg_op *op, const void *ref, void *dev);
+int sipg_to_reference_layout(sipg_op *op, const void *dev, void *ref);
 
 #ifdef __cplusplus
 }

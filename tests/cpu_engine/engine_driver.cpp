@@ -39,8 +39,8 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
     void *x, *b;
     sipg_malloc(&x, n * 8);
     sipg_malloc(&b, n * 8);
-    sipg_memcpy_h2d(b, bh.data(), n * 8);
-    sipg_memcpy_h2d(x, xh.data(), n * 8);
+    sipg_upload(op, bh.data(), b);  // reference order -> element-major device order
+    sipg_upload(op, xh.data(), x);
     sipg_cg_opts o{tol, SIPG_TOL_REL, maxit, rf, 1, restart};
     sipg_cg_info info{};
     if (mode == 0) {
@@ -54,7 +54,7 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
     } else {
         rc = sipg_cg_mixed(op, x, b, &o, &info);
     }
-    sipg_memcpy_d2h(xh.data(), x, n * 8);
+    sipg_download(op, x, xh.data());
     std::vector<double> xo(n, 0.0);
     double rro = 0;
     const int ito = orc_cg(d, d, p, pen, bh.data(), xo.data(), tol, 1, maxit, &rro);

@@ -46,12 +46,25 @@ static double sum_partials(const double *part, int np) {
     return s;
 }
 
+// the device layout is element-major; the oracle takes the reference order
+template <typename T>
+static void apply_aos(int dimx, int dimy, int p, double pen, const T *x, T *y) {
+    const int64_t noe = (int64_t)dimx * dimy, nq = (int64_t)(p + 1) * (p + 1);
+    std::vector<T> xr(noe * nq), yr(noe * nq);
+    for (int64_t e = 0; e < noe; ++e)
+        for (int64_t q = 0; q < nq; ++q) xr[q * noe + e] = x[e * nq + q];
+    if constexpr (sizeof(T) == 8) orc_apply(dimx, dimy, p, pen, xr.data(), yr.data());
+    else orc_apply_f32(dimx, dimy, p, (float)pen, xr.data(), yr.data());
+    for (int64_t e = 0; e < noe; ++e)
+        for (int64_t q = 0; q < nq; ++q) y[e * nq + q] = yr[q * noe + e];
+}
+
 int launch_apply(int dtype, int p, const void *x, void *y, const void *, int dimx, int dimy,
                  double pen, double *partials, const cg_state *st, int slot, hipStream_t) {
     if (partials && (st->s[slot].done | st->err)) return SIPG_OK;
     const int64_t n = (int64_t)dimx * dimy * (p + 1) * (p + 1);
-    if (dtype == SIPG_F64) orc_apply(dimx, dimy, p, pen, tp<double>(x), tp<double>(y));
-    else orc_apply_f32(dimx, dimy, p, (float)pen, tp<float>(x), tp<float>(y));
+    if (dtype == SIPG_F64) apply_aos<double>(dimx, dimy, p, pen, tp<double>(x), tp<double>(y));
+    else apply_aos<float>(dimx, dimy, p, pen, tp<float>(x), tp<float>(y));
     if (partials) put_partials(partials, kMockApplyBlocks, dotv(dtype, x, y, n));
     return SIPG_OK;
 }
@@ -243,6 +256,17 @@ int launch_convert(int from, int to, const void *src, void *dst, int64_t n, hipS
         if (to == SIPG_F64) tp<double>(dst)[i] = v;
         else tp<float>(dst)[i] = (float)v;
     }
+    return SIPG_OK;
+}
+
+int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t noe, int nq, hipStream_t) {
+    const size_t sz = dtype == SIPG_F64 ? 8 : 4;
+    for (int64_t e = 0; e < noe; ++e)
+        for (int64_t q = 0; q < nq; ++q) {
+            const int64_t r = q * noe + e, d = e * nq + q;
+            if (to_device) std::memcpy((char *)dst + d * sz, (const char *)src + r * sz, sz);
+            else std::memcpy((char *)dst + r * sz, (const char *)src + d * sz, sz);
+        }
     return SIPG_OK;
 }
 

@@ -27,14 +27,20 @@ def results():
     return {r["case"]: r for r in map(json.loads, out.stdout.strip().splitlines())}
 
 
+# The engine's vectors are element-major and the oracle's are in the
+# reference order, so dot products sum the same terms in another order:
+# the iterates agree to rounding, not bit for bit.
+ROUNDING = 1e-11
+
+
 def test_plain_cg_is_the_reference_iteration(results):
     r = results["plain"]
-    assert r["rc"] == 0 and r["iters"] == r["oracle_iters"] and r["diff"] == 0.0
+    assert r["rc"] == 0 and r["iters"] == r["oracle_iters"] and r["diff"] <= ROUNDING
 
 
 def test_iteration_cap(results):
     r = results["plain_cap"]
-    assert r["rc"] == 6 and r["iters"] == 5 and r["diff"] == 0.0
+    assert r["rc"] == 6 and r["iters"] == 5 and r["diff"] <= ROUNDING
 
 
 def test_replacement_reaches_true_residual(results):
@@ -45,7 +51,7 @@ def test_replacement_reaches_true_residual(results):
 
 def test_no_replacement_matches_plain(results):
     r = results["ex_noreplace"]
-    assert r["rc"] == 0 and r["diff"] == 0.0 and r["iters"] == r["oracle_iters"]
+    assert r["rc"] == 0 and r["diff"] <= ROUNDING and r["iters"] == r["oracle_iters"]
 
 
 @pytest.mark.parametrize("case", ["mixed", "mixed_keep_p"])

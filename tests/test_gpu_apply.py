@@ -16,7 +16,8 @@ def _x(n, seed):
 
 
 @pytest.mark.parametrize("p", list(range(1, 17)))
-@pytest.mark.parametrize("dx,dy", [(5, 3), (17, 9), (1, 1), (16, 4), (33, 2)])
+# (50, 14) has workgroups whose halo ring lies inside the mesh on all sides
+@pytest.mark.parametrize("dx,dy", [(5, 3), (17, 9), (1, 1), (16, 4), (33, 2), (50, 14)])
 def test_apply_f64_matches_oracle(sipg, oracle, p, dx, dy):
     pen = 5.0 * (p + 1) ** 2
     op = sipg.Operator(dx, dy, p, pen, sipg.F64)
@@ -28,7 +29,7 @@ def test_apply_f64_matches_oracle(sipg, oracle, p, dx, dy):
 
 
 @pytest.mark.parametrize("p", [2, 3, 8, 12, 15, 16])
-@pytest.mark.parametrize("dx,dy", [(7, 5), (16, 8)])
+@pytest.mark.parametrize("dx,dy", [(7, 5), (16, 8), (50, 14)])
 def test_apply_f32_close_to_oracle(sipg, oracle, p, dx, dy):
     pen = 5.0 * (p + 1) ** 2
     op = sipg.Operator(dx, dy, p, pen, sipg.F32)
@@ -94,3 +95,30 @@ def test_tables_match_oracle(sipg, oracle):
         assert np.max(np.abs(D - Do)) < 1e-14 * np.max(np.abs(Do)) * (p + 1)
         Ko = oracle.k1(p)
         assert np.max(np.abs(K - Ko)) < 1e-14 * np.max(np.abs(Ko)) * (p + 1)
+
+
+@pytest.mark.parametrize("dtype", ["F64", "F32"])
+def test_layout_converters_round_trip(sipg, oracle, dtype):
+    # a reference-order device array (a mode_vector's data) converted to the
+    # element-major order must equal the host upload, and the operator applied
+    # to it, converted back, must equal the oracle's reference-order result
+    dim, p = 11, 5
+    dt = getattr(sipg, dtype)
+    op = sipg.Operator(dim, dim, p, 180.0, dt)
+    npdt = np.float64 if dt == sipg.F64 else np.float32
+    x = _x(op.n, 9).astype(npdt)
+    ref_buf = sipg.DeviceBuffer(op.n * op.itemsize)
+    ref_buf.upload(x)                                   # raw copy: reference order
+    dev = op.vector()
+    assert op.to_device_layout(ref_buf, dev) == 0
+    assert np.array_equal(op.download(dev), x.astype(np.float64))
+    y = op.vector()
+    assert op.apply(dev, y) == 0
+    back = sipg.DeviceBuffer(op.n * op.itemsize)
+    assert op.to_reference_layout(y, back) == 0
+    got = back.download(npdt).astype(np.float64)
+    assert np.array_equal(got, op.download(y))
+    ref = oracle.apply(dim, dim, p, 180.0, x.astype(np.float64))
+    tol = TOL64 if dt == sipg.F64 else 2e-6 * (p + 1)
+    assert np.max(np.abs(got - ref)) <= tol * np.max(np.abs(ref))
+    assert op.to_device_layout(ref_buf, ref_buf) == sipg.ERR_ALIAS
