@@ -24,6 +24,11 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 DIM, P = 256, 8
 TOL = 1e-10
+# Iteration caps: the FP64 solve needs about 8.2k iterations here and the
+# mixed one about 40k FP32 iterations; a solve that hits its cap fails
+# loudly instead of running to the default cap of n iterations.
+MAXIT_FP64 = 30000
+MAXIT_MIXED = 60000
 
 
 def log(*a):
@@ -36,10 +41,14 @@ def gpu_solve(sipg, op, b, x, mixed):
     sipg.synchronize()
     t0 = time.perf_counter()
     if mixed:
-        st, info = op.cg_mixed(x, b, TOL, sipg.TOL_REL)
+        st, info = op.cg_mixed(x, b, TOL, sipg.TOL_REL, maxit=MAXIT_MIXED, replace_factor=1e-2,
+                               restart=1)
     else:
-        st, info = op.cg_ex(x, b, TOL, sipg.TOL_REL, replace_factor=1e-3, true_stop=True,
-                            restart=True)
+        # restart cycles with a true residual each time the recursive one has
+        # fallen 1000x: the only variant measured to reach 1e-10 here, where
+        # the FP64 floor of the true residual is about 8e-11
+        st, info = op.cg_ex(x, b, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=1e-3,
+                            true_stop=True, restart=1)
     t1 = time.perf_counter()
     if st != sipg.OK:
         raise RuntimeError(f"{'mixed' if mixed else 'fp64'} solve failed: {sipg.strerror(st)}")
@@ -56,12 +65,15 @@ def cpu_baseline(pen, modes, gpu_iters, n_dof):
     x = np.zeros_like(b)
     O.apply(DIM, DIM, P, pen, b)                  # warm-up (page in, thread pool)
     reps = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         O.apply(DIM, DIM, P, pen, b)
         reps.append(time.perf_counter() - t0)
     t_apply = float(np.median(reps))
-    k = 20
+    # about 15 s of CG iterations, at least 5
+    t0 = time.perf_counter()
+    O.cg(DIM, DIM, P, pen, b, x0=x, tol=0.0, tol_mode=1, maxit=3)
+    k = int(max(5, min(1000, 15.0 / ((time.perf_counter() - t0) / 3))))
     t0 = time.perf_counter()
     O.cg(DIM, DIM, P, pen, b, x0=x, tol=0.0, tol_mode=1, maxit=k)
     t_iter = (time.perf_counter() - t0) / k
@@ -77,12 +89,66 @@ def cpu_baseline(pen, modes, gpu_iters, n_dof):
     }
 
 
+def profile_apply(sipg, op, reps):
+    """N applies of the 256x256 p=8 FP64 operator and N copies of one
+    vector of the same length: the copy kernel moves exactly 8n bytes in
+    and 8n bytes out, which calibrates FETCH_SIZE / WRITE_SIZE."""
+    x, y, z = op.vector(), op.vector(), op.vector()
+    x.upload(np.random.default_rng(1).uniform(-1, 1, op.n))
+    op.time_apply(x, y, reps)
+    for _ in range(reps):
+        sipg.check(sipg.lib().sipg_vec_copy(sipg.F64, op.n, x.ptr, z.ptr, None), "vec_copy")
+    sipg.synchronize()
+    log(f"profile-apply: {reps} applies and {reps} copies of {op.n} doubles "
+        f"({8 * op.n} bytes per vector)")
+
+
+def hbm_traffic(n):
+    """HBM bytes per apply launch from two rocprofv3 counter runs (FETCH_SIZE,
+    then WRITE_SIZE) of `bench.py --profile-apply`, each scaled so that the
+    copy kernel of the same run reads / writes exactly 8n bytes.  Runs as a
+    child process; returns (bytes or None, note)."""
+    import shutil
+    import subprocess
+    import tempfile
+    from sipg.rocprof_summary import pmc_means
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    out = {}
+    with tempfile.TemporaryDirectory(prefix="sipg_pmc_") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = ["rocprofv3", "--pmc", counter, "-f", "csv", "-d", d, "-o", "pmc", "--",
+                   sys.executable, os.path.abspath(__file__), "--profile-apply", "10"]
+            try:
+                subprocess.run(cmd, check=True, timeout=120, cwd=tmp, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL)
+            except (subprocess.SubprocessError, OSError) as exc:
+                return None, f"rocprofv3 {counter} run failed: {exc}"
+            app, _ = pmc_means(d, r"apply_kernel")
+            cpy, _ = pmc_means(d, r"copy_kernel")
+            if counter not in app or counter not in cpy or cpy[counter][0] <= 0:
+                return None, f"no {counter} values for apply/copy"
+            factor = 8.0 * n / (cpy[counter][0] * 1024.0)      # counters are in KiB
+            out[counter] = (app[counter][0] * 1024.0 * factor, factor)
+    total = out["FETCH_SIZE"][0] + out["WRITE_SIZE"][0]
+    note = (f"FETCH {out['FETCH_SIZE'][0] / 1e6:.2f} MB (x{out['FETCH_SIZE'][1]:.3f}), "
+            f"WRITE {out['WRITE_SIZE'][0] / 1e6:.2f} MB (x{out['WRITE_SIZE'][1]:.3f}) per launch, "
+            f"scaled by the copy kernel's 8n-byte reads and writes")
+    return total, note
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs that measure roofline.traffic")
+    ap.add_argument("--profile-apply", type=int, default=0, metavar="N",
+                    help="only launch N operator applies and N copies of one vector "
+                         "(for hardware-counter runs), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -98,6 +164,10 @@ def main():
     b = op.vector()
     op.rhs_fourier(modes, b)
     x = op.vector()
+
+    if args.profile_apply:
+        profile_apply(sipg, op, args.profile_apply)
+        return
 
     # ---- operator-apply roofline (dominant kernel) -----------------------
     xa = op.vector()
@@ -167,6 +237,11 @@ def main():
     it = int(np.median(iters))
     log(f"timed ({solver}): {ms_step:.1f} ms/solve, {value:.3e} DoF/s")
 
+    traffic, traffic_note = (None, "not measured (--no-traffic)")
+    if not args.no_traffic:
+        traffic, traffic_note = hbm_traffic(n)
+        log(f"traffic: {traffic_note}")
+
     out = {
         "metric": "DoF/s to 1e-10 rel-res: mixed+batched vs plain FP64 GPU; p50/p99 solve latency",
         "value": value,
@@ -198,7 +273,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": apply_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_note": traffic_note,
             "kernel": "apply_kernel<double,8,16,4> (fused volume + flux_term6a + flux_term6b)",
             "algorithmic_bytes_per_launch": apply_bytes,
         },

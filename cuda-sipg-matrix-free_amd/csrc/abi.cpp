@@ -259,6 +259,12 @@ int sipg_vec_axpy(int dtype, int64_t n, double alpha, const void *x, void *y, vo
     return launch_axpy(dtype, alpha, x, y, n, (hipStream_t)stream);
 }
 
+int sipg_vec_copy(int dtype, int64_t n, const void *x, void *y, void *stream) {
+    if (!x || !y || n < 0 || (dtype != SIPG_F64 && dtype != SIPG_F32)) return SIPG_ERR_INVALID;
+    if (overlaps(x, y, (size_t)n * dtype_size(dtype))) return SIPG_ERR_ALIAS;
+    return launch_copy(dtype, x, y, n, (hipStream_t)stream);
+}
+
 int sipg_vec_xpby(int dtype, int64_t n, const void *r, double beta, void *p, void *stream) {
     if (!r || !p || n < 0 || (dtype != SIPG_F64 && dtype != SIPG_F32)) return SIPG_ERR_INVALID;
     return launch_xpby(dtype, r, beta, p, n, (hipStream_t)stream);

@@ -34,6 +34,9 @@
 #include "sipg_internal.hpp"
 #include "basis_constexpr.hpp"
 
+#include <algorithm>
+#include <atomic>
+
 namespace sipg_internal {
 
 // BX x BY elements per workgroup, NE = BX*BY, threads = NE*(P+1) <= 1024,
@@ -78,6 +81,16 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 // the same node of consecutive tiles then hit distinct banks.
 __host__ __device__ constexpr int tile_stride(int nq) { return (nq % 2) ? nq : nq + 1; }
 
+// A copy of v that the compiler cannot see through.  The block loop of the
+// apply recomputes the staging index arithmetic from it in every pass, which
+// is cheaper than holding ~20 loop-invariant offsets in registers across
+// the compute phase.  (A register move; no memory access.)
+__device__ __forceinline__ int opaque(int v) {
+    int r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
 // LDS position of the idx-th value of consecutive tiles stored with row
 // stride TS >= NQ.
 template <int NQ, int TS>
@@ -107,6 +120,10 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, int dimx, int dimy
     constexpr int NCOLV = 2 * BY * NQ;
     constexpr int NS = SL_COL + 2 * BY;
     constexpr int NT = NE * N1;
+    constexpr int KR = (NROWV + NT - 1) / NT;    // row values per thread
+    constexpr int KC = (NCOLV + NT - 1) / NT;    // halo-column values per thread
+    constexpr int NOUT = BY * RUN;
+    constexpr int KO = (NOUT + NT - 1) / NT;
 
     __shared__ T lds[NS * TS];
     __shared__ double red[(NT + 63) / 64];
@@ -122,22 +139,29 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, int dimx, int dimy
     struct CP { __device__ T operator[](int i) const { return (T)CT.D[i][P]; } } DP;
 
     const int tid = threadIdx.x;
-    const int lb = xcd_remap(blockIdx.x, nblk);
-    const int bx = lb % nbx, by = lb / nbx;
-    const int x0 = bx * BX, y0 = by * BY;
-    // offsets below are relative to the block's first own tile, in int:
-    // a block's footprint spans (BY+2) rows of dimx*(p+1)^2 values
+    // offsets are relative to a block's first own tile, in int: a block's
+    // footprint spans (BY+2) rows of dimx*(p+1)^2 values
     const int RS = dimx * NQ;                    // row stride
     const int GAP = RS - RUN;
-    const T *xb = x + ((size_t)y0 * dimx + x0) * NQ;
-    const bool interior = x0 >= 1 && x0 + BX < dimx && y0 >= 1 && y0 + BY < dimy;
 
-    // ---- stage: row runs as one linear index space, then halo columns.
+    // Persistent walk: the workgroups split the blocks into contiguous
+    // ranges (consecutive blocks are horizontal neighbours), and each
+    // workgroup loads block b+1 into registers while it computes block b.
+    const int G = gridDim.x;
+    const int lw = xcd_remap(blockIdx.x, G);
+    const int bbeg = (int)((int64_t)lw * nblk / G);
+    const int bend = (int)((int64_t)(lw + 1) * nblk / G);
+
     // Row value idx lies in run r = idx / RUN at offset idx + r*GAP - RS
-    // from xb and in LDS at lidx(idx) (tiles are consecutive slots).
-    {
-        constexpr int KR = (NROWV + NT - 1) / NT;
-        T buf[KR];
+    // from the block's first tile and in LDS at lds_index(idx) (the row
+    // tiles are consecutive slots); halo-column value idx is node idx % NQ of
+    // column tile idx / NQ.
+    T buf[KR], cbuf[KC];
+    auto prefetch = [&](int b) {
+        const int tid = opaque(threadIdx.x);
+        const int x0 = (b % nbx) * BX, y0 = (b / nbx) * BY;
+        const T *xb = x + ((size_t)y0 * dimx + x0) * NQ;
+        const bool interior = x0 >= 1 && x0 + BX < dimx && y0 >= 1 && y0 + BY < dimy;
         if (interior) {
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
@@ -146,11 +170,6 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, int dimx, int dimy
                     const int r = idx / RUN;
                     buf[k] = xb[idx + r * GAP - RS];
                 }
-            }
-#pragma unroll
-            for (int k = 0; k < KR; ++k) {
-                const int idx = tid + k * NT;
-                if ((k + 1) * NT <= NROWV || idx < NROWV) lds[lds_index<NQ, TS>(idx)] = buf[k];
             }
         } else {
 #pragma unroll
@@ -162,14 +181,7 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, int dimx, int dimy
                 const bool ok = idx < NROWV && gy >= 0 && gy < dimy && x0 + t < dimx;
                 buf[k] = ok ? xb[idx + r * GAP - RS] : T(0);
             }
-#pragma unroll
-            for (int k = 0; k < KR; ++k) {
-                const int idx = tid + k * NT;
-                if (idx < NROWV) lds[lds_index<NQ, TS>(idx)] = buf[k];
-            }
         }
-        constexpr int KC = (NCOLV + NT - 1) / NT;
-        T cbuf[KC];
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
             const int idx = tid + k * NT;
@@ -179,192 +191,228 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, int dimx, int dimy
             const bool ok = idx < NCOLV && y0 + row < dimy && (right ? x0 + BX < dimx : x0 >= 1);
             cbuf[k] = ok ? xb[row * RS + (right ? BX * NQ : -NQ) + q] : T(0);
         }
+    };
+    auto stage = [&]() {
+        const int tid = opaque(threadIdx.x);
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int idx = tid + k * NT;
+            if ((k + 1) * NT <= NROWV || idx < NROWV) lds[lds_index<NQ, TS>(idx)] = buf[k];
+        }
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
             const int idx = tid + k * NT;
             const int kk = idx / NQ, q = idx - kk * NQ;
             if (idx < NCOLV) lds[(SL_COL + kk) * TS + q] = cbuf[k];
         }
-    }
-    __syncthreads();
+    };
 
-    // ---- thread (s, j): element s of the block, line j ---------------------
+    // ---- thread (s, j): element s of a block, line j ---------------------
     const int s = tid % NE;
     int j = tid / NE;
     if (NE % 64 == 0) j = __builtin_amdgcn_readfirstlane(j);
     const int ex = s % BX, ey = s / BX;
-    const int gx = x0 + ex, gy = y0 + ey;
-    const bool Lb = (gx == 0), Rb = (gx == dimx - 1);
-    const bool Ub = (gy == 0), Db = (gy == dimy - 1);   // y grows downwards
     const T wj = w[j];
     const T pen2 = pen + pen;
     const int sl = (ey + 1) * BX + ex;                  // own LDS slot
     const T *tile = lds + sl * TS;
+    const T *tlL = lds + (ex > 0 ? sl - 1 : SL_COL + ey) * TS;
+    const T *tlR = lds + (ex < BX - 1 ? sl + 1 : SL_COL + BY + ey) * TS;
+    const T *tlD = lds + (sl + BX) * TS;                // element below (y+1)
+    const T *tlU = lds + (sl - BX) * TS;                // element above (y-1)
+    T *otile = lds + sl * TS;
 
-    // column j (x-direction, flux_term6a) and row j (y-direction, flux_term6b)
-    T u[N1], v[N1];
-#pragma unroll
-    for (int i = 0; i < N1; ++i) {
-        u[i] = tile[i * N1 + j];
-        v[i] = tile[j * N1 + i];
-    }
-    T ev, fv, Ax, Bx, g, h, Cy, Dy;
-    {
-        T nL = 0, tL = 0, nR = 0, tR = 0;
-        const T *tlL = lds + (ex > 0 ? sl - 1 : SL_COL + ey) * TS;
-        const T *tlR = lds + (ex < BX - 1 ? sl + 1 : SL_COL + BY + ey) * TS;
-        if (!Lb) {
-#pragma unroll
-            for (int i = 0; i < N1; ++i) {
-                const T t = tlL[i * N1 + j];
-                nL += DP[i] * t;
-                if (i == P) tL = t;
-            }
-        }
-        if (!Rb) {
-#pragma unroll
-            for (int i = 0; i < N1; ++i) {
-                const T t = tlR[i * N1 + j];
-                nR += D0[i] * t;
-                if (i == 0) tR = t;
-            }
-        }
-        T dL = 0, dR = 0;
+    double dot = 0.0;
+    if (bbeg < bend) prefetch(bbeg);
+    for (int b = bbeg; b < bend; ++b) {
+        const int x0 = (b % nbx) * BX, y0 = (b / nbx) * BY;
+        stage();
+        __syncthreads();
+        if (b + 1 < bend) prefetch(b + 1);
+
+        const int gx = x0 + ex, gy = y0 + ey;
+        const bool Lb = (gx == 0), Rb = (gx == dimx - 1);
+        const bool Ub = (gy == 0), Db = (gy == dimy - 1);   // y grows downwards
+
+        // column j (x-direction, flux_term6a) and row j (y-direction, flux_term6b)
+        T u[N1], v[N1];
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-            dL += D0[i] * u[i];
-            dR += DP[i] * u[i];
+            u[i] = tile[i * N1 + j];
+            v[i] = tile[j * N1 + i];
         }
-        ev = wj * (Lb ? u[0] : T(0.5) * (u[0] - tL));
-        fv = wj * (Rb ? -u[P] : T(0.5) * (tR - u[P]));
-        Ax = Lb ? dL : T(0.5) * (dL + nL);
-        Bx = Rb ? -dR : T(-0.5) * (dR + nR);
-    }
-    {
-        T nD = 0, tD = 0, nU = 0, tU = 0;
-        const T *tlD = lds + (sl + BX) * TS;   // element below (y+1)
-        const T *tlU = lds + (sl - BX) * TS;   // element above (y-1)
-        if (!Db) {
+        T ev, fv, Ax, Bx, g, h, Cy, Dy;
+        {
+            T nL = 0, tL = 0, nR = 0, tR = 0;
+            if (!Lb) {
 #pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                const T t = tlD[j * N1 + k];
-                nD += DP[k] * t;
-                if (k == P) tD = t;
-            }
-        }
-        if (!Ub) {
-#pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                const T t = tlU[j * N1 + k];
-                nU += D0[k] * t;
-                if (k == 0) tU = t;
-            }
-        }
-        T dD = 0, dU = 0;
-#pragma unroll
-        for (int k = 0; k < N1; ++k) {
-            dD += D0[k] * v[k];
-            dU += DP[k] * v[k];
-        }
-        g = wj * (Db ? v[0] : T(0.5) * (v[0] - tD));
-        h = wj * (Ub ? -v[P] : T(0.5) * (tU - v[P]));
-        Cy = Db ? dD : T(0.5) * (dD + nD);
-        Dy = Ub ? -dU : T(-0.5) * (dU + nU);
-    }
-
-    // ---- both sum-factorised contractions, Yx[a] = (K1 U)[a][j] and
-    // Yy[a] = (U K1)[j][a] (K1 symmetric), in even-odd form: each constant
-    // of the half-size E/O blocks serves both directions while it is live.
-    T Yx[N1], Yy[N1];
-    {
-        constexpr cx::EvenOdd<P> EO = cx::make_even_odd<P>();
-        constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
-        T ue[ME], uo[H > 0 ? H : 1], ve[ME], vo[H > 0 ? H : 1];
-#pragma unroll
-        for (int i = 0; i < H; ++i) {
-            ue[i] = u[i] + u[P - i];
-            uo[i] = u[i] - u[P - i];
-            ve[i] = v[i] + v[P - i];
-            vo[i] = v[i] - v[P - i];
-        }
-        if (ME > H) { ue[ME - 1] = u[H]; ve[ME - 1] = v[H]; }
-#pragma unroll
-        for (int a = 0; a < ME; ++a) {
-            T ex_ = 0, ey_ = 0, ox_ = 0, oy_ = 0;
-#pragma unroll
-            for (int i = 0; i < ME; ++i) {
-                const T c = (T)EO.E[a][i];
-                ex_ += c * ue[i];
-                ey_ += c * ve[i];
-            }
-            if (a < H) {
-#pragma unroll
-                for (int i = 0; i < H; ++i) {
-                    const T c = (T)EO.O[a][i];
-                    ox_ += c * uo[i];
-                    oy_ += c * vo[i];
+                for (int i = 0; i < N1; ++i) {
+                    const T t = tlL[i * N1 + j];
+                    nL += DP[i] * t;
+                    if (i == P) tL = t;
                 }
-                Yx[a] = ex_ + ox_;
-                Yx[P - a] = ex_ - ox_;
-                Yy[a] = ey_ + oy_;
-                Yy[P - a] = ey_ - oy_;
-            } else {
-                Yx[a] = ex_;
-                Yy[a] = ey_;
             }
-        }
-    }
+            if (!Rb) {
 #pragma unroll
-    for (int a = 0; a < N1; ++a) {
-        Yx[a] = wj * Yx[a] + D0[a] * ev + DP[a] * fv;
-        Yy[a] = wj * Yy[a] + D0[a] * g + DP[a] * h;
-    }
-    Yx[0] += wj * Ax + (Lb ? pen : pen2) * ev;
-    Yx[P] += wj * Bx - (Rb ? pen : pen2) * fv;
-    Yy[0] += wj * Cy + (Db ? pen : pen2) * g;
-    Yy[P] += wj * Dy - (Ub ? pen : pen2) * h;
+                for (int i = 0; i < N1; ++i) {
+                    const T t = tlR[i * N1 + j];
+                    nR += D0[i] * t;
+                    if (i == 0) tR = t;
+                }
+            }
+            T dL = 0, dR = 0;
+#pragma unroll
+            for (int i = 0; i < N1; ++i) {
+                dL += D0[i] * u[i];
+                dR += DP[i] * u[i];
+            }
+            ev = wj * (Lb ? u[0] : T(0.5) * (u[0] - tL));
+            fv = wj * (Rb ? -u[P] : T(0.5) * (tR - u[P]));
+            Ax = Lb ? dL : T(0.5) * (dL + nL);
+            Bx = Rb ? -dR : T(-0.5) * (dR + nR);
+        }
+        {
+            T nD = 0, tD = 0, nU = 0, tU = 0;
+            if (!Db) {
+#pragma unroll
+                for (int k = 0; k < N1; ++k) {
+                    const T t = tlD[j * N1 + k];
+                    nD += DP[k] * t;
+                    if (k == P) tD = t;
+                }
+            }
+            if (!Ub) {
+#pragma unroll
+                for (int k = 0; k < N1; ++k) {
+                    const T t = tlU[j * N1 + k];
+                    nU += D0[k] * t;
+                    if (k == 0) tU = t;
+                }
+            }
+            T dD = 0, dU = 0;
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                dD += D0[k] * v[k];
+                dU += DP[k] * v[k];
+            }
+            g = wj * (Db ? v[0] : T(0.5) * (v[0] - tD));
+            h = wj * (Ub ? -v[P] : T(0.5) * (tU - v[P]));
+            Cy = Db ? dD : T(0.5) * (dD + nD);
+            Dy = Ub ? -dU : T(-0.5) * (dU + nU);
+        }
 
-    // ---- Y[a][b] = Yx (column j) + Yy (row j), assembled in the element's
-    // own LDS tile, then copied out tile by tile.
-    __syncthreads();                                   // all tile reads done
-    T *otile = lds + sl * TS;
+        // ---- both sum-factorised contractions, Yx[a] = (K1 U)[a][j] and
+        // Yy[a] = (U K1)[j][a] (K1 symmetric), in even-odd form: each constant
+        // of the half-size E/O blocks serves both directions while it is live.
+        T Yx[N1], Yy[N1];
+        {
+            constexpr cx::EvenOdd<P> EO = cx::make_even_odd<P>();
+            constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
+            T ue[ME], uo[H > 0 ? H : 1], ve[ME], vo[H > 0 ? H : 1];
 #pragma unroll
-    for (int a = 0; a < N1; ++a) otile[a * N1 + j] = Yx[a];
-    __syncthreads();
-    double dot = 0.0;
+            for (int i = 0; i < H; ++i) {
+                ue[i] = u[i] + u[P - i];
+                uo[i] = u[i] - u[P - i];
+                ve[i] = v[i] + v[P - i];
+                vo[i] = v[i] - v[P - i];
+            }
+            if (ME > H) { ue[ME - 1] = u[H]; ve[ME - 1] = v[H]; }
 #pragma unroll
-    for (int b = 0; b < N1; ++b) {
-        const T yv = otile[j * N1 + b] + Yy[b];
-        otile[j * N1 + b] = yv;
-        if (DOT) dot += (double)v[b] * (double)yv;
-    }
-    __syncthreads();
-    {
-        // copy the block's own rows out: BY row runs, linear as in staging
-        constexpr int NOUT = BY * RUN;
-        constexpr int KO = (NOUT + NT - 1) / NT;
-        T *yb = y + ((size_t)y0 * dimx + x0) * NQ;
-        const bool full = x0 + BX <= dimx && y0 + BY <= dimy;
+            for (int a = 0; a < ME; ++a) {
+                T ex_ = 0, ey_ = 0, ox_ = 0, oy_ = 0;
 #pragma unroll
-        for (int k = 0; k < KO; ++k) {
-            const int idx = tid + k * NT;
-            if ((k + 1) * NT <= NOUT || idx < NOUT) {
-                const int r = idx / RUN;
-                bool ok = true;
-                if (!full) ok = y0 + r < dimy && x0 + (idx - r * RUN) / NQ < dimx;
-                if (ok) yb[idx + r * GAP] = lds[lds_index<NQ, TS>(idx + RUN)];
+                for (int i = 0; i < ME; ++i) {
+                    const T c = (T)EO.E[a][i];
+                    ex_ += c * ue[i];
+                    ey_ += c * ve[i];
+                }
+                if (a < H) {
+#pragma unroll
+                    for (int i = 0; i < H; ++i) {
+                        const T c = (T)EO.O[a][i];
+                        ox_ += c * uo[i];
+                        oy_ += c * vo[i];
+                    }
+                    Yx[a] = ex_ + ox_;
+                    Yx[P - a] = ex_ - ox_;
+                    Yy[a] = ey_ + oy_;
+                    Yy[P - a] = ey_ - oy_;
+                } else {
+                    Yx[a] = ex_;
+                    Yy[a] = ey_;
+                }
             }
         }
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            Yx[a] = wj * Yx[a] + D0[a] * ev + DP[a] * fv;
+            Yy[a] = wj * Yy[a] + D0[a] * g + DP[a] * h;
+        }
+        Yx[0] += wj * Ax + (Lb ? pen : pen2) * ev;
+        Yx[P] += wj * Bx - (Rb ? pen : pen2) * fv;
+        Yy[0] += wj * Cy + (Db ? pen : pen2) * g;
+        Yy[P] += wj * Dy - (Ub ? pen : pen2) * h;
+
+        // ---- Y[a][b] = Yx (column j) + Yy (row j), assembled in the
+        // element's own LDS tile, then copied out as contiguous row runs.
+        __syncthreads();                                   // all tile reads done
+#pragma unroll
+        for (int a = 0; a < N1; ++a) otile[a * N1 + j] = Yx[a];
+        __syncthreads();
+        const bool valid = gx < dimx && gy < dimy;
+#pragma unroll
+        for (int c = 0; c < N1; ++c) {
+            const T yv = otile[j * N1 + c] + Yy[c];
+            otile[j * N1 + c] = yv;
+            if (DOT && valid) dot += (double)v[c] * (double)yv;
+        }
+        __syncthreads();
+        {
+            const int tid = opaque(threadIdx.x);
+            T *yb = y + ((size_t)y0 * dimx + x0) * NQ;
+            const bool full = x0 + BX <= dimx && y0 + BY <= dimy;
+#pragma unroll
+            for (int k = 0; k < KO; ++k) {
+                const int idx = tid + k * NT;
+                if ((k + 1) * NT <= NOUT || idx < NOUT) {
+                    const int r = idx / RUN;
+                    bool ok = true;
+                    if (!full) ok = y0 + r < dimy && x0 + (idx - r * RUN) / NQ < dimx;
+                    if (ok) yb[idx + r * GAP] = lds[lds_index<NQ, TS>(idx + RUN)];
+                }
+            }
+        }
+        __syncthreads();                                   // LDS free for the next block
     }
     if (DOT) {
-        const bool valid = (gx < dimx) && (gy < dimy);
-        if (!valid) dot = 0.0;
         const double bs = block_sum<NT>(dot, red);
         if (tid == 0) partials[blockIdx.x] = bs;
     }
 }
 
 namespace {
+
+// Workgroups of one kernel that the device holds at once: the persistent
+// grid.  Cached per device (occupancy depends only on the kernel).
+template <typename T, int P, bool DOT>
+int resident_groups() {
+    using S = ApplyShape<T, P>;
+    constexpr int kMaxDev = 64;
+    static std::atomic<int> cache[kMaxDev];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) dev = 0;
+    int g = cache[dev].load(std::memory_order_relaxed);
+    if (g > 0) return g;
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, reinterpret_cast<const void *>(&apply_kernel<T, P, S::BX, S::BY, DOT>),
+            S::BX * S::BY * (P + 1), 0) != hipSuccess)
+        per = 0;
+    g = (cus > 0 ? cus : 1) * (per > 0 ? per : 1);
+    cache[dev].store(g, std::memory_order_relaxed);
+    return g;
+}
 
 template <typename T, int P>
 int launch_one(const void *x, void *y, const void *tab, int dimx, int dimy, double pen,
@@ -374,12 +422,14 @@ int launch_one(const void *x, void *y, const void *tab, int dimx, int dimy, doub
     constexpr int BX = S::BX, BY = S::BY;
     const int nbx = (dimx + BX - 1) / BX, nby = (dimy + BY - 1) / BY;
     const int nblk = nbx * nby;
-    dim3 grid(nblk), block(BX * BY * (P + 1));
+    dim3 block(BX * BY * (P + 1));
     if (partials) {
+        dim3 grid(std::min(nblk, resident_groups<T, P, true>()));
         hipLaunchKernelGGL((apply_kernel<T, P, BX, BY, true>), grid, block, 0, s,
                            (const T *)x, (T *)y, dimx, dimy, nbx, nblk, (T)pen,
                            partials, st, slot);
     } else {
+        dim3 grid(std::min(nblk, resident_groups<T, P, false>()));
         hipLaunchKernelGGL((apply_kernel<T, P, BX, BY, false>), grid, block, 0, s,
                            (const T *)x, (T *)y, dimx, dimy, nbx, nblk, (T)pen,
                            partials, st, slot);
@@ -389,10 +439,12 @@ int launch_one(const void *x, void *y, const void *tab, int dimx, int dimy, doub
     return SIPG_OK;
 }
 
+// Grid size of the fused apply + p.Ap launch = number of p.Ap partials.
 template <typename T, int P>
 int nblocks_one(int dimx, int dimy) {
     using S = ApplyShape<T, P>;
-    return ((dimx + S::BX - 1) / S::BX) * ((dimy + S::BY - 1) / S::BY);
+    const int nblk = ((dimx + S::BX - 1) / S::BX) * ((dimy + S::BY - 1) / S::BY);
+    return std::min(nblk, resident_groups<T, P, true>());
 }
 
 template <typename T, int... Ps>

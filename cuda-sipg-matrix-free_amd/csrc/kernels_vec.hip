@@ -78,6 +78,14 @@ __global__ __launch_bounds__(NT) void axpy_kernel(T alpha, const T *__restrict__
         y[i] = y[i] + alpha * x[i];
 }
 
+// device copy of an existing vector (copy(), mode_vector.hpp:314-359,
+// without its reallocation)
+template <typename T>
+__global__ __launch_bounds__(NT) void copy_kernel(const T *__restrict__ x, T *__restrict__ y, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        y[i] = x[i];
+}
+
 // beta_kernel (conjugate_gradient_kernels.hpp:26-38): p = r + beta p
 template <typename T>
 __global__ __launch_bounds__(NT) void xpby_kernel(const T *__restrict__ r, T beta, T *__restrict__ p,
@@ -102,7 +110,7 @@ __global__ __launch_bounds__(NT) void alpha_kernel(T alpha, T *__restrict__ x, c
 __global__ __launch_bounds__(NT) void cg_init_finalize_kernel(const double *__restrict__ prr,
                                                               const double *__restrict__ pbb,
                                                               int np, cg_state *st, double tol,
-                                                              int tol_mode, int maxit) {
+                                                              int tol_mode, int maxit, int true_stop) {
     const double rr = reduce_partials_dev(prr, np);
     const double bb = reduce_partials_dev(pbb, np);
     if (threadIdx.x == 0) {
@@ -113,7 +121,7 @@ __global__ __launch_bounds__(NT) void cg_init_finalize_kernel(const double *__re
         s0.it = 0;
         s0.done = (sqrt(rr) > stop && maxit > 0) ? 0 : 1;   // while-condition of :69
         s0.checked = 1;                                     // r is the true residual here
-        s0.pad = 0;
+        s0.flags = 0;
         st->s[0] = s0;
         st->s[1] = s0;
         st->rr0 = rr;
@@ -123,7 +131,7 @@ __global__ __launch_bounds__(NT) void cg_init_finalize_kernel(const double *__re
         st->maxit = maxit;
         st->nrep = 0;
         st->err = (!isfinite(rr) || !isfinite(bb)) ? 3 : 0;
-        st->restart_p = 0;
+        st->true_stop = true_stop;
     }
 }
 
@@ -159,7 +167,14 @@ __global__ __launch_bounds__(NT) void cg_update_xr_kernel(T *__restrict__ x, con
 }
 
 // p = r + beta p, beta = r.r(new) / r.r(old) (beta_kernel and
-// conjugate_gradient_gpu.hpp:85-95); block 0 writes the next slot.
+// conjugate_gradient_gpu.hpp:85-95); block 0 writes the next slot.  When r
+// was replaced by the true residual in this iteration (kSlotReplaced on the
+// slot read), r.r(new) is the true residual's: the reliable update of van
+// der Vorst & Ye keeps the direction, p = r_true + beta p, unless
+// kSlotRestart asks for p = r.  With st->true_stop the iteration finishes
+// only on a true residual below the tolerance; a recursive one raises
+// kSlotWantCheck instead and the iteration goes on until the host inserts
+// the true residual.
 template <typename T>
 __global__ __launch_bounds__(NT) void cg_update_p_kernel(const T *__restrict__ r, T *__restrict__ p,
                                                          int64_t n, const double *__restrict__ prr,
@@ -170,23 +185,38 @@ __global__ __launch_bounds__(NT) void cg_update_p_kernel(const T *__restrict__ r
         return;
     }
     const double rr_new = reduce_partials_dev(prr, np);
-    const double beta_d = rr_new / cur.rr;
+    const bool replaced = (cur.flags & kSlotReplaced) != 0;
+    const double beta_d = (cur.flags & kSlotRestart) ? 0.0 : rr_new / cur.rr;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const bool met = !(sqrt(rr_new) > st->stop);
+        const bool need_true = st->true_stop && !replaced;
         cg_slot nx;
         nx.rr = rr_new;
         nx.alpha = 0.0;
         nx.beta = beta_d;
         nx.pAp = 0.0;
         nx.it = cur.it + 1;
-        nx.done = (!(sqrt(rr_new) > st->stop) || nx.it >= st->maxit) ? 1 : 0;
-        nx.checked = 0;
-        nx.pad = 0;
+        nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
+        nx.checked = replaced ? 1 : 0;
+        nx.flags = (met && need_true) ? kSlotWantCheck : 0;
         st->s[slot ^ 1] = nx;
+        if (replaced) st->rr_ref = rr_new;
         if (!isfinite(rr_new)) st->err = 3;
     }
     const T beta = (T)beta_d;
     for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
         p[i] = r[i] + beta * p[i];
+}
+
+// Runs between an iteration's x/r update and its p update, after r has
+// been overwritten with b - A x and the rr block sums with its own.
+__global__ void mark_replaced_kernel(cg_state *st, int slot, int restart) {
+    if (threadIdx.x != 0) return;
+    cg_slot c = st->s[slot];
+    if (c.done | st->err) return;
+    c.flags |= kSlotReplaced | (restart ? kSlotRestart : 0);
+    st->s[slot] = c;
+    st->nrep += 1;
 }
 
 // out = (TO)(b - Ax); optional p = out; block sums of (b-Ax)^2 and b^2.
@@ -223,44 +253,6 @@ __global__ __launch_bounds__(NT) void accumulate_kernel(TO *__restrict__ x, TL *
         x[i] += (TO)xl[i];
         xl[i] = TL(0);
     }
-}
-
-// After r has been replaced by the true residual b - A x: put its r.r in the
-// slot the next iteration reads, then either confirm the finish (tolerance
-// met, or out of iterations) or resume a recursion that stopped early.  The
-// old search direction is kept only while the true residual agrees with the
-// recursion (within 2x): a larger gap, or a recursion that had already
-// converged (its direction is rounding noise by then), continues from
-// p = r instead -- keeping p there makes the iteration diverge.
-__global__ __launch_bounds__(NT) void replace_finalize_kernel(const double *__restrict__ prr, int np,
-                                                              cg_state *st, int slot) {
-    if (st->err) return;
-    const double rr = reduce_partials_dev(prr, np);
-    if (threadIdx.x == 0) {
-        cg_slot c = st->s[slot];
-        st->restart_p = (rr > 4.0 * c.rr || c.done) ? 1 : 0;
-        c.rr = rr;
-        st->rr_ref = rr;
-        st->nrep += 1;
-        if (!isfinite(rr)) {
-            st->err = 3;
-        } else if (!(sqrt(rr) > st->stop) || c.it >= st->maxit) {
-            c.done = 1;
-            c.checked = 1;
-        } else {
-            c.done = 0;
-            c.checked = 0;
-        }
-        st->s[slot] = c;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(NT) void restart_p_kernel(const T *__restrict__ r, T *__restrict__ p,
-                                                       int64_t n, const cg_state *st) {
-    if (!st->restart_p || st->err) return;
-    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
-        p[i] = r[i];
 }
 
 template <typename T>
@@ -364,6 +356,15 @@ int launch_reduce_partials(const double *partials, int np, double *out, hipStrea
     return SIPG_OK;
 }
 
+int launch_copy(int dtype, const void *x, void *y, int64_t n, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(copy_kernel<double>, dim3(nb), dim3(NT), 0, s, (const double *)x, (double *)y, n),
+        hipLaunchKernelGGL(copy_kernel<float>, dim3(nb), dim3(NT), 0, s, (const float *)x, (float *)y, n));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
 int launch_axpy(int dtype, double alpha, const void *x, void *y, int64_t n, hipStream_t s) {
     const int nb = vec_num_blocks(n);
     DISPATCH_T(dtype,
@@ -393,9 +394,9 @@ int launch_alpha_update(int dtype, double alpha, void *x, const void *p, void *r
 }
 
 int launch_cg_init_finalize(const double *part_rr, const double *part_bb, int np, cg_state *st,
-                            double tol, int tol_mode, int maxit, hipStream_t s) {
+                            double tol, int tol_mode, int maxit, int true_stop, hipStream_t s) {
     hipLaunchKernelGGL(cg_init_finalize_kernel, dim3(1), dim3(NT), 0, s, part_rr, part_bb, np, st, tol,
-                       tol_mode, maxit);
+                       tol_mode, maxit, true_stop);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }
@@ -456,17 +457,8 @@ int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, h
     return SIPG_OK;
 }
 
-int launch_replace_finalize(const double *prr, int np, cg_state *st, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(replace_finalize_kernel, dim3(1), dim3(NT), 0, s, prr, np, st, slot);
-    SIPG_HIP_TRY(hipGetLastError());
-    return SIPG_OK;
-}
-
-int launch_restart_p(int dtype, const void *r, void *p, int64_t n, const cg_state *st, hipStream_t s) {
-    const int nb = vec_num_blocks(n);
-    DISPATCH_T(dtype,
-        hipLaunchKernelGGL(restart_p_kernel<double>, dim3(nb), dim3(NT), 0, s, (const double *)r, (double *)p, n, st),
-        hipLaunchKernelGGL(restart_p_kernel<float>, dim3(nb), dim3(NT), 0, s, (const float *)r, (float *)p, n, st));
+int launch_mark_replaced(cg_state *st, int slot, int restart, hipStream_t s) {
+    hipLaunchKernelGGL(mark_replaced_kernel, dim3(1), dim3(64), 0, s, st, slot, restart);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

@@ -55,9 +55,17 @@ struct cg_slot {
     double pAp;
     int it;         // iterations completed
     int done;       // 0 running, 1 finished (tolerance met or iteration cap)
-    int checked;    // 1 when `done` was confirmed by a true residual b - A x
-    int pad;
+    int checked;    // 1 when rr is a true residual b - A x
+    int flags;      // kSlot* bits
 };
+// cg_slot::flags.  kSlotReplaced / kSlotRestart are set on the slot an
+// iteration reads, between its x/r update and its p update, when r was
+// just replaced by the true residual (and p is to restart from r);
+// kSlotWantCheck is set by the p update when the recursive residual meets
+// the tolerance but a true residual is required to finish.
+constexpr int kSlotReplaced = 1;
+constexpr int kSlotRestart = 2;
+constexpr int kSlotWantCheck = 4;
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b
@@ -67,7 +75,7 @@ struct cg_state {
     int maxit;
     int err;         // 0, 2 breakdown (p.Ap <= 0), 3 non-finite -- terminal
     int nrep;        // true-residual recomputations so far
-    int restart_p;   // set by a true residual that disagrees with the recursion
+    int true_stop;   // 1: finishing requires a true residual below the tolerance
 };
 
 // Reduction partial buffers hold at most this many block sums.
@@ -88,12 +96,13 @@ int launch_dot_partials(int dtype, const void *a, const void *b, int64_t n, doub
                         int nblk, hipStream_t s);
 int launch_reduce_partials(const double *partials, int np, double *out, hipStream_t s);
 int launch_axpy(int dtype, double alpha, const void *x, void *y, int64_t n, hipStream_t s);
+int launch_copy(int dtype, const void *x, void *y, int64_t n, hipStream_t s);
 int launch_xpby(int dtype, const void *r, double beta, void *p, int64_t n, hipStream_t s);
 int launch_alpha_update(int dtype, double alpha, void *x, const void *p, void *r, const void *Ap,
                         int64_t n, hipStream_t s);
 // CG pieces with device-resident scalars
 int launch_cg_init_finalize(const double *part_rr, const double *part_bb, int np, cg_state *st,
-                            double tol, int tol_mode, int maxit, hipStream_t s);
+                            double tol, int tol_mode, int maxit, int true_stop, hipStream_t s);
 int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *Ap, int64_t n,
                         const double *part_pAp, int np_pAp, double *part_rr, int nblk,
                         cg_state *st, int slot, hipStream_t s);
@@ -106,11 +115,10 @@ int launch_true_residual(int in_dtype, int out_dtype, const void *b, const void 
                          void *p_copy, int64_t n, double *prr, double *pbb, int nblk, hipStream_t s);
 // x += xl; xl = 0   (x in x_dtype, xl in xl_dtype; f64+=f64, f64+=f32, f32+=f32)
 int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t s);
-// Folds the true-residual block sums into the state: records r.r in the
-// live slot, confirms convergence or resumes the iteration.
-int launch_replace_finalize(const double *prr, int np, cg_state *st, int slot, hipStream_t s);
-// p = r when the last true residual asked for a restart (st->restart_p)
-int launch_restart_p(int dtype, const void *r, void *p, int64_t n, const cg_state *st, hipStream_t s);
+// Marks the slot an iteration reads as replaced (r is now b - A x and the
+// rr block sums are its own), optionally with a restart p = r; a no-op
+// for a finished slot.
+int launch_mark_replaced(cg_state *st, int slot, int restart, hipStream_t s);
 int launch_convert(int from_dtype, int to_dtype, const void *src, void *dst, int64_t n,
                    hipStream_t s);
 // reference order <-> element-major device order (to_device = 1: ref -> dev)

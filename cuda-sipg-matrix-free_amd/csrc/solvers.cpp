@@ -36,7 +36,8 @@ struct Engine {
     const void *b = nullptr;         // right-hand side in outer precision
     void *AxO = nullptr;             // scratch for A x in outer precision
     bool mixed = false;              // fold xI into xO at each true residual
-    bool restart = false;            // p = r at each true residual
+    int restart = 0;                 // p = r at a true residual: 0 never, 1 always,
+                                     // 2 once the true residual stagnates
     double replace_factor = 0.0;
     bool true_stop = false;
     int maxit = 0;
@@ -58,10 +59,15 @@ struct Engine {
         if (mixed) SIPG_HIP_TRY(hipMemsetAsync(xI, 0, (size_t)opI->n * dtype_size(opI->dtype), s));
         if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, p, opI->n, P.rr, P.bb, nb, s)))
             return rc;
-        return launch_cg_init_finalize(P.rr, P.bb, nb, st, tol, tol_mode, maxit, s);
+        return launch_cg_init_finalize(P.rr, P.bb, nb, st, tol, tol_mode, maxit, true_stop ? 1 : 0, s);
     }
 
-    int iteration() {
+    // One CG iteration.  With `replace`, r is recomputed as b - A x between
+    // the x/r update and the p update (after folding the accumulated
+    // increments into x), so the p update forms beta and p from the true
+    // residual: the reliable update of van der Vorst & Ye.  `restart`
+    // makes that p update restart from p = r.
+    int iteration(bool replace, bool restart_p) {
         const int slot = (int)(launched & 1);
         int rc;
         if ((rc = launch_apply(opI->dtype, opI->p, p, Ap, opI->d_tab, opI->dimx, opI->dimy, opI->pen,
@@ -76,24 +82,19 @@ struct Engine {
         }
         if ((rc = launch_cg_update_xr(opI->dtype, xI, p, r, Ap, opI->n, src, np, P.rr, nb, st, slot, s)))
             return rc;
+        if (replace) {
+            if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s))) return rc;
+            if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
+                                   opO->pen, nullptr, nullptr, 0, s)))
+                return rc;
+            if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, nullptr, opI->n, P.rr,
+                                           nullptr, nb, s)))
+                return rc;
+            if ((rc = launch_mark_replaced(st, slot, restart_p ? 1 : 0, s))) return rc;
+        }
         if ((rc = launch_cg_update_p(opI->dtype, r, p, opI->n, P.rr, nb, nb, st, slot, s))) return rc;
         ++launched;
         return SIPG_OK;
-    }
-
-    // r <- b - A x (true residual) for the slot the next iteration reads
-    int replacement() {
-        const int slot = (int)(launched & 1);
-        int rc;
-        if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s))) return rc;
-        if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
-                               opO->pen, nullptr, nullptr, 0, s)))
-            return rc;
-        if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, restart ? p : nullptr,
-                                       opI->n, P.rr, nullptr, nb, s)))
-            return rc;
-        if ((rc = launch_replace_finalize(P.rr, nb, st, slot, s))) return rc;
-        return restart ? SIPG_OK : launch_restart_p(opI->dtype, r, p, opI->n, st, s);
     }
 
     int run(sipg_cg_info *info) {
@@ -110,34 +111,48 @@ struct Engine {
         int chunk = 4;
         int inflight = 0, buf = 0;
         int64_t pending_launched[2] = {0, 0};
+        // adaptive restart: the direction is kept while each true residual
+        // is well below the previous one (the recursion and the true
+        // residual agree); once a replacement interval gains less than 2x in
+        // the true residual norm, later replacements restart from p = r
+        bool restarting = restart == 1;
+        int seen_rep = 0;
+        double last_ref = cur.rr_ref;
         // Each pass: decide on the newest polled state `cur`, enqueue one
-        // chunk of iterations and a state copy, and block on the older of
-        // two outstanding copies.  Iterations past the finish are no-ops on
+        // chunk of iterations (4 growing to 32; 8 near the tolerance) and a
+        // state copy, and block on the older of two outstanding copies.  Iterations past the finish are no-ops on
         // the device, so running one or two chunks ahead costs little.
         for (;;) {
             const cg_slot cs = cur.s[cur_launched & 1];
-            if (cur.err) break;
+            if (cur.err || cs.done) break;      // done is final: later iterations are no-ops
+            // a replacement is decided only once the previous one shows in
+            // the polled state, so each is based on an up-to-date rr_ref
             const bool rep_visible = cur.nrep == nrep_enq;
-            if (rep_visible && cs.done && (cs.checked || !true_stop)) break;   // final
-            bool want_rep = false;
+            if (cur.nrep > seen_rep) {
+                if (restart == 2 && cur.rr_ref > 0.25 * last_ref) restarting = true;
+                last_ref = cur.rr_ref;
+                seen_rep = cur.nrep;
+            }
+            bool want_rep = false, restart_p = restarting;
             if (rep_visible) {
-                if (true_stop && cs.done && !cs.checked) want_rep = true;
-                if (!cs.done && replace_factor > 0.0 &&
-                    cs.rr <= replace_factor * replace_factor * cur.rr_ref)
+                if (true_stop && (cs.flags & kSlotWantCheck)) want_rep = true;
+                if (replace_factor > 0.0 && cs.rr <= replace_factor * replace_factor * cur.rr_ref)
                     want_rep = true;
                 // divergence guard: a recursion that grew 100x past the last
-                // true residual is recomputed (and restarted, see finalize)
-                if (!cs.done && (replace_factor > 0.0 || true_stop) && cs.rr > 1e4 * cur.rr_ref)
+                // true residual is recomputed and restarted from p = r
+                if ((replace_factor > 0.0 || true_stop) && cs.rr > 1e4 * cur.rr_ref) {
                     want_rep = true;
+                    restart_p = true;
+                }
             }
-            if (want_rep) {
-                if ((rc = replacement())) break;
-                ++nrep_enq;
-            }
-            for (int c = 0; c < chunk; ++c)
-                if ((rc = iteration())) break;
+            // near the tolerance, short chunks keep the host's lag small
+            const bool near = true_stop && cs.rr <= 1e4 * cur.stop * cur.stop;
+            const int n_it = near ? std::min(chunk, 8) : chunk;
+            for (int c = 0; c < n_it; ++c)
+                if ((rc = iteration(want_rep && c == 0, restart_p))) break;
             if (rc) break;
-            if (chunk < 64) chunk *= 2;
+            if (want_rep) ++nrep_enq;
+            if (chunk < 32) chunk *= 2;
             SIPG_HIP_TRY(hipMemcpyAsync(&hst[buf], st, sizeof(cg_state), hipMemcpyDeviceToHost, s));
             SIPG_HIP_TRY(hipEventRecord(ev[buf], s));
             pending_launched[buf] = launched;
@@ -164,7 +179,7 @@ struct Engine {
         int status = SIPG_OK;
         if (f.err == 2) status = SIPG_ERR_BREAKDOWN;
         else if (f.err == 3) status = SIPG_ERR_NONFINITE;
-        else if (!(std::sqrt(fs.rr) <= f.stop)) status = SIPG_NOT_CONVERGED;
+        else if (!(std::sqrt(fs.rr) <= f.stop) || (true_stop && !fs.checked)) status = SIPG_NOT_CONVERGED;
         if (info) {
             info->status = status;
             info->iters = fs.it;
@@ -182,6 +197,7 @@ int check_opts(const sipg_cg_opts *o) {
     if (!(o->tol >= 0.0) || (o->tol_mode != SIPG_TOL_ABS && o->tol_mode != SIPG_TOL_REL))
         return SIPG_ERR_INVALID;
     if (!(o->replace_factor >= 0.0 && o->replace_factor < 1.0)) return SIPG_ERR_INVALID;
+    if (o->restart < 0 || o->restart > 2) return SIPG_ERR_INVALID;
     return SIPG_OK;
 }
 
@@ -209,7 +225,7 @@ int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg
     E.b = b;
     E.replace_factor = o->replace_factor;
     E.true_stop = o->true_residual_stop != 0;
-    E.restart = o->restart != 0;
+    E.restart = o->restart;
     E.maxit = (int)std::min<int64_t>(o->maxit > 0 ? (int64_t)o->maxit : op->n, INT_MAX);
     E.s = op->stream;
     E.st = op->d_state;
@@ -272,8 +288,8 @@ int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts,
     E.b = b;
     E.AxO = op->ws;
     E.mixed = true;
-    E.restart = opts->restart != 0;
-    E.replace_factor = opts->replace_factor > 0.0 ? opts->replace_factor : 0.01;
+    E.restart = opts->restart;
+    E.replace_factor = opts->replace_factor > 0.0 ? opts->replace_factor : 0.1;
     E.true_stop = true;
     E.maxit = (int)std::min<int64_t>(opts->maxit > 0 ? (int64_t)opts->maxit : op->n, INT_MAX);
     E.s = op->stream;

@@ -95,6 +95,7 @@ def _declare(L):
         "sipg_cg_mixed": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
         "sipg_vec_dot": (i, [i, i64, _vp, _vp, _dp, _vp]),
         "sipg_vec_axpy": (i, [i, i64, d, _vp, _vp, _vp]),
+        "sipg_vec_copy": (i, [i, i64, _vp, _vp, _vp]),
         "sipg_vec_xpby": (i, [i, i64, _vp, d, _vp, _vp]),
         "sipg_vec_alpha_update": (i, [i, i64, d, _vp, _vp, _vp, _vp, _vp]),
         "sipg_forcing_draw": (i, [ctypes.c_uint64, i, i, i, _ip, _dp, _ip, _ip]),
@@ -261,13 +262,13 @@ class Operator:
 
     def cg_ex(self, x, b, tol, tol_mode=TOL_REL, maxit=0, replace_factor=0.0, true_stop=True,
               restart=False):
-        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1 if true_stop else 0, 1 if restart else 0)
+        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1 if true_stop else 0, int(restart))
         info = CgInfo()
         st = lib().sipg_cg_ex(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
         return st, info
 
     def cg_mixed(self, x, b, tol, tol_mode=TOL_REL, maxit=0, replace_factor=0.0, restart=True):
-        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1, 1 if restart else 0)
+        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1, int(restart))
         info = CgInfo()
         st = lib().sipg_cg_mixed(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
         return st, info

@@ -116,6 +116,9 @@ int sipg_alpha_update(sipg_op *op, double alpha, void *x, const void *p, void *r
 int sipg_vec_dot(int dtype, int64_t n, const void *a, const void *b, double *result, void *stream);
 int sipg_vec_axpy(int dtype, int64_t n, double alpha, const void *x, void *y, void *stream);
 int sipg_vec_xpby(int dtype, int64_t n, const void *r, double beta, void *p, void *stream);
+/* y = x on the device (copy(), mode_vector.hpp:314-359, into an existing
+ * buffer of the same length; overlapping buffers rejected) */
+int sipg_vec_copy(int dtype, int64_t n, const void *x, void *y, void *stream);
 int sipg_vec_alpha_update(int dtype, int64_t n, double alpha, void *x, const void *p, void *r,
                           const void *Ap, void *stream);
 
@@ -134,20 +137,25 @@ int sipg_cg(sipg_op *op, void *x, const void *b, double tol, int tol_mode, int m
 /* Extended CG.  Same iteration as sipg_cg plus, optionally, residual
  * replacement (van der Vorst & Ye 2000): whenever the recursive sqrt(r.r)
  * has fallen by `replace_factor` since the last true residual, r is
- * recomputed as b - A x and the iteration continues with the same search
- * direction; with `true_residual_stop` the finish is confirmed on the true
- * residual (a recursion that stopped early resumes).  This has no
- * counterpart in the reference, whose CG stops on the recursive residual
- * only; it is what makes a 1e-10 relative residual reachable at kappa ~ 1e8. */
+ * recomputed as b - A x between the x/r and the p update of an iteration
+ * (the increments since the last one are kept apart and folded into x
+ * there); with `true_residual_stop` the iteration finishes only on a true
+ * residual below the tolerance (once the recursive one meets it, the true
+ * residual is inserted within a few iterations).  This has no counterpart
+ * in the reference, whose CG stops on the recursive residual only; it is
+ * what makes a relative residual near the FP64 floor reachable. */
 typedef struct {
     double tol;              /* stopping tolerance                              */
     int tol_mode;            /* SIPG_TOL_ABS or SIPG_TOL_REL                    */
     int maxit;               /* <= 0: n                                         */
     double replace_factor;   /* in (0,1): recompute r at each such drop; 0: off */
     int true_residual_stop;  /* 1: confirm convergence on b - A x               */
-    int restart;             /* 1: also reset the search direction p = r at each
-                                true residual (iterative refinement: a fresh
-                                Krylov space per cycle); 0: keep p            */
+    int restart;             /* 0: keep the search direction at each true
+                                residual (reliable update); 1: restart from
+                                p = r at each (iterative refinement, a fresh
+                                Krylov space per cycle); 2: keep it while each
+                                true residual improves on the previous one by
+                                2x or more, then restart at every later one  */
 } sipg_cg_opts;
 
 typedef struct {
@@ -165,13 +173,14 @@ int sipg_cg_ex(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, si
 /* Mixed-precision solve (no counterpart in the reference): the CG
  * iteration runs in FP32 (operator apply, vector updates, FP64-accumulated
  * dots) on an FP32 twin of `op`; each time the FP32 residual has fallen by
- * `replace_factor` (default 0.01 when 0 is passed) the FP32 iterate is
+ * `replace_factor` (default 0.1 when 0 is passed) the FP32 iterate is
  * folded into the FP64 solution and the true residual b - A x is
  * recomputed in FP64 (defect correction).  With opts->restart = 1 each
  * cycle restarts the FP32 CG from that residual (iterative refinement);
  * with 0 the search direction is kept (a reliable update, Sleijpen & van der
  * Vorst 1996; Clark et al. 2010), which is cheaper while eps32 * kappa(A)
- * is small but loses convergence once it is not.  The final convergence
+ * is small but loses convergence once it is not; 2 switches from the first
+ * to the second when the true residual stagnates.  The final convergence
  * test is on the FP64 true residual.  op must be SIPG_F64; x (in/out) and b
  * are FP64 device vectors. */
 int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);

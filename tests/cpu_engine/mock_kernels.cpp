@@ -83,6 +83,11 @@ int launch_reduce_partials(const double *partials, int np, double *out, hipStrea
 template <typename T> static void axpy_t(T a, const T *x, T *y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = y[i] + a * x[i];
 }
+int launch_copy(int dtype, const void *x, void *y, int64_t n, hipStream_t) {
+    std::memcpy(y, x, (size_t)n * (dtype == SIPG_F64 ? 8 : 4));
+    return SIPG_OK;
+}
+
 int launch_axpy(int dtype, double alpha, const void *x, void *y, int64_t n, hipStream_t) {
     if (dtype == SIPG_F64) axpy_t<double>(alpha, tp<double>(x), tp<double>(y), n);
     else axpy_t<float>((float)alpha, tp<float>(x), tp<float>(y), n);
@@ -109,7 +114,7 @@ int launch_alpha_update(int dtype, double alpha, void *x, const void *p, void *r
 }
 
 int launch_cg_init_finalize(const double *prr, const double *pbb, int np, cg_state *st, double tol,
-                            int tol_mode, int maxit, hipStream_t) {
+                            int tol_mode, int maxit, int true_stop, hipStream_t) {
     const double rr = sum_partials(prr, np), bb = sum_partials(pbb, np);
     const double stop = tol_mode == SIPG_TOL_REL ? tol * std::sqrt(bb) : tol;
     cg_slot s0{};
@@ -125,7 +130,7 @@ int launch_cg_init_finalize(const double *prr, const double *pbb, int np, cg_sta
     st->maxit = maxit;
     st->nrep = 0;
     st->err = (!std::isfinite(rr) || !std::isfinite(bb)) ? 3 : 0;
-    st->restart_p = 0;
+    st->true_stop = true_stop;
     return SIPG_OK;
 }
 
@@ -162,13 +167,19 @@ static void update_p_t(const T *r, T *p, int64_t n, const double *prr, int np, c
     const cg_slot cur = st->s[slot];
     if (cur.done | st->err) { st->s[slot ^ 1] = cur; return; }
     const double rr_new = sum_partials(prr, np);
-    const double beta_d = rr_new / cur.rr;
+    const bool replaced = (cur.flags & kSlotReplaced) != 0;
+    const double beta_d = (cur.flags & kSlotRestart) ? 0.0 : rr_new / cur.rr;
+    const bool met = !(std::sqrt(rr_new) > st->stop);
+    const bool need_true = st->true_stop && !replaced;
     cg_slot nx{};
     nx.rr = rr_new;
     nx.beta = beta_d;
     nx.it = cur.it + 1;
-    nx.done = (!(std::sqrt(rr_new) > st->stop) || nx.it >= st->maxit) ? 1 : 0;
+    nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
+    nx.checked = replaced ? 1 : 0;
+    nx.flags = (met && need_true) ? kSlotWantCheck : 0;
     st->s[slot ^ 1] = nx;
+    if (replaced) st->rr_ref = rr_new;
     if (!std::isfinite(rr_new)) st->err = 3;
     const T b = (T)beta_d;
     for (int64_t i = 0; i < n; ++i) p[i] = r[i] + b * p[i];
@@ -229,24 +240,12 @@ int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, h
     return SIPG_OK;
 }
 
-int launch_replace_finalize(const double *prr, int np, cg_state *st, int slot, hipStream_t) {
-    if (st->err) return SIPG_OK;
-    const double rr = sum_partials(prr, np);
+int launch_mark_replaced(cg_state *st, int slot, int restart, hipStream_t) {
     cg_slot c = st->s[slot];
-    st->restart_p = (rr > 4.0 * c.rr || c.done) ? 1 : 0;
-    c.rr = rr;
-    st->rr_ref = rr;
-    st->nrep += 1;
-    if (!std::isfinite(rr)) st->err = 3;
-    else if (!(std::sqrt(rr) > st->stop) || c.it >= st->maxit) { c.done = 1; c.checked = 1; }
-    else { c.done = 0; c.checked = 0; }
+    if (c.done | st->err) return SIPG_OK;
+    c.flags |= kSlotReplaced | (restart ? kSlotRestart : 0);
     st->s[slot] = c;
-    return SIPG_OK;
-}
-
-int launch_restart_p(int dtype, const void *r, void *p, int64_t n, const cg_state *st, hipStream_t) {
-    if (!st->restart_p || st->err) return SIPG_OK;
-    std::memcpy(p, r, (size_t)n * (dtype == SIPG_F64 ? 8 : 4));
+    st->nrep += 1;
     return SIPG_OK;
 }
 

@@ -43,15 +43,25 @@ def test_iteration_cap(results):
     assert r["rc"] == 6 and r["iters"] == 5 and r["diff"] <= ROUNDING
 
 
+# With a true-residual stop the device keeps iterating after the recursive
+# residual meets the tolerance until the host has inserted the true
+# residual; the host polls one chunk (at most 8 iterations near the
+# tolerance, 32 elsewhere) behind.
+LAG = 48
+
+
 def test_replacement_reaches_true_residual(results):
+    # reliable updates keep the search direction: the iteration count stays
+    # that of the reference iteration, plus the polling lag
     r = results["ex_replace"]
     assert r["rc"] == 0 and r["reps"] >= 2 and r["relres"] <= 1e-10
-    assert abs(r["iters"] - r["oracle_iters"]) <= 0.05 * r["oracle_iters"]
+    assert r["oracle_iters"] <= r["iters"] <= r["oracle_iters"] + LAG
 
 
-def test_no_replacement_matches_plain(results):
+def test_true_residual_stop_without_periodic_replacement(results):
     r = results["ex_noreplace"]
-    assert r["rc"] == 0 and r["diff"] <= ROUNDING and r["iters"] == r["oracle_iters"]
+    assert r["rc"] == 0 and r["reps"] == 1 and r["relres"] <= 1e-10
+    assert r["oracle_iters"] <= r["iters"] <= r["oracle_iters"] + LAG
 
 
 @pytest.mark.parametrize("case", ["mixed", "mixed_keep_p"])

@@ -242,3 +242,8 @@ def test_operator_free_vector_kernels(sipg):
     sipg.check(sipg.lib().sipg_vec_axpy(sipg.F64, n, 2.0, A.ptr, B.ptr, None), "vec_axpy")
     sipg.synchronize()
     assert np.array_equal(B.download(np.float64), b + 2.0 * a)
+    C = sipg.DeviceBuffer(8 * n)
+    sipg.check(sipg.lib().sipg_vec_copy(sipg.F64, n, A.ptr, C.ptr, None), "vec_copy")
+    sipg.synchronize()
+    assert np.array_equal(C.download(np.float64), a)
+    assert sipg.lib().sipg_vec_copy(sipg.F64, n, A.ptr, A.ptr, None) == sipg.ERR_ALIAS
