@@ -135,7 +135,12 @@ struct Engine {
             }
             bool want_rep = false, restart_p = restarting;
             if (rep_visible) {
-                if (true_stop && (cs.flags & kSlotWantCheck)) want_rep = true;
+                // the recursion met the tolerance: check it on a true
+                // residual, but not before it has fallen 4x below the last
+                // one -- near the FP64 floor the true residual lags the
+                // recursion, and replacing at every small drop stalls it
+                if (true_stop && (cs.flags & kSlotWantCheck) && cs.rr <= 0.0625 * cur.rr_ref)
+                    want_rep = true;
                 if (replace_factor > 0.0 && cs.rr <= replace_factor * replace_factor * cur.rr_ref)
                     want_rep = true;
                 // divergence guard: a recursion that grew 100x past the last
