@@ -1,0 +1,363 @@
+// apply_block.hpp -- the device pieces of the fused SIPG operator shared by
+// the single-mesh kernel (kernels_apply.hip) and the batched kernel
+// (kernels_batch.hip): block geometry, the register prefetch of a block and
+// its halo ring, the LDS stage, and the operator on a staged block.
+#pragma once
+#include "sipg_internal.hpp"
+#include "basis_constexpr.hpp"
+
+namespace sipg_internal {
+
+// BX x BY elements per workgroup, NE = BX*BY, threads = NE*(P+1) <= 1024,
+// LDS = (P+1)^2 * (NE + 2BX + 2BY) * sizeof(T) <= 160 KiB.
+template <typename T, int P>
+struct ApplyShape {
+    static constexpr int N1 = P + 1;
+    static constexpr bool big64 = sizeof(T) == 8 && P >= 13;
+    static constexpr bool small = N1 > 16;
+    static constexpr int BX = (big64 || small) ? 8 : 16;
+    static constexpr int BY = 4;
+};
+
+__device__ __forceinline__ int xcd_remap(int b, int nblk) {
+    // workgroup id b runs on XCD (b % 8); give XCD k the k-th contiguous
+    // share of the logical block range.
+    const int xcd = b & 7;
+    const int idx = b >> 3;
+    const int q = nblk >> 3, r = nblk & 7;
+    const int start = xcd * q + (xcd < r ? xcd : r);
+    return start + idx;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    // wave64 butterfly, then a fixed-order sum over waves: deterministic.
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    constexpr int NW = (NT + 63) / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NW; ++i) s += red[i];
+    }
+    return s;   // valid in thread 0
+}
+
+// Odd LDS row stride (in elements of T) for a (p+1)^2 tile: lanes that read
+// the same node of consecutive tiles then hit distinct banks.
+__host__ __device__ constexpr int tile_stride(int nq) { return (nq % 2) ? nq : nq + 1; }
+
+// A copy of v that the compiler cannot see through.  The block loop of the
+// apply recomputes the staging index arithmetic from it in every pass, which
+// is cheaper than holding ~20 loop-invariant offsets in registers across
+// the compute phase.  (A register move; no memory access.)
+__device__ __forceinline__ int opaque(int v) {
+    int r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+// LDS position of the idx-th value of consecutive tiles stored with row
+// stride TS >= NQ.
+template <int NQ, int TS>
+__device__ __forceinline__ int lds_index(int idx) {
+    if constexpr (TS == NQ) return idx;
+    else return idx + (TS - NQ) * (idx / NQ);
+}
+
+// Compile-time geometry of a BX x BY block of degree-P elements.  LDS slots:
+// (BY+2) rows of BX tiles for the element rows y0-1 .. y0+BY (slot r*BX + t
+// holds element (x0+t, y0-1+r)), then the left halo column (x0-1, y0+k) at
+// SL_COL+k and the right one (x0+BX, y0+k) at SL_COL+BY+k.  A block's own
+// element (ex, ey) sits in slot (ey+1)*BX+ex, so its up/down neighbours are
+// always slot -/+ BX.
+template <typename T, int P, int BX, int BY>
+struct BlockGeo {
+    static constexpr int N1 = P + 1;
+    static constexpr int NQ = N1 * N1;
+    static constexpr int TS = tile_stride(NQ);
+    static constexpr int NE = BX * BY;
+    static constexpr int RUN = BX * NQ;                 // values of one row run
+    static constexpr int NROWV = (BY + 2) * RUN;
+    static constexpr int SL_COL = (BY + 2) * BX;
+    static constexpr int NCOLV = 2 * BY * NQ;
+    static constexpr int NS = SL_COL + 2 * BY;
+    static constexpr int NT = NE * N1;                  // threads
+    static constexpr int KR = (NROWV + NT - 1) / NT;    // row values per thread
+    static constexpr int KC = (NCOLV + NT - 1) / NT;    // halo-column values per thread
+    static constexpr int NOUT = BY * RUN;
+    static constexpr int KO = (NOUT + NT - 1) / NT;
+    static constexpr int LDS = NS * TS;                 // elements of T
+};
+
+// A block's values in flight between the global loads and the LDS stage.
+template <typename T, int P, int BX, int BY>
+struct BlockRegs {
+    T buf[BlockGeo<T, P, BX, BY>::KR];
+    T cbuf[BlockGeo<T, P, BX, BY>::KC];
+};
+
+// Loads block (x0, y0) of a dimx x dimy element-major vector x into
+// registers: row value idx lies in run r = idx / RUN at offset
+// idx + r*(RS - RUN) - RS from the block's first tile (RS = dimx*(p+1)^2);
+// halo-column value idx is node idx % NQ of column tile idx / NQ.  Values
+// outside the mesh are zero.  Interior blocks take a path without checks.
+template <typename T, int P, int BX, int BY>
+__device__ __forceinline__ void block_prefetch(BlockRegs<T, P, BX, BY> &R, const T *__restrict__ x,
+                                               int dimx, int dimy, int x0, int y0)
+{
+    using G = BlockGeo<T, P, BX, BY>;
+    T *buf = R.buf, *cbuf = R.cbuf;
+    constexpr int NQ = G::NQ, NT = G::NT, RUN = G::RUN, NROWV = G::NROWV, NCOLV = G::NCOLV;
+    const int tid = opaque(threadIdx.x);
+    const int RS = dimx * NQ, GAP = RS - RUN;
+    const T *xb = x + ((size_t)y0 * dimx + x0) * NQ;
+    const bool interior = x0 >= 1 && x0 + BX < dimx && y0 >= 1 && y0 + BY < dimy;
+    if (interior) {
+#pragma unroll
+        for (int k = 0; k < G::KR; ++k) {
+            const int idx = tid + k * NT;
+            if ((k + 1) * NT <= NROWV || idx < NROWV) {
+                const int r = idx / RUN;
+                buf[k] = xb[idx + r * GAP - RS];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < G::KR; ++k) {
+            const int idx = tid + k * NT;
+            const int r = idx / RUN;
+            const int t = (idx - r * RUN) / NQ;
+            const int gy = y0 - 1 + r;
+            const bool ok = idx < NROWV && gy >= 0 && gy < dimy && x0 + t < dimx;
+            buf[k] = ok ? xb[idx + r * GAP - RS] : T(0);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < G::KC; ++k) {
+        const int idx = tid + k * NT;
+        const int kk = idx / NQ, q = idx - kk * NQ;
+        const bool right = kk >= BY;
+        const int row = right ? kk - BY : kk;
+        const bool ok = idx < NCOLV && y0 + row < dimy && (right ? x0 + BX < dimx : x0 >= 1);
+        cbuf[k] = ok ? xb[row * RS + (right ? BX * NQ : -NQ) + q] : T(0);
+    }
+}
+
+template <typename T, int P, int BX, int BY>
+__device__ __forceinline__ void block_stage(T *lds, const BlockRegs<T, P, BX, BY> &R)
+{
+    using G = BlockGeo<T, P, BX, BY>;
+    const T *buf = R.buf, *cbuf = R.cbuf;
+    constexpr int NQ = G::NQ, TS = G::TS, NT = G::NT;
+    const int tid = opaque(threadIdx.x);
+#pragma unroll
+    for (int k = 0; k < G::KR; ++k) {
+        const int idx = tid + k * NT;
+        if ((k + 1) * NT <= G::NROWV || idx < G::NROWV) lds[lds_index<NQ, TS>(idx)] = buf[k];
+    }
+#pragma unroll
+    for (int k = 0; k < G::KC; ++k) {
+        const int idx = tid + k * NT;
+        const int kk = idx / NQ, q = idx - kk * NQ;
+        if (idx < G::NCOLV) lds[(G::SL_COL + kk) * TS + q] = cbuf[k];
+    }
+}
+
+// The operator on the staged block (x0, y0), result written to y; returns
+// this thread's share of x.y over valid elements when DOT.  Contains the
+// block's barriers and leaves LDS free for the next stage.
+//
+// Thread (s, j): element s of the block, line j.  Column j of the element
+// (x-direction, flux_term6a) and row j (y-direction, flux_term6b) come from
+// LDS, then both sum-factorised contractions Yx[a] = (K1 U)[a][j] and
+// Yy[a] = (U K1)[j][a] (K1 symmetric) in even-odd form, the face terms, and
+// the assembly Y[a][b] = Yx (column j) + Yy (row j) over the element's own
+// tile before the block's rows are copied out as contiguous runs.
+template <typename T, int P, int BX, int BY, bool DOT>
+__device__ __forceinline__ double block_compute(T *lds, T *__restrict__ y, int dimx, int dimy,
+                                                int x0, int y0, T pen)
+{
+    using G = BlockGeo<T, P, BX, BY>;
+    constexpr int N1 = G::N1, NQ = G::NQ, TS = G::TS, NE = G::NE, NT = G::NT;
+    constexpr int SL_COL = G::SL_COL, RUN = G::RUN;
+
+    // degree-p tables as compile-time constants (see basis_constexpr.hpp)
+    constexpr cx::Tables<P> CT = cx::make_tables<P>();
+    struct CW { __device__ T operator[](int i) const { return (T)CT.w[i]; } } w;
+    struct C0 { __device__ T operator[](int i) const { return (T)CT.D[i][0]; } } D0;
+    struct CP { __device__ T operator[](int i) const { return (T)CT.D[i][P]; } } DP;
+
+    const int tid = threadIdx.x;
+    const int s = tid % NE;
+    int j = tid / NE;
+    if (NE % 64 == 0) j = __builtin_amdgcn_readfirstlane(j);
+    const int ex = s % BX, ey = s / BX;
+    const T wj = w[j];
+    const T pen2 = pen + pen;
+    const int sl = (ey + 1) * BX + ex;                  // own LDS slot
+    const T *tile = lds + sl * TS;
+    const T *tlL = lds + (ex > 0 ? sl - 1 : SL_COL + ey) * TS;
+    const T *tlR = lds + (ex < BX - 1 ? sl + 1 : SL_COL + BY + ey) * TS;
+    const T *tlD = lds + (sl + BX) * TS;                // element below (y+1)
+    const T *tlU = lds + (sl - BX) * TS;                // element above (y-1)
+    T *otile = lds + sl * TS;
+
+    const int gx = x0 + ex, gy = y0 + ey;
+    const bool Lb = (gx == 0), Rb = (gx == dimx - 1);
+    const bool Ub = (gy == 0), Db = (gy == dimy - 1);   // y grows downwards
+
+    T u[N1], v[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+        u[i] = tile[i * N1 + j];
+        v[i] = tile[j * N1 + i];
+    }
+    T ev, fv, Ax, Bx, g, h, Cy, Dy;
+    {
+        T nL = 0, tL = 0, nR = 0, tR = 0;
+        if (!Lb) {
+#pragma unroll
+            for (int i = 0; i < N1; ++i) {
+                const T t = tlL[i * N1 + j];
+                nL += DP[i] * t;
+                if (i == P) tL = t;
+            }
+        }
+        if (!Rb) {
+#pragma unroll
+            for (int i = 0; i < N1; ++i) {
+                const T t = tlR[i * N1 + j];
+                nR += D0[i] * t;
+                if (i == 0) tR = t;
+            }
+        }
+        T dL = 0, dR = 0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+            dL += D0[i] * u[i];
+            dR += DP[i] * u[i];
+        }
+        ev = wj * (Lb ? u[0] : T(0.5) * (u[0] - tL));
+        fv = wj * (Rb ? -u[P] : T(0.5) * (tR - u[P]));
+        Ax = Lb ? dL : T(0.5) * (dL + nL);
+        Bx = Rb ? -dR : T(-0.5) * (dR + nR);
+    }
+    {
+        T nD = 0, tD = 0, nU = 0, tU = 0;
+        if (!Db) {
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const T t = tlD[j * N1 + k];
+                nD += DP[k] * t;
+                if (k == P) tD = t;
+            }
+        }
+        if (!Ub) {
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const T t = tlU[j * N1 + k];
+                nU += D0[k] * t;
+                if (k == 0) tU = t;
+            }
+        }
+        T dD = 0, dU = 0;
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            dD += D0[k] * v[k];
+            dU += DP[k] * v[k];
+        }
+        g = wj * (Db ? v[0] : T(0.5) * (v[0] - tD));
+        h = wj * (Ub ? -v[P] : T(0.5) * (tU - v[P]));
+        Cy = Db ? dD : T(0.5) * (dD + nD);
+        Dy = Ub ? -dU : T(-0.5) * (dU + nU);
+    }
+
+    // even-odd form: each constant of the half-size E/O blocks serves both
+    // directions while it is live
+    T Yx[N1], Yy[N1];
+    {
+        constexpr cx::EvenOdd<P> EO = cx::make_even_odd<P>();
+        constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
+        T ue[ME], uo[H > 0 ? H : 1], ve[ME], vo[H > 0 ? H : 1];
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            ue[i] = u[i] + u[P - i];
+            uo[i] = u[i] - u[P - i];
+            ve[i] = v[i] + v[P - i];
+            vo[i] = v[i] - v[P - i];
+        }
+        if (ME > H) { ue[ME - 1] = u[H]; ve[ME - 1] = v[H]; }
+#pragma unroll
+        for (int a = 0; a < ME; ++a) {
+            T ex_ = 0, ey_ = 0, ox_ = 0, oy_ = 0;
+#pragma unroll
+            for (int i = 0; i < ME; ++i) {
+                const T c = (T)EO.E[a][i];
+                ex_ += c * ue[i];
+                ey_ += c * ve[i];
+            }
+            if (a < H) {
+#pragma unroll
+                for (int i = 0; i < H; ++i) {
+                    const T c = (T)EO.O[a][i];
+                    ox_ += c * uo[i];
+                    oy_ += c * vo[i];
+                }
+                Yx[a] = ex_ + ox_;
+                Yx[P - a] = ex_ - ox_;
+                Yy[a] = ey_ + oy_;
+                Yy[P - a] = ey_ - oy_;
+            } else {
+                Yx[a] = ex_;
+                Yy[a] = ey_;
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < N1; ++a) {
+        Yx[a] = wj * Yx[a] + D0[a] * ev + DP[a] * fv;
+        Yy[a] = wj * Yy[a] + D0[a] * g + DP[a] * h;
+    }
+    Yx[0] += wj * Ax + (Lb ? pen : pen2) * ev;
+    Yx[P] += wj * Bx - (Rb ? pen : pen2) * fv;
+    Yy[0] += wj * Cy + (Db ? pen : pen2) * g;
+    Yy[P] += wj * Dy - (Ub ? pen : pen2) * h;
+
+    __syncthreads();                                   // all tile reads done
+#pragma unroll
+    for (int a = 0; a < N1; ++a) otile[a * N1 + j] = Yx[a];
+    __syncthreads();
+    const bool valid = gx < dimx && gy < dimy;
+    double dot = 0.0;
+#pragma unroll
+    for (int c = 0; c < N1; ++c) {
+        const T yv = otile[j * N1 + c] + Yy[c];
+        otile[j * N1 + c] = yv;
+        if (DOT && valid) dot += (double)v[c] * (double)yv;
+    }
+    __syncthreads();
+    {
+        const int tid2 = opaque(threadIdx.x);
+        const int GAP = dimx * NQ - RUN;
+        T *yb = y + ((size_t)y0 * dimx + x0) * NQ;
+        const bool full = x0 + BX <= dimx && y0 + BY <= dimy;
+#pragma unroll
+        for (int k = 0; k < G::KO; ++k) {
+            const int idx = tid2 + k * NT;
+            if ((k + 1) * NT <= G::NOUT || idx < G::NOUT) {
+                const int r = idx / RUN;
+                bool ok = true;
+                if (!full) ok = y0 + r < dimy && x0 + (idx - r * RUN) / NQ < dimx;
+                if (ok) yb[idx + r * GAP] = lds[lds_index<NQ, TS>(idx + RUN)];
+            }
+        }
+    }
+    __syncthreads();                                   // LDS free for the next block
+    return dot;
+}
+
+}  // namespace sipg_internal

@@ -78,6 +78,30 @@ struct cg_state {
     int true_stop;   // 1: finishing requires a true residual below the tolerance
 };
 
+// ---- batched solves (kernels_batch.hip, batch.cpp) ------------------------
+// One instance of a batch: its vectors start at `off` in the concatenated
+// arrays (element-major within the instance).
+struct batch_inst {
+    int64_t off;
+    int dimx, dimy;
+    double pen;
+};
+// A BX x BY block of an instance for the operator kernel.
+struct batch_item {
+    int inst, x0, y0, pad;
+};
+// A range [begin, end) of the concatenated vectors, inside one instance.
+struct batch_chunk {
+    int inst, pad;
+    int64_t begin, end;
+};
+// Candidate block shapes of the batched operator (the autotuner picks one
+// per degree): 16x4, 8x4, 8x2 elements.
+constexpr int kBatchShapes = 3;
+extern const int kBatchShapeBX[kBatchShapes];
+extern const int kBatchShapeBY[kBatchShapes];
+bool batch_shape_supported(int p, int shape);
+
 // Reduction partial buffers hold at most this many block sums.
 constexpr int kMaxPartials = 2048;
 constexpr int kVecThreads = 256;
@@ -121,6 +145,32 @@ int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, h
 int launch_mark_replaced(cg_state *st, int slot, int restart, hipStream_t s);
 int launch_convert(int from_dtype, int to_dtype, const void *src, void *dst, int64_t n,
                    hipStream_t s);
+// batched launchers (FP64); counts live in device memory so a captured
+// graph stays valid while the lists shrink; max_* size the grids.  With
+// item_part (the fused p.Ap variant) blocks of finished instances are
+// skipped, which needs states/slot.
+int launch_batch_apply(int p, int shape, const double *X, double *Y, const batch_inst *inst,
+                       const batch_item *items, const int *count, int max_items, double *item_part,
+                       const cg_state *states, int slot, hipStream_t s);
+int launch_batch_alpha(const int *list, const int *count, int max_inst, const int2 *item_range,
+                       const double *item_part, cg_state *states, int slot, hipStream_t s);
+int launch_batch_update_xr(double *X, const double *Pv, double *Rv, const double *APv,
+                           const batch_chunk *chunks, const int *count, int max_chunks,
+                           double *chunk_part, const cg_state *states, int slot, hipStream_t s);
+int launch_batch_beta(const int *list, const int *count, int max_inst, const int2 *chunk_range,
+                      const double *chunk_part, cg_state *states, int slot, hipStream_t s);
+int launch_batch_update_p(const double *Rv, double *Pv, const batch_chunk *chunks, const int *count,
+                          int max_chunks, const cg_state *states, int slot, hipStream_t s);
+int launch_batch_residual(const double *B, const double *AX, double *Rv, double *Pv,
+                          const batch_chunk *chunks, const int *count, int max_chunks, double *part_rr,
+                          double *part_bb, hipStream_t s);
+int launch_batch_init(const int *list, const int *count, int max_inst, const int2 *chunk_range,
+                      const double *part_rr, const double *part_bb, cg_state *states, double tol,
+                      int tol_mode, const int *maxit, hipStream_t s);
+int launch_batch_check(const int *list, const int *count, int max_inst, const int2 *chunk_range,
+                       const double *part_rr, cg_state *states, int slot, int *restart, hipStream_t s);
+int launch_batch_restart_p(const double *Rv, double *Pv, const batch_chunk *chunks, const int *count,
+                           int max_chunks, const int *restart, hipStream_t s);
 // reference order <-> element-major device order (to_device = 1: ref -> dev)
 int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t noe, int nq,
                   hipStream_t s);

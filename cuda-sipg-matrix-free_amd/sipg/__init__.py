@@ -105,6 +105,18 @@ def _declare(L):
         "sipg_download": (i, [_vp, _vp, _dp]),
         "sipg_to_device_layout": (i, [_vp, _vp, _vp]),
         "sipg_to_reference_layout": (i, [_vp, _vp, _vp]),
+        "sipg_batch_create": (i, [ctypes.POINTER(_vp), i, _ip, _ip, _ip, _dp, _vp]),
+        "sipg_batch_destroy": (i, [_vp]),
+        "sipg_batch_size": (i, [_vp, ctypes.POINTER(i64)]),
+        "sipg_batch_offset": (i, [_vp, i, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "sipg_batch_tune": (i, [_vp]),
+        "sipg_batch_tuning": (i, [_vp, i, _ip, _ip, _dp]),
+        "sipg_batch_set_shape": (i, [_vp, i, i, i]),
+        "sipg_batch_apply": (i, [_vp, _vp, _vp]),
+        "sipg_batch_rhs_fourier": (i, [_vp, i, i, _dp, _ip, _ip, _vp]),
+        "sipg_batch_upload": (i, [_vp, i, _dp, _vp]),
+        "sipg_batch_download": (i, [_vp, i, _vp, _dp]),
+        "sipg_batch_cg": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -337,3 +349,76 @@ def basis_tables(p):
 def pen_sigma(p):
     """Kernel penalty for sigma = 10 (p+1)^2 / h: pen = sigma/2 = 5 (p+1)^2."""
     return 5.0 * (p + 1) ** 2
+
+
+class Batch:
+    """Many independent FP64 problems solved together (sipg_batch_*).
+    Instance i: a dims[i][0] x dims[i][1] mesh of degree degrees[i]."""
+
+    def __init__(self, dimx, dimy, degrees, pens, stream=None):
+        ni = len(degrees)
+        self.ninst = ni
+        self.dimx = np.ascontiguousarray(dimx, dtype=np.int32)
+        self.dimy = np.ascontiguousarray(dimy, dtype=np.int32)
+        self.p = np.ascontiguousarray(degrees, dtype=np.int32)
+        self.pen = np.ascontiguousarray(pens, dtype=np.float64)
+        self.h = _vp()
+        check(lib().sipg_batch_create(ctypes.byref(self.h), ni, self.dimx.ctypes.data_as(_ip),
+                                      self.dimy.ctypes.data_as(_ip), self.p.ctypes.data_as(_ip),
+                                      self.pen.ctypes.data_as(_dp), _vp(stream) if stream else None),
+              "batch_create")
+        n = ctypes.c_int64()
+        check(lib().sipg_batch_size(self.h, ctypes.byref(n)), "batch_size")
+        self.n = n.value
+        self.offsets = np.zeros(ni + 1, dtype=np.int64)
+        for i in range(ni):
+            o, k = ctypes.c_int64(), ctypes.c_int64()
+            check(lib().sipg_batch_offset(self.h, i, ctypes.byref(o), ctypes.byref(k)), "batch_offset")
+            self.offsets[i] = o.value
+        self.offsets[ni] = self.n
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.sipg_batch_destroy(self.h)
+            self.h = None
+
+    def vector(self):
+        return DeviceBuffer(self.n * 8)
+
+    def tune(self):
+        check(lib().sipg_batch_tune(self.h), "batch_tune")
+
+    def tuning(self, p):
+        bx, by, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().sipg_batch_tuning(self.h, p, ctypes.byref(bx), ctypes.byref(by), ctypes.byref(us)),
+              "batch_tuning")
+        return bx.value, by.value, us.value
+
+    def set_shape(self, p, bx, by):
+        return lib().sipg_batch_set_shape(self.h, p, bx, by)
+
+    def apply(self, x, y):
+        return lib().sipg_batch_apply(self.h, x.ptr, y.ptr)
+
+    def rhs_fourier(self, inst, modes, b):
+        a, m, n = modes
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        check(lib().sipg_batch_rhs_fourier(self.h, inst, len(a), a.ctypes.data_as(_dp), m.ctypes.data_as(_ip),
+                                           n.ctypes.data_as(_ip), b.ptr), "batch_rhs_fourier")
+
+    def upload(self, inst, host, dev):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        check(lib().sipg_batch_upload(self.h, inst, host.ctypes.data_as(_dp), dev.ptr), "batch_upload")
+
+    def download(self, inst, dev):
+        out = np.empty(int(self.offsets[inst + 1] - self.offsets[inst]))
+        check(lib().sipg_batch_download(self.h, inst, dev.ptr, out.ctypes.data_as(_dp)), "batch_download")
+        return out
+
+    def cg(self, x, b, tol, tol_mode=TOL_REL, maxit=0, true_stop=True):
+        o = CgOpts(tol, tol_mode, maxit, 0.0, 1 if true_stop else 0, 0)
+        infos = (CgInfo * self.ninst)()
+        st = lib().sipg_batch_cg(self.h, x.ptr, b.ptr, ctypes.byref(o), infos)
+        return st, list(infos)

@@ -188,6 +188,52 @@ int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts,
 /* ||b - A x||_2 / ||b||_2 in FP64 (op must be SIPG_F64). */
 int sipg_true_relres(sipg_op *op, const void *x, const void *b, double *relres);
 
+/* ---- batches of independent solves ------------------------------------ */
+/* A batch holds any number of independent SIPG problems (instance i: a
+ * dimx[i] x dimy[i] mesh of degree p[i] with penalty pen[i]) and solves them
+ * together, FP64.  No counterpart in the reference, whose degree tables are
+ * process globals (device_const.hpp:10-18) so that one process handles one
+ * degree at a time; each instance follows the single-problem semantics of
+ * sipg_cg_ex.  Vectors of a batch are one device array of sipg_batch_size
+ * elements: instance i occupies [offset_i, offset_i + n_i), element-major
+ * (see the layout note above).  stream: as in sipg_op_create, except that a
+ * null stream makes the batch create its own (CG iterations are replayed
+ * from a captured hipGraph, which the null stream cannot hold). */
+typedef struct sipg_batch sipg_batch;
+int sipg_batch_create(sipg_batch **out, int ninst, const int *dimx, const int *dimy, const int *p,
+                      const double *pen, void *stream);
+int sipg_batch_destroy(sipg_batch *b);
+int sipg_batch_size(const sipg_batch *b, int64_t *total_n);
+int sipg_batch_offset(const sipg_batch *b, int inst, int64_t *offset, int64_t *n);
+/* Chooses the operator kernel's block shape for every degree of the batch
+ * by timing the candidates (16x4, 8x4, 8x2 elements) on the batch's own
+ * instances; done on first use if not called.  sipg_batch_tuning reports
+ * the choice for degree p and its time per apply of all instances of that
+ * degree (-1 before tuning). */
+int sipg_batch_tune(sipg_batch *b);
+int sipg_batch_tuning(const sipg_batch *b, int p, int *bx, int *by, double *us_per_apply);
+/* Fixes the block shape (bx x by elements, one of the candidates) for the
+ * instances of degree p, in place of tuning (for callers that tune offline,
+ * and for tests of every shape). */
+int sipg_batch_set_shape(sipg_batch *b, int p, int bx, int by);
+/* y = A_i x_i for every instance (x, y: batch vectors, not overlapping). */
+int sipg_batch_apply(sipg_batch *b, const void *x, void *y);
+/* Fourier forcing of instance `inst` into its part of the batch vector rhs
+ * (as sipg_rhs_fourier; square meshes). */
+int sipg_batch_rhs_fourier(sipg_batch *b, int inst, int nmodes, const double *a, const int *m,
+                           const int *n, void *rhs);
+/* Host data of one instance in the reference order <-> its part of a batch
+ * vector (synchronous). */
+int sipg_batch_upload(sipg_batch *b, int inst, const double *host, void *dev);
+int sipg_batch_download(sipg_batch *b, int inst, const void *dev, double *host);
+/* CG on every instance: x (in/out) and rhs are batch vectors.  Uses tol,
+ * tol_mode, maxit (<= 0: n_i per instance) and true_residual_stop of opts
+ * (an instance whose recursion meets the tolerance is confirmed on b - A x,
+ * or resumes from it); replace_factor and restart are not used.  info, if
+ * not null, receives one record per instance.  Returns SIPG_OK when every
+ * instance converged, else the first failing instance's status. */
+int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opts, sipg_cg_info *info);
+
 /* ---- right-hand side and error norms (device) -------------------------- */
 /* Seeded Fourier forcing u = sum_k a_k sin(m_k pi x) sin(n_k pi y) drawn
  * from std::mt19937_64(seed): K = kmin + r%(kmax-kmin+1), then per mode

@@ -6,10 +6,13 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <vector>
 
 typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 };
 typedef struct mock_stream *hipStream_t;
+struct int2 { int x, y; };
 typedef struct mock_event *hipEvent_t;
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyDefault };
 #define hipEventDisableTiming 2
@@ -40,3 +43,47 @@ inline hipError_t hipEventDestroy(hipEvent_t e) { std::free(e); return hipSucces
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+
+// streams, pinned memory and graphs: a stream is a tag; "capture" records
+// the work the mock launchers would run, and a graph replays it
+#define hipStreamNonBlocking 1
+#define hipHostMallocDefault 0
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+typedef struct mock_graph *hipGraph_t;
+typedef struct mock_graph *hipGraphExec_t;
+typedef struct mock_node *hipGraphNode_t;
+struct mock_graph { std::vector<std::function<void()>> work; };
+inline std::vector<std::function<void()>> *&mock_capture() {
+    static std::vector<std::function<void()>> *cap = nullptr;
+    return cap;
+}
+// runs `f` now, or records it while a stream is being captured
+inline void mock_run(std::function<void()> f) {
+    if (mock_capture()) mock_capture()->push_back(std::move(f));
+    else f();
+}
+inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (hipStream_t)std::malloc(8); return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t s) { std::free(s); return hipSuccess; }
+template <typename T> inline hipError_t hipHostMalloc(T **p, size_t n, unsigned) { return hipMalloc(p, n); }
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    if (mock_capture()) return hipErrorInvalidValue;
+    mock_capture() = new std::vector<std::function<void()>>();
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
+    if (!mock_capture()) return hipErrorInvalidValue;
+    *g = new mock_graph{std::move(*mock_capture())};
+    delete mock_capture();
+    mock_capture() = nullptr;
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *, char *, size_t) {
+    *e = new mock_graph{g->work};
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (auto &f : e->work) f();
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }

@@ -1,0 +1,255 @@
+// CPU stand-ins for the batched launchers (kernels_batch.hip), for the CPU
+// test build of batch.cpp.  Each mirrors its kernel's rules (skips of
+// finished instances, carry-forward, per-chunk and per-item partial sums,
+// check / resume); the operator is the oracle's, applied to whole instances
+// (the engine always lists all blocks of an instance together).  Work is run
+// through mock_run, so it is recorded while a stream is captured and
+// replayed by the mock graph.  Test infrastructure only.
+#include "../../cuda-sipg-matrix-free_amd/csrc/sipg_internal.hpp"
+
+#include <cmath>
+#include <set>
+#include <vector>
+
+extern "C" void orc_apply(int dimx, int dimy, int p, double pen, const double *x, double *y);
+
+namespace sipg_internal {
+
+const int kBatchShapeBX[kBatchShapes] = {16, 8, 8};
+const int kBatchShapeBY[kBatchShapes] = {4, 4, 2};
+
+bool batch_shape_supported(int p, int shape) {
+    if (p < 1 || p > kMaxDegree || shape < 0 || shape >= kBatchShapes) return false;
+    return shape != 0 || p <= 12;
+}
+
+namespace {
+
+void apply_instance(const batch_inst &in, int p, const double *X, double *Y) {
+    const int64_t noe = (int64_t)in.dimx * in.dimy, nq = (int64_t)(p + 1) * (p + 1);
+    std::vector<double> xr(noe * nq), yr(noe * nq);
+    const double *x = X + in.off;
+    double *y = Y + in.off;
+    for (int64_t e = 0; e < noe; ++e)
+        for (int64_t q = 0; q < nq; ++q) xr[q * noe + e] = x[e * nq + q];
+    orc_apply(in.dimx, in.dimy, p, in.pen, xr.data(), yr.data());
+    for (int64_t e = 0; e < noe; ++e)
+        for (int64_t q = 0; q < nq; ++q) y[e * nq + q] = yr[q * noe + e];
+}
+
+bool finished(const cg_state *st, int slot) { return st->s[slot].done | st->err; }
+
+}  // namespace
+
+int launch_batch_apply(int p, int shape, const double *X, double *Y, const batch_inst *inst,
+                       const batch_item *items, const int *count, int max_items, double *item_part,
+                       const cg_state *states, int slot, hipStream_t) {
+    if (!batch_shape_supported(p, shape)) return SIPG_ERR_UNSUPPORTED;
+    if (item_part && !states) return SIPG_ERR_INVALID;
+    mock_run([=]() {
+        const int n = *count;
+        if (n > max_items) std::abort();
+        const int BX = kBatchShapeBX[shape], BY = kBatchShapeBY[shape];
+        const int64_t nq = (int64_t)(p + 1) * (p + 1);
+        std::set<int> done_inst;
+        for (int k = 0; k < n; ++k) {
+            const batch_item it = items[k];
+            if (item_part && finished(states + it.inst, slot)) continue;
+            const batch_inst in = inst[it.inst];
+            if (!done_inst.count(it.inst)) {
+                apply_instance(in, p, X, Y);
+                done_inst.insert(it.inst);
+            }
+            if (item_part) {
+                double s = 0.0;
+                for (int ey = it.y0; ey < it.y0 + BY && ey < in.dimy; ++ey)
+                    for (int ex = it.x0; ex < it.x0 + BX && ex < in.dimx; ++ex) {
+                        const int64_t b = in.off + ((int64_t)ey * in.dimx + ex) * nq;
+                        for (int64_t q = 0; q < nq; ++q) s += X[b + q] * Y[b + q];
+                    }
+                item_part[k] = s;
+            }
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_alpha(const int *list, const int *count, int, const int2 *item_range, const double *item_part,
+                       cg_state *states, int slot, hipStream_t) {
+    mock_run([=]() {
+        for (int k = 0; k < *count; ++k) {
+            cg_state *st = states + list[k];
+            if (finished(st, slot)) continue;
+            const int2 r = item_range[list[k]];
+            double pAp = 0.0;
+            for (int i = r.x; i < r.y; ++i) pAp += item_part[i];
+            const double alpha = st->s[slot].rr / pAp;
+            if (!(pAp > 0.0) || !std::isfinite(alpha)) {
+                st->err = std::isfinite(pAp) ? 2 : 3;
+            } else {
+                st->s[slot].alpha = alpha;
+                st->s[slot].pAp = pAp;
+            }
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_update_xr(double *X, const double *Pv, double *Rv, const double *APv, const batch_chunk *chunks,
+                           const int *count, int max_chunks, double *chunk_part, const cg_state *states,
+                           int slot, hipStream_t) {
+    mock_run([=]() {
+        if (*count > max_chunks) std::abort();
+        for (int k = 0; k < *count; ++k) {
+            const batch_chunk c = chunks[k];
+            const cg_state *st = states + c.inst;
+            if (finished(st, slot)) continue;
+            const double a = st->s[slot].alpha;
+            double s = 0.0;
+            for (int64_t i = c.begin; i < c.end; ++i) {
+                X[i] += a * Pv[i];
+                Rv[i] -= a * APv[i];
+                s += Rv[i] * Rv[i];
+            }
+            chunk_part[k] = s;
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_beta(const int *list, const int *count, int, const int2 *chunk_range, const double *chunk_part,
+                      cg_state *states, int slot, hipStream_t) {
+    mock_run([=]() {
+        for (int k = 0; k < *count; ++k) {
+            cg_state *st = states + list[k];
+            const cg_slot cur = st->s[slot];
+            if (cur.done | st->err) {
+                st->s[slot ^ 1] = cur;
+                continue;
+            }
+            const int2 r = chunk_range[list[k]];
+            double rr = 0.0;
+            for (int i = r.x; i < r.y; ++i) rr += chunk_part[i];
+            cg_slot nx{};
+            nx.rr = rr;
+            nx.beta = rr / cur.rr;
+            nx.it = cur.it + 1;
+            nx.done = (!(std::sqrt(rr) > st->stop) || nx.it >= st->maxit) ? 1 : 0;
+            st->s[slot ^ 1] = nx;
+            if (!std::isfinite(rr)) st->err = 3;
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_update_p(const double *Rv, double *Pv, const batch_chunk *chunks, const int *count, int,
+                          const cg_state *states, int slot, hipStream_t) {
+    mock_run([=]() {
+        for (int k = 0; k < *count; ++k) {
+            const batch_chunk c = chunks[k];
+            const cg_state *st = states + c.inst;
+            if (finished(st, slot)) continue;
+            const double beta = st->s[slot ^ 1].beta;
+            for (int64_t i = c.begin; i < c.end; ++i) Pv[i] = Rv[i] + beta * Pv[i];
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_residual(const double *B, const double *AX, double *Rv, double *Pv, const batch_chunk *chunks,
+                          const int *count, int max_chunks, double *part_rr, double *part_bb, hipStream_t) {
+    mock_run([=]() {
+        if (*count > max_chunks) std::abort();
+        for (int k = 0; k < *count; ++k) {
+            const batch_chunk c = chunks[k];
+            double srr = 0.0, sbb = 0.0;
+            for (int64_t i = c.begin; i < c.end; ++i) {
+                const double r = B[i] - AX[i];
+                Rv[i] = r;
+                if (Pv) Pv[i] = r;
+                srr += r * r;
+                sbb += B[i] * B[i];
+            }
+            part_rr[k] = srr;
+            part_bb[k] = sbb;
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_init(const int *list, const int *count, int, const int2 *chunk_range, const double *part_rr,
+                      const double *part_bb, cg_state *states, double tol, int tol_mode, const int *maxit,
+                      hipStream_t) {
+    mock_run([=]() {
+        for (int k = 0; k < *count; ++k) {
+            const int i = list[k];
+            const int2 r = chunk_range[i];
+            double rr = 0.0, bb = 0.0;
+            for (int c = r.x; c < r.y; ++c) {
+                rr += part_rr[c];
+                bb += part_bb[c];
+            }
+            cg_state *st = states + i;
+            const double stop = tol_mode == SIPG_TOL_REL ? tol * std::sqrt(bb) : tol;
+            cg_slot s0{};
+            s0.rr = rr;
+            s0.done = (std::sqrt(rr) > stop && maxit[i] > 0) ? 0 : 1;
+            s0.checked = 1;
+            st->s[0] = s0;
+            st->s[1] = s0;
+            st->bb = bb;
+            st->stop = stop;
+            st->rr0 = rr;
+            st->rr_ref = rr;
+            st->maxit = maxit[i];
+            st->nrep = 0;
+            st->err = (!std::isfinite(rr) || !std::isfinite(bb)) ? 3 : 0;
+            st->true_stop = 1;
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_check(const int *list, const int *count, int, const int2 *chunk_range, const double *part_rr,
+                       cg_state *states, int slot, int *restart, hipStream_t) {
+    mock_run([=]() {
+        for (int k = 0; k < *count; ++k) {
+            const int i = list[k];
+            const int2 r = chunk_range[i];
+            double rr = 0.0;
+            for (int c = r.x; c < r.y; ++c) rr += part_rr[c];
+            cg_state *st = states + i;
+            cg_slot c = st->s[slot];
+            restart[i] = rr > 4.0 * c.rr ? 1 : 0;
+            st->rr_ref = rr;
+            st->nrep += 1;
+            if (!std::isfinite(rr)) {
+                st->err = 3;
+            } else if (!(std::sqrt(rr) > st->stop) || c.it >= st->maxit) {
+                c.done = 1;
+                c.checked = 1;
+            } else {
+                c.done = 0;
+                c.checked = 0;
+            }
+            c.rr = rr;
+            st->s[slot] = c;
+            st->s[slot ^ 1] = c;
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_restart_p(const double *Rv, double *Pv, const batch_chunk *chunks, const int *count, int,
+                           const int *restart, hipStream_t) {
+    mock_run([=]() {
+        for (int k = 0; k < *count; ++k) {
+            const batch_chunk c = chunks[k];
+            if (!restart[c.inst]) continue;
+            for (int64_t i = c.begin; i < c.end; ++i) Pv[i] = Rv[i];
+        }
+    });
+    return SIPG_OK;
+}
+
+}  // namespace sipg_internal

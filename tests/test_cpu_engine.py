@@ -73,3 +73,37 @@ def test_mixed_precision_converges_in_fp64(results, case):
 def test_mixed_cap_reports_not_converged(results):
     r = results["mixed_cap"]
     assert r["rc"] == 6 and r["iters"] == 30
+
+
+@pytest.fixture(scope="module")
+def batch_results(results):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="halt_on_error=1")
+    out = subprocess.run([os.path.join(ENG, "_build", "batch_driver")], check=True, capture_output=True,
+                         text=True, env=env, timeout=600)
+    assert "runtime error" not in out.stderr and "AddressSanitizer" not in out.stderr, out.stderr
+    rows = [json.loads(l) for l in out.stdout.strip().splitlines()]
+    by = {}
+    for r in rows:
+        by.setdefault(r["case"], []).append(r)
+    return by
+
+
+def test_batch_engine_solves_every_instance(batch_results):
+    # graph capture and replay, list compaction and true-residual checks of
+    # the batched engine (batch.cpp); each instance is the reference CG
+    for r in batch_results["batch"]:
+        assert r["rc"] == 0 and r["status"] == 0 and r["relres"] <= 1e-10, r
+        assert abs(r["iters"] - r["oracle_iters"]) <= 2, r
+        assert r["diff"] <= 1e-9 and r["reps"] == 1 and r["apply_err"] == 0.0, r
+
+
+def test_batch_engine_iteration_cap(batch_results):
+    for r in batch_results["batch_cap"]:
+        assert r["rc"] == 6 and r["status"] == 6 and r["iters"] == 5, r
+
+
+def test_batch_engine_breakdown_is_per_instance(batch_results):
+    rows = batch_results["batch_breakdown"]
+    assert all(r["rc"] == 8 for r in rows)
+    assert [r["status"] for r in rows] == [0, 8, 0]
+    assert rows[0]["relres"] <= 1e-10 and rows[2]["relres"] <= 1e-10

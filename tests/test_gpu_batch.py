@@ -1,0 +1,122 @@
+"""Batched solves (sipg_batch_*) against the single-problem operator and
+solver, which the other GPU tests pin to the oracle.  The batched kernels run
+the same per-block arithmetic, so the operator must agree bit for bit for
+every block shape; the batched CG must reach the same true relative
+residual, with per-instance iteration counts close to the single solver's."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MESHES = [(5, 3, 2), (17, 9, 4), (1, 1, 3), (16, 4, 8), (33, 2, 6), (12, 12, 10), (7, 7, 2), (20, 13, 5),
+          (9, 11, 13), (6, 5, 16), (18, 10, 2)]
+
+
+def _batch(sipg, meshes):
+    dx = [m[0] for m in meshes]
+    dy = [m[1] for m in meshes]
+    ps = [m[2] for m in meshes]
+    return sipg.Batch(dx, dy, ps, [sipg.pen_sigma(p) for p in ps])
+
+
+@pytest.mark.parametrize("shape", [(16, 4), (8, 4), (8, 2)])
+def test_batch_apply_equals_single_apply(sipg, shape):
+    B = _batch(sipg, MESHES)
+    for p in sorted({m[2] for m in MESHES}):
+        rc = B.set_shape(p, *shape)
+        assert rc in (sipg.OK, sipg.ERR_UNSUPPORTED)
+        if rc == sipg.ERR_UNSUPPORTED:
+            assert shape == (16, 4) and p >= 13
+    x, y = B.vector(), B.vector()
+    xs = []
+    for i, (dx, dy, p) in enumerate(MESHES):
+        xi = np.random.default_rng(10 + i).uniform(-1, 1, dx * dy * (p + 1) ** 2)
+        xs.append(xi)
+        B.upload(i, xi, x)
+    assert B.apply(x, y) == sipg.OK
+    for i, (dx, dy, p) in enumerate(MESHES):
+        op = sipg.Operator(dx, dy, p, sipg.pen_sigma(p), sipg.F64)
+        assert np.array_equal(B.download(i, y), op.apply_host(xs[i])), (i, shape)
+
+
+def test_batch_tuning_picks_a_candidate(sipg):
+    B = _batch(sipg, MESHES)
+    B.tune()
+    for p in sorted({m[2] for m in MESHES}):
+        bx, by, us = B.tuning(p)
+        assert (bx, by) in ((16, 4), (8, 4), (8, 2)) and us > 0
+        assert not (p >= 13 and bx == 16)
+
+
+def _fourier_batch(sipg, meshes):
+    B = _batch(sipg, meshes)
+    b = B.vector()
+    for i in range(len(meshes)):
+        B.rhs_fourier(i, sipg.forcing_draw(i, 1, 4, 4), b)
+    return B, b
+
+
+SQUARE = [(4, 2), (8, 3), (16, 4), (24, 8), (5, 10), (32, 2), (12, 6), (20, 7), (9, 5), (4, 9), (28, 3), (16, 8)]
+
+
+def test_batch_cg_reaches_tolerance_like_single_solves(sipg):
+    meshes = [(d, d, p) for d, p in SQUARE]
+    B, b = _fourier_batch(sipg, meshes)
+    x = B.vector()
+    x.zero()
+    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL)
+    assert st == sipg.OK, sipg.strerror(st)
+    for i, (d, _, p) in enumerate(meshes):
+        op = sipg.Operator(d, d, p, sipg.pen_sigma(p), sipg.F64)
+        xi = op.to_device(B.download(i, x))
+        bi = op.to_device(B.download(i, b))
+        rel = op.true_relres(xi, bi)
+        assert rel <= 1e-10, (i, rel)
+        assert infos[i].status == sipg.OK and infos[i].true_res_norm <= 1e-10 * infos[i].b_norm * 1.0000001
+        xs = op.vector()
+        xs.zero()
+        st1, it1, _ = op.cg(xs, bi, 1e-10, sipg.TOL_REL)
+        assert st1 == sipg.OK
+        assert abs(infos[i].iters - it1) <= max(3, 0.03 * it1), (i, infos[i].iters, it1)
+        ref = op.download(xs)
+        got = B.download(i, x)
+        assert np.linalg.norm(got - ref) <= 1e-7 * np.linalg.norm(ref), i
+
+
+def test_batch_cg_deterministic(sipg):
+    meshes = [(d, d, p) for d, p in SQUARE[:6]]
+    B, b = _fourier_batch(sipg, meshes)
+    x1, x2 = B.vector(), B.vector()
+    x1.zero()
+    x2.zero()
+    st1, i1 = B.cg(x1, b, 1e-10, sipg.TOL_REL)
+    st2, i2 = B.cg(x2, b, 1e-10, sipg.TOL_REL)
+    assert st1 == st2 == sipg.OK
+    assert [k.iters for k in i1] == [k.iters for k in i2]
+    assert np.array_equal(x1.download(np.float64), x2.download(np.float64))
+
+
+def test_batch_iteration_cap(sipg):
+    meshes = [(d, d, p) for d, p in SQUARE[:5]]
+    B, b = _fourier_batch(sipg, meshes)
+    x = B.vector()
+    x.zero()
+    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL, maxit=5)
+    assert st == sipg.NOT_CONVERGED
+    assert all(k.iters == 5 and k.status == sipg.NOT_CONVERGED for k in infos)
+
+
+def test_batch_breakdown_stays_in_its_instance(sipg):
+    # a negative penalty makes one instance indefinite; the others converge
+    meshes = [(8, 8, 3), (8, 8, 3), (12, 12, 4)]
+    B = sipg.Batch([8, 8, 12], [8, 8, 12], [3, 3, 4], [80.0, -500.0, 125.0])
+    b = B.vector()
+    for i in range(3):
+        B.rhs_fourier(i, sipg.forcing_draw(i, 1, 4, 4), b)
+    x = B.vector()
+    x.zero()
+    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL)
+    assert st == sipg.ERR_BREAKDOWN
+    assert infos[1].status == sipg.ERR_BREAKDOWN
+    assert infos[0].status == sipg.OK and infos[2].status == sipg.OK
+    del meshes
