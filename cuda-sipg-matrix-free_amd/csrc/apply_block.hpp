@@ -32,10 +32,17 @@ __device__ __forceinline__ int xcd_remap(int b, int nblk) {
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double *red) {
     // wave64 butterfly, then a fixed-order sum over waves: deterministic.
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    // When NT is not a multiple of 64 the last wave is partial: its lanes
+    // >= NT % 64 do not exist and their registers hold nothing meaningful,
+    // so a partner outside the workgroup contributes zero.
     constexpr int NW = (NT + 63) / 64;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o = __shfl_xor(v, off, 64);
+        if (NT % 64 != 0 && wid * 64 + (lane ^ off) >= NT) o = 0.0;
+        v += o;
+    }
     if (lane == 0) red[wid] = v;
     __syncthreads();
     double s = 0.0;

@@ -702,7 +702,8 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
         int st = SIPG_OK;
         if (f.err == 2) st = SIPG_ERR_BREAKDOWN;
         else if (f.err == 3) st = SIPG_ERR_NONFINITE;
-        else if (!(std::sqrt(c.rr) <= f.stop) || (true_stop && !c.checked)) st = SIPG_NOT_CONVERGED;
+        else if (!(std::sqrt(c.rr) <= (true_stop ? f.stop_true : f.stop)) || (true_stop && !c.checked))
+            st = SIPG_NOT_CONVERGED;
         if (info) {
             info[i].status = st;
             info[i].iters = c.it;

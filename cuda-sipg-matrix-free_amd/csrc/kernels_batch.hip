@@ -269,6 +269,7 @@ __global__ __launch_bounds__(NT) void batch_init_kernel(const int *__restrict__ 
             st->s[1] = s0;
             st->bb = bb;
             st->stop = stop;
+            st->stop_true = stop;
             st->rr0 = rr;
             st->rr_ref = rr;
             st->maxit = maxit[i];
@@ -280,9 +281,11 @@ __global__ __launch_bounds__(NT) void batch_init_kernel(const int *__restrict__ 
 }
 
 // After r = b - A x for the listed (finished) instances: confirm the finish
-// on the true residual, or resume the iteration from it.  The direction is
-// kept while the true residual is within 2x of the recursive one
-// (restart[i] = 0), else the next kernel restarts it from p = r.
+// on the true residual, or resume the iteration from it.  A resumed
+// instance restarts from p = r (a direction built from the recursive
+// residual does not fit the replaced one) and its recursion must now reach
+// the threshold tightened by the gap just observed, so that the next check
+// lands below the true one.
 __global__ __launch_bounds__(NT) void batch_check_kernel(const int *__restrict__ list, const int *__restrict__ count,
                                                          const int2 *__restrict__ chunk_range,
                                                          const double *__restrict__ part_rr, cg_state *states,
@@ -296,17 +299,22 @@ __global__ __launch_bounds__(NT) void batch_check_kernel(const int *__restrict__
         if (threadIdx.x == 0) {
             cg_state *st = states + i;
             cg_slot c = st->s[slot];
-            restart[i] = rr > 4.0 * c.rr ? 1 : 0;
             st->rr_ref = rr;
             st->nrep += 1;
+            restart[i] = 0;
             if (!isfinite(rr)) {
                 st->err = 3;
-            } else if (!(sqrt(rr) > st->stop) || c.it >= st->maxit) {
+            } else if (!(sqrt(rr) > st->stop_true) || c.it >= st->maxit) {
                 c.done = 1;
                 c.checked = 1;
             } else {
                 c.done = 0;
                 c.checked = 0;
+                restart[i] = 1;
+                // recursion gap g = sqrt(rr / rr_rec) >= 1: aim the recursion
+                // at stop_true / g (at most 100x tighter per check)
+                const double g = sqrt(rr / fmax(c.rr, 1e-300));
+                st->stop = fmin(st->stop, st->stop_true / fmin(fmax(g, 1.0), 100.0));
             }
             c.rr = rr;
             st->s[slot] = c;

@@ -125,6 +125,7 @@ __global__ __launch_bounds__(NT) void cg_init_finalize_kernel(const double *__re
         st->s[0] = s0;
         st->s[1] = s0;
         st->rr0 = rr;
+        st->stop_true = stop;
         st->rr_ref = rr;
         st->bb = bb;
         st->stop = stop;

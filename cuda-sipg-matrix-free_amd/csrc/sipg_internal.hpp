@@ -69,7 +69,9 @@ constexpr int kSlotWantCheck = 4;
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b
-    double stop;     // stopping threshold on sqrt(r.r)
+    double stop;     // stopping threshold on sqrt(r.r) of the recursion
+    double stop_true;  // threshold for a true residual (batched solver: the
+                       // recursion's is tightened after a failed check)
     double rr0;      // initial r.r
     double rr_ref;   // r.r of the last true residual
     int maxit;

@@ -199,6 +199,7 @@ int launch_batch_init(const int *list, const int *count, int, const int2 *chunk_
             st->s[1] = s0;
             st->bb = bb;
             st->stop = stop;
+            st->stop_true = stop;
             st->rr0 = rr;
             st->rr_ref = rr;
             st->maxit = maxit[i];
@@ -220,17 +221,20 @@ int launch_batch_check(const int *list, const int *count, int, const int2 *chunk
             for (int c = r.x; c < r.y; ++c) rr += part_rr[c];
             cg_state *st = states + i;
             cg_slot c = st->s[slot];
-            restart[i] = rr > 4.0 * c.rr ? 1 : 0;
             st->rr_ref = rr;
             st->nrep += 1;
+            restart[i] = 0;
             if (!std::isfinite(rr)) {
                 st->err = 3;
-            } else if (!(std::sqrt(rr) > st->stop) || c.it >= st->maxit) {
+            } else if (!(std::sqrt(rr) > st->stop_true) || c.it >= st->maxit) {
                 c.done = 1;
                 c.checked = 1;
             } else {
                 c.done = 0;
                 c.checked = 0;
+                restart[i] = 1;
+                const double g = std::sqrt(rr / std::fmax(c.rr, 1e-300));
+                st->stop = std::fmin(st->stop, st->stop_true / std::fmin(std::fmax(g, 1.0), 100.0));
             }
             c.rr = rr;
             st->s[slot] = c;

@@ -124,6 +124,7 @@ int launch_cg_init_finalize(const double *prr, const double *pbb, int np, cg_sta
     st->s[0] = s0;
     st->s[1] = s0;
     st->rr0 = rr;
+    st->stop_true = stop;
     st->rr_ref = rr;
     st->bb = bb;
     st->stop = stop;

@@ -83,6 +83,26 @@ def test_batch_cg_reaches_tolerance_like_single_solves(sipg):
         assert np.linalg.norm(got - ref) <= 1e-7 * np.linalg.norm(ref), i
 
 
+@pytest.mark.parametrize("shape", [(16, 4), (8, 4), (8, 2)])
+def test_batch_cg_every_shape(sipg, shape):
+    # 8x2 blocks of degree p have 16(p+1) threads: partial waves for every p
+    meshes = [(d, d, p) for d, p in SQUARE[:8]]
+    B, b = _fourier_batch(sipg, meshes)
+    for p in sorted({m[2] for m in meshes}):
+        assert B.set_shape(p, *shape) == sipg.OK
+    x = B.vector()
+    x.zero()
+    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL, maxit=5000)
+    assert st == sipg.OK, [(k.status, k.iters) for k in infos]
+    for i, (d, _, p) in enumerate(meshes):
+        op = sipg.Operator(d, d, p, sipg.pen_sigma(p), sipg.F64)
+        bi = op.to_device(B.download(i, b))
+        xs = op.vector()
+        xs.zero()
+        st1, it1, _ = op.cg(xs, bi, 1e-10, sipg.TOL_REL)
+        assert abs(infos[i].iters - it1) <= max(3, 0.03 * it1), (i, shape, infos[i].iters, it1)
+
+
 def test_batch_cg_deterministic(sipg):
     meshes = [(d, d, p) for d, p in SQUARE[:6]]
     B, b = _fourier_batch(sipg, meshes)

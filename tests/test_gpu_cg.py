@@ -83,6 +83,25 @@ def test_cg_iterations_track_oracle(sipg, oracle):
     assert np.linalg.norm(op.download(x) - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
+# p = 14 and 16 in FP64 use 8x4-element workgroups of 480 and 544 threads:
+# the fused p.Ap sum then ends in a partial wave
+@pytest.mark.parametrize("p", [13, 14, 16])
+def test_cg_partial_wave_degrees_track_oracle(sipg, oracle, p):
+    modes = sipg.forcing_draw(0, 4, 4, 4)
+    dim = 6
+    pen = sipg.pen_sigma(p)
+    bh = oracle.rhs(dim, p, pen, oracle.problem(0, modes))
+    xo, ito, _ = oracle.cg(dim, dim, p, pen, bh, tol=1e-10, tol_mode=1)
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    b = op.to_device(bh)
+    x = op.vector()
+    x.zero()
+    st, it, _ = op.cg(x, b, 1e-10, sipg.TOL_REL)
+    assert st == sipg.OK
+    assert abs(it - ito) <= max(2, 0.02 * ito), (it, ito)
+    assert np.linalg.norm(op.download(x) - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
 def test_cg_absolute_tolerance_semantics(sipg, oracle):
     dim, p, pen = 8, 2, 45.0
     bh = oracle.rhs(dim, p, pen, oracle.problem(5))
