@@ -103,6 +103,137 @@ def profile_apply(sipg, op, reps):
         f"({8 * op.n} bytes per vector)")
 
 
+def batched_leg(sipg, log_every=True):
+    """2048 independent FP64 solves in one batch: mesh side ~ U{4..64},
+    degree ~ U{2..10} (numpy default_rng(4)), instance i forced by
+    forcing_draw(i, 1..4 modes, m,n <= 4), pen = 5 (p+1)^2, 1e-10 true
+    relative residual each.  Compared with solving a sample of the same
+    instances one after another with the single-problem solver."""
+    N = 2048
+    rng = np.random.default_rng(4)
+    dims = rng.integers(4, 65, N)
+    ps = rng.integers(2, 11, N)
+    pens = [sipg.pen_sigma(int(p)) for p in ps]
+    B = sipg.Batch(dims, dims, ps, pens)
+    b = B.vector()
+    for i in range(N):
+        B.rhs_fourier(i, sipg.forcing_draw(i, 1, 4, 4), b)
+    B.tune()
+    shapes = {int(p): B.tuning(int(p))[:2] for p in sorted(set(ps.tolist()))}
+    x = B.vector()
+    times = []
+    for _ in range(2):                      # the first solve is the warm-up
+        x.zero()
+        sipg.synchronize()
+        t0 = time.perf_counter()
+        st, infos = B.cg(x, b, TOL, sipg.TOL_REL, maxit=20000, replace_factor=3e-3)
+        times.append(time.perf_counter() - t0)
+    t = times[-1]
+    its = np.array([k.iters for k in infos])
+    rel = max(k.true_res_norm / k.b_norm for k in infos)
+    failed = [(int(dims[i]), int(ps[i]), int(infos[i].iters), sipg.strerror(infos[i].status))
+              for i in range(N) if infos[i].status != sipg.OK]
+    # looping the single-problem solver over every 64th instance
+    tl, dof = 0.0, 0
+    for i in range(0, N, 64):
+        d, p = int(dims[i]), int(ps[i])
+        op = sipg.Operator(d, d, p, pens[i], sipg.F64)
+        bi = op.vector()
+        op.rhs_fourier(sipg.forcing_draw(i, 1, 4, 4), bi)
+        xi = op.vector()
+        xi.zero()
+        sipg.synchronize()
+        t0 = time.perf_counter()
+        sti, _ = op.cg_ex(xi, bi, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=3e-3, true_stop=True)
+        tl += time.perf_counter() - t0
+        dof += op.n
+        if sti != sipg.OK:
+            raise RuntimeError(f"single solve {i} failed: {sipg.strerror(sti)}")
+    out = {"instances": N, "dof": int(B.n), "ms_per_batch_solve": 1e3 * t, "dof_per_s": B.n / t,
+           "status": sipg.strerror(st), "failed_instances": failed[:20], "n_failed": len(failed),
+           "iterations": {"min": int(its.min()), "median": int(np.median(its)), "max": int(its.max())},
+           "max_true_relres": rel, "block_shape_per_degree": shapes,
+           "loop_single_solver": {"instances": len(range(0, N, 64)), "dof_per_s": dof / tl},
+           "speedup_vs_loop": (B.n / t) / (dof / tl)}
+    log(f"batched: {N} solves, {B.n} DoF in {1e3 * t:.0f} ms = {B.n / t:.3e} DoF/s "
+        f"(loop of single solves {dof / tl:.3e} DoF/s), max true relres {rel:.2e}")
+    return out
+
+
+def serving_leg(sipg, seconds, rate):
+    """Poisson arrivals (rate per second, numpy default_rng(0x5EED)) for
+    `seconds`: mesh side log-uniform in 8..256, degree ~ U{2..12}, 70 %
+    marked mixed precision, tolerance 1e-8 or 1e-10, request k forced by
+    forcing_draw(k, 1..4 modes).  Served by one sipg_server with an 80 GB
+    HBM budget; latency = completion - scheduled arrival."""
+    rng = np.random.default_rng(0x5EED)
+    gaps = rng.exponential(1.0 / rate, int(rate * seconds * 2) + 16)
+    arrive = np.cumsum(gaps)
+    arrive = arrive[arrive < seconds]
+    K = len(arrive)
+    dims = np.rint(np.exp(rng.uniform(np.log(8), np.log(256), K))).astype(int)
+    ps = rng.integers(2, 13, K)
+    mixed = rng.random(K) < 0.7
+    tols = np.where(rng.random(K) < 0.5, 1e-8, 1e-10)
+    # block shapes from tuning a sample of the same distribution
+    sample = slice(0, min(K, 96))
+    Bt = sipg.Batch(dims[sample], dims[sample], ps[sample], [sipg.pen_sigma(int(p)) for p in ps[sample]])
+    Bt.tune()
+    S = sipg.Server(80e9)
+    for p in sorted(set(ps[sample].tolist())):
+        bx, by, _ = Bt.tuning(int(p))
+        S.set_shape(int(p), bx, by)
+    del Bt
+    done = {}
+    lat = np.zeros(K)
+    nxt = 0
+    ids = {}
+    max_queued = max_running = 0
+    t0 = time.perf_counter()
+    last_log = t0
+    while len(done) < K:
+        now = time.perf_counter() - t0
+        while nxt < K and arrive[nxt] <= now:
+            d, p = int(dims[nxt]), int(ps[nxt])
+            rid = S.submit(d, p, sipg.pen_sigma(p), sipg.forcing_draw(nxt, 1, 4, 4), float(tols[nxt]),
+                           maxit=MAXIT_FP64)
+            ids[rid] = nxt
+            nxt += 1
+        st = S.stats()
+        max_queued = max(max_queued, st["queued"])
+        max_running = max(max_running, st["running"])
+        if st["running"] == 0 and st["queued"] == 0 and nxt < K:
+            time.sleep(max(0.0, min(0.01, arrive[nxt] - now)))
+            continue
+        S.progress()
+        while True:
+            r = S.take()
+            if r is None:
+                break
+            k = ids[r[0]]
+            done[k] = r[1]
+            lat[k] = time.perf_counter() - t0 - arrive[k]
+        if time.perf_counter() - last_log > 5.0:
+            last_log = time.perf_counter()
+            log(f"serving: {len(done)}/{K} done, {st['running']} running, {st['queued']} queued")
+    wall = time.perf_counter() - t0
+    statuses = [done[k].status for k in range(K)]
+    rel_ok = all(done[k].true_res_norm <= tols[k] * done[k].b_norm * 1.0000001 for k in range(K))
+    dof = int(sum(int(dims[k]) ** 2 * (int(ps[k]) + 1) ** 2 for k in range(K)))
+    out = {"arrival_rate_per_s": rate, "arrival_window_s": seconds, "requests": K,
+           "solves_per_s": K / wall, "dof_per_s": dof / wall, "wall_s": wall,
+           "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                         "max": float(lat.max())},
+           "max_queued": max_queued, "max_running": max_running,
+           "all_converged": all(s == sipg.OK for s in statuses), "all_meet_tolerance": rel_ok,
+           "mixed_marked": int(mixed.sum()),
+           "note": ("requests marked mixed precision are served by the FP64 path: FP32 inner CG "
+                    "needed 5-6x the FP64 time on these operators (eps32 * kappa ~ 1)")}
+    log(f"serving: {K} requests in {wall:.1f} s, {K / wall:.1f} solves/s, p50 {out['latency_s']['p50']:.3f} s, "
+        f"p99 {out['latency_s']['p99']:.3f} s")
+    return out
+
+
 def hbm_traffic(n):
     """HBM bytes per apply launch from two rocprofv3 counter runs (FETCH_SIZE,
     then WRITE_SIZE) of `bench.py --profile-apply`, each scaled so that the
@@ -144,6 +275,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 2048-instance batched leg")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="arrival window of the serving leg (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="arrivals per second")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs that measure roofline.traffic")
     ap.add_argument("--profile-apply", type=int, default=0, metavar="N",
@@ -237,6 +372,9 @@ def main():
     it = int(np.median(iters))
     log(f"timed ({solver}): {ms_step:.1f} ms/solve, {value:.3e} DoF/s")
 
+    batched = None if args.no_batch else batched_leg(sipg)
+    serving = serving_leg(sipg, args.serve_seconds, args.serve_rate) if args.serve_seconds > 0 else None
+
     traffic, traffic_note = (None, "not measured (--no-traffic)")
     if not args.no_traffic:
         traffic, traffic_note = hbm_traffic(n)
@@ -268,6 +406,8 @@ def main():
                            "iterations": it64[-1], "true_relres": rel64},
             "mixed": mixed,
             "apply_us_f64": ms_apply * 1e3,
+            "batched": batched,
+            "serving": serving,
         },
         "roofline": {
             "bound": "hbm",

@@ -15,10 +15,12 @@
 //                  memory, so shrinking them as instances finish keeps the
 //                  graph valid;
 //   host loop      one replay in flight ahead; per-instance states are
-//                  polled one replay behind.  An instance whose recursion
-//                  met the tolerance gets a true residual b - A x between two
-//                  replays (finish confirmed, or the iteration resumes from
-//                  it), and finished instances leave the lists.
+//                  polled one replay behind.  The host picks the instances
+//                  whose r is to be replaced by b - A x in the first
+//                  iteration of a coming replay (reliable updates, as in
+//                  sipg_cg_ex: periodically, and once the recursion meets
+//                  the tolerance, which then finishes only on that true
+//                  residual), and finished instances leave the lists.
 //
 // The block shape of the operator kernel is chosen per degree by timing the
 // candidates on the batch's own instances (sipg_batch_tune).
@@ -37,7 +39,7 @@ namespace {
 
 constexpr int64_t kChunk = 8192;   // vector elements per chunk
 constexpr int kGraphIters = 8;     // even: every replay starts on slot 0
-enum { SET_ACTIVE = 0, SET_ALL = 1, SET_CHECK = 2, NSETS = 3 };
+enum { SET_ACTIVE = 0, SET_ALL = 1, SET_REPL = 2, NSETS = 3 };
 
 struct Lists {
     std::vector<std::vector<batch_item>> items;   // per group
@@ -70,9 +72,8 @@ struct sipg_batch {
     batch_inst *d_inst = nullptr;
     cg_state *d_state = nullptr;
     cg_state *h_state = nullptr;         // pinned, 2 x ninst
-    int *d_maxit = nullptr;
-    int *d_restart = nullptr;
-    double *ws = nullptr;                // R, P, AP
+    batch_ctl *d_ctl = nullptr;
+    double *ws = nullptr;                // R, P, AP, Z (increments since the last true residual)
     double *d_item_part = nullptr, *d_chunk_part = nullptr, *d_part2 = nullptr, *d_part3 = nullptr;
     int item_total = 0, chunk_cap = 0;
     int *d_counts = nullptr;             // NSETS x (2 + ngroups)
@@ -94,6 +95,7 @@ struct sipg_batch {
     double *R() { return ws; }
     double *Pv() { return ws + total(); }
     double *AP() { return ws + 2 * total(); }
+    double *Z() { return ws + 3 * total(); }
 };
 
 namespace {
@@ -176,8 +178,11 @@ int apply_set(sipg_batch &b, int set, const double *x, double *y) {
     return SIPG_OK;
 }
 
-int enqueue_iteration(sipg_batch &b, double *X, int slot) {
-    const int A = SET_ACTIVE;
+// One CG iteration of the active instances.  With `repl`, the instances of
+// the replacement set get r = b - A x between the x/r update and the p
+// update (reliable update; the set is empty in most replays).
+int enqueue_iteration(sipg_batch &b, double *X, const double *Bv, int slot, bool repl) {
+    const int A = SET_ACTIVE, Q = SET_REPL;
     int rc;
     for (size_t g = 0; g < b.groups.size(); ++g) {
         auto &G = b.groups[g];
@@ -189,11 +194,20 @@ int enqueue_iteration(sipg_batch &b, double *X, int slot) {
     if ((rc = launch_batch_alpha(b.d_list[A], b.count(A, 0), b.ninst, b.d_item_range[A], b.d_item_part,
                                  b.d_state, slot, b.stream)))
         return rc;
-    if ((rc = launch_batch_update_xr(X, b.Pv(), b.R(), b.AP(), b.d_chunks[A], b.count(A, 1), b.chunk_cap,
+    // increments go to z; x changes only at a true residual (group update)
+    if ((rc = launch_batch_update_xr(b.Z(), b.Pv(), b.R(), b.AP(), b.d_chunks[A], b.count(A, 1), b.chunk_cap,
                                      b.d_chunk_part, b.d_state, slot, b.stream)))
         return rc;
+    if (repl) {
+        if ((rc = launch_batch_fold(X, b.Z(), b.d_chunks[Q], b.count(Q, 1), b.chunk_cap, b.stream)) ||
+            (rc = apply_set(b, Q, X, b.AP())) ||
+            (rc = launch_batch_residual(Bv, b.AP(), b.R(), nullptr, b.d_chunks[Q], b.count(Q, 1), b.chunk_cap,
+                                        b.d_part2, b.d_part3, b.stream)) ||
+            (rc = launch_batch_mark(b.d_list[Q], b.count(Q, 0), b.ninst, b.d_state, slot, b.stream)))
+            return rc;
+    }
     if ((rc = launch_batch_beta(b.d_list[A], b.count(A, 0), b.ninst, b.d_chunk_range[A], b.d_chunk_part,
-                                b.d_state, slot, b.stream)))
+                                b.d_chunk_range[Q], b.d_part2, b.d_state, slot, b.stream)))
         return rc;
     return launch_batch_update_p(b.R(), b.Pv(), b.d_chunks[A], b.count(A, 1), b.chunk_cap, b.d_state, slot,
                                  b.stream);
@@ -326,7 +340,7 @@ int sipg_batch_destroy(sipg_batch *b) {
     for (auto &G : b->groups)
         for (int s = 0; s < NSETS; ++s)
             if (G.d_items[s]) (void)hipFree(G.d_items[s]);
-    void *dev[] = {b->d_inst, b->d_state, b->d_maxit, b->d_restart, b->ws, b->d_item_part, b->d_chunk_part,
+    void *dev[] = {b->d_inst, b->d_state, b->d_ctl, b->ws, b->d_item_part, b->d_chunk_part,
                    b->d_part2, b->d_part3, b->d_counts, b->d_modes};
     for (void *q : dev)
         if (q) (void)hipFree(q);
@@ -393,8 +407,8 @@ int sipg_batch_create(sipg_batch **out, int ninst, const int *dimx, const int *d
     b->chunk_cap = 0;
     for (int i = 0; i < ninst; ++i) b->chunk_cap += (int)((b->off[i + 1] - b->off[i] + kChunk - 1) / kChunk);
     if ((rc = dmalloc(&b->d_inst, ninst)) || (rc = dmalloc(&b->d_state, ninst)) ||
-        (rc = dmalloc(&b->d_maxit, ninst)) || (rc = dmalloc(&b->d_restart, ninst)) ||
-        (rc = dmalloc(&b->ws, 3 * (size_t)b->total())) || (rc = dmalloc(&b->d_chunk_part, b->chunk_cap)) ||
+        (rc = dmalloc(&b->d_ctl, ninst)) ||
+        (rc = dmalloc(&b->ws, 4 * (size_t)b->total())) || (rc = dmalloc(&b->d_chunk_part, b->chunk_cap)) ||
         (rc = dmalloc(&b->d_part2, b->chunk_cap)) || (rc = dmalloc(&b->d_part3, b->chunk_cap)) ||
         (rc = dmalloc(&b->d_counts, NSETS * b->ncount())) || (rc = dmalloc(&b->d_modes, 192)))
         return fail(rc);
@@ -413,7 +427,7 @@ int sipg_batch_create(sipg_batch **out, int ninst, const int *dimx, const int *d
         return fail(SIPG_ERR_NOMEM);
     }
     if ((e = hipMemcpy(b->d_inst, hi.data(), ninst * sizeof(batch_inst), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemset(b->ws, 0, 3 * (size_t)b->total() * sizeof(double))) != hipSuccess) {
+        (e = hipMemset(b->ws, 0, 4 * (size_t)b->total() * sizeof(double))) != hipSuccess) {
         set_last_hip_error(e);
         return fail(SIPG_ERR_HIP);
     }
@@ -536,6 +550,7 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
     if (!b || !x || !rhs || !opts) return SIPG_ERR_INVALID;
     if (!(opts->tol >= 0.0) || (opts->tol_mode != SIPG_TOL_ABS && opts->tol_mode != SIPG_TOL_REL))
         return SIPG_ERR_INVALID;
+    if (!(opts->replace_factor >= 0.0 && opts->replace_factor < 1.0)) return SIPG_ERR_INVALID;
     if (overlaps(x, rhs, (size_t)b->total() * 8)) return SIPG_ERR_ALIAS;
     int rc = ensure_tuned(*b);
     if (rc) return rc;
@@ -545,25 +560,28 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
     const double *Bv = (const double *)rhs;
     const bool true_stop = opts->true_residual_stop != 0;
     std::vector<int> maxit(ni);
+    std::vector<batch_ctl> ctl(ni);
     for (int i = 0; i < ni; ++i) {
         const int64_t n = B.off[i + 1] - B.off[i];
         maxit[i] = (int)std::min<int64_t>(opts->maxit > 0 ? (int64_t)opts->maxit : n, INT_MAX);
+        ctl[i] = batch_ctl{opts->tol, opts->tol_mode, maxit[i]};
     }
-    SIPG_HIP_TRY(hipMemcpyAsync(B.d_maxit, maxit.data(), ni * sizeof(int), hipMemcpyHostToDevice, B.stream));
+    SIPG_HIP_TRY(hipMemcpy(B.d_ctl, ctl.data(), ni * sizeof(batch_ctl), hipMemcpyHostToDevice));
 
-    // r = b - A x, p = r, initial states
+    // r = b - A x, p = r, z = 0, initial states
+    SIPG_HIP_TRY(hipMemsetAsync(B.Z(), 0, (size_t)B.total() * sizeof(double), B.stream));
     if ((rc = apply_set(B, SET_ALL, X, B.AP()))) return rc;
     if ((rc = launch_batch_residual(Bv, B.AP(), B.R(), B.Pv(), B.d_chunks[SET_ALL], B.count(SET_ALL, 1),
                                     B.chunk_cap, B.d_part2, B.d_part3, B.stream)))
         return rc;
     if ((rc = launch_batch_init(B.d_list[SET_ALL], B.count(SET_ALL, 0), ni, B.d_chunk_range[SET_ALL], B.d_part2,
-                                B.d_part3, B.d_state, opts->tol, opts->tol_mode, B.d_maxit, B.stream)))
+                                B.d_part3, B.d_state, B.d_ctl, B.stream)))
         return rc;
     cg_state *hs0 = B.h_state, *hs1 = B.h_state + ni;
     SIPG_HIP_TRY(hipMemcpyAsync(hs0, B.d_state, ni * sizeof(cg_state), hipMemcpyDeviceToHost, B.stream));
     SIPG_HIP_TRY(hipStreamSynchronize(B.stream));
 
-    enum { ACTIVE, CHECKING, FINISHED };
+    enum { ACTIVE, FINISHED };
     std::vector<char> phase(ni, ACTIVE);
     int n_open = 0;
     for (int i = 0; i < ni; ++i) {
@@ -578,7 +596,10 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
         build_lists(B, in_set, L);
         return upload_lists(B, SET_ACTIVE, L);
     };
-    if ((rc = upload_active())) return rc;
+    std::vector<char> none(ni, 0);
+    Lists Q;
+    build_lists(B, none, Q);
+    if ((rc = upload_active()) || (rc = upload_lists(B, SET_REPL, Q))) return rc;
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -594,7 +615,7 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
         SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
         SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
         SIPG_HIP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeRelaxed));
-        for (int it = 0; it < kGraphIters && !rc; ++it) rc = enqueue_iteration(B, X, it & 1);
+        for (int it = 0; it < kGraphIters && !rc; ++it) rc = enqueue_iteration(B, X, Bv, it & 1, it == 0);
         const hipError_t ec = hipStreamEndCapture(B.stream, &graph);
         if (rc || ec != hipSuccess) {
             if (ec != hipSuccess) set_last_hip_error(ec);
@@ -609,8 +630,8 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
         }
     }
     // One replay in flight ahead; state copy j is processed while replay
-    // j+1 runs, and whatever it decides (checks, shorter lists) is enqueued
-    // before copy j+1, which therefore shows its outcome.
+    // j+1 runs, and whatever it decides (replacement set, shorter lists) is
+    // enqueued before replay j+2 and copy j+1.
     int64_t replays = 0;
     const int64_t max_replays = (*std::max_element(maxit.begin(), maxit.end()) + kGraphIters) / kGraphIters + 64;
     int finished_since = 0;
@@ -618,6 +639,8 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
     for (int i = 0; i < ni; ++i)
         if (phase[i] != FINISHED) open_work += B.off[i + 1] - B.off[i];
     int64_t done_work = 0;
+    std::vector<int> nrep_req(ni, 0);
+    const double rf2 = opts->replace_factor * opts->replace_factor;
     auto launch_replay = [&]() -> int {
         SIPG_HIP_TRY(hipGraphLaunch(exec, B.stream));
         ++replays;
@@ -638,42 +661,34 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
         for (;;) {
             SIPG_HIP_TRY(hipEventSynchronize(ev[cur]));
             const cg_state *hs = cur ? hs1 : hs0;
-            std::vector<int> to_check;
+            std::vector<char> repl(ni, 0);
+            bool any_repl = false;
             for (int i = 0; i < ni; ++i) {
                 if (phase[i] == FINISHED) continue;
                 const cg_slot &c = hs[i].s[0];
-                if (hs[i].err) {
+                if (hs[i].err || c.done) {
                     phase[i] = FINISHED;
-                } else if (phase[i] == CHECKING) {
-                    phase[i] = (c.done && c.checked) ? FINISHED : ACTIVE;
-                } else if (c.done) {
-                    if (c.checked || !true_stop) phase[i] = FINISHED;
-                    else to_check.push_back(i);
-                }
-                if (phase[i] == FINISHED) {
                     --n_open;
                     ++finished_since;
                     done_work += B.off[i + 1] - B.off[i];
+                    continue;
+                }
+                // a replacement is requested once the previous one shows in
+                // the polled state; the final one once the recursion has
+                // fallen 4x below the last true residual (as in sipg_cg_ex)
+                if (hs[i].nrep != nrep_req[i]) continue;
+                const bool check = true_stop && (c.flags & kSlotWantCheck) && c.rr <= 0.0625 * hs[i].rr_ref;
+                const bool periodic = rf2 > 0.0 && c.rr <= rf2 * hs[i].rr_ref;
+                if (check || periodic) {
+                    repl[i] = 1;
+                    any_repl = true;
+                    ++nrep_req[i];
                 }
             }
-            if (!to_check.empty()) {
-                std::vector<char> chk(ni, 0);
-                for (int i : to_check) {
-                    chk[i] = 1;
-                    phase[i] = CHECKING;
-                }
-                Lists C;
-                build_lists(B, chk, C);
-                if ((rc = upload_lists(B, SET_CHECK, C)) || (rc = apply_set(B, SET_CHECK, X, B.AP())) ||
-                    (rc = launch_batch_residual(Bv, B.AP(), B.R(), nullptr, B.d_chunks[SET_CHECK],
-                                                B.count(SET_CHECK, 1), B.chunk_cap, B.d_part2, B.d_part3,
-                                                B.stream)) ||
-                    (rc = launch_batch_check(B.d_list[SET_CHECK], B.count(SET_CHECK, 0), ni,
-                                             B.d_chunk_range[SET_CHECK], B.d_part2, B.d_state, 0, B.d_restart,
-                                             B.stream)) ||
-                    (rc = launch_batch_restart_p(B.R(), B.Pv(), B.d_chunks[SET_CHECK], B.count(SET_CHECK, 1),
-                                                 B.chunk_cap, B.d_restart, B.stream)))
-                    break;
+            // the replacement set of replay j+2 (empty most of the time)
+            if (any_repl || Q.list.size()) {
+                build_lists(B, repl, Q);
+                if ((rc = upload_lists(B, SET_REPL, Q))) break;
             }
             // shorter lists once a tenth of the open work has finished
             if (finished_since > 0 && n_open > 0 && 10 * done_work >= open_work) {
@@ -694,6 +709,11 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
     }
     cleanup();
     if (rc && rc != SIPG_NOT_CONVERGED) return rc;
+    // increments after each instance's last true residual (none unless it
+    // stopped on its iteration cap)
+    int frc = launch_batch_fold(X, B.Z(), B.d_chunks[SET_ALL], B.count(SET_ALL, 1), B.chunk_cap, B.stream);
+    if (frc) return frc;
+    SIPG_HIP_TRY(hipStreamSynchronize(B.stream));
     SIPG_HIP_TRY(hipMemcpy(hs0, B.d_state, ni * sizeof(cg_state), hipMemcpyDeviceToHost));
     int status = rc;
     for (int i = 0; i < ni; ++i) {
@@ -702,8 +722,7 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
         int st = SIPG_OK;
         if (f.err == 2) st = SIPG_ERR_BREAKDOWN;
         else if (f.err == 3) st = SIPG_ERR_NONFINITE;
-        else if (!(std::sqrt(c.rr) <= (true_stop ? f.stop_true : f.stop)) || (true_stop && !c.checked))
-            st = SIPG_NOT_CONVERGED;
+        else if (!(std::sqrt(c.rr) <= f.stop) || (true_stop && !c.checked)) st = SIPG_NOT_CONVERGED;
         if (info) {
             info[i].status = st;
             info[i].iters = c.it;

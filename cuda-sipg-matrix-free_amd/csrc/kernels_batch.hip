@@ -39,21 +39,24 @@ void batch_apply_kernel(const T *__restrict__ X, T *__restrict__ Y, const batch_
     const int lw = xcd_remap(blockIdx.x, NG);
     const int kbeg = (int)((int64_t)lw * n / NG);
     const int kend = (int)((int64_t)(lw + 1) * n / NG);
-    // first item >= k of an unfinished instance: 64 items per step, one per
-    // lane, then the lowest set bit of the ballot (the same in every wave)
+    // first item >= k of an unfinished instance: SCAN items per step, one
+    // per lane, then the lowest set bit of the ballot.  SCAN is the lane
+    // count of the smallest wave (the last one is partial when NT is not a
+    // multiple of 64), so every wave reaches the same answer.
+    constexpr int SCAN = (G::NT % 64) ? (G::NT % 64) : 64;
     auto next_live = [&](int k) -> int {
         if (!DOT) return k;
         const int lane = threadIdx.x & 63;
         while (k < kend) {
             const int kk = k + lane;
             bool lv = false;
-            if (kk < kend) {
+            if (lane < SCAN && kk < kend) {
                 const cg_state *st = states + items[kk].inst;
                 lv = !(st->s[slot].done | st->err);
             }
             const unsigned long long m = __ballot(lv);
             if (m) return __builtin_amdgcn_readfirstlane(k + __ffsll((long long)m) - 1);
-            k += 64;
+            k += SCAN;
         }
         return kend;
     };
@@ -162,10 +165,17 @@ __global__ __launch_bounds__(NT) void batch_update_xr_kernel(T *__restrict__ X, 
 }
 
 // beta = r.r(new) / r.r(old) per active instance; writes the next slot
-// (conjugate_gradient_gpu.hpp:85-95 and the while-condition of :69).
+// (conjugate_gradient_gpu.hpp:85-95 and the while-condition of :69).  An
+// instance whose r was replaced by b - A x in this iteration (kSlotReplaced,
+// set by batch_mark_kernel) takes r.r(new) from the replacement's chunk
+// sums and keeps its direction (reliable update); with st->true_stop it
+// finishes only on such a true residual, and a recursion that meets the
+// tolerance raises kSlotWantCheck for the host to schedule one.
 __global__ __launch_bounds__(NT) void batch_beta_kernel(const int *__restrict__ list, const int *__restrict__ count,
                                                         const int2 *__restrict__ chunk_range,
                                                         const double *__restrict__ chunk_part,
+                                                        const int2 *__restrict__ repl_range,
+                                                        const double *__restrict__ repl_part,
                                                         cg_state *states, int slot) {
     __shared__ double red[NT / 64];
     const int n = *count;
@@ -177,20 +187,55 @@ __global__ __launch_bounds__(NT) void batch_beta_kernel(const int *__restrict__ 
             if (threadIdx.x == 0) st->s[slot ^ 1] = cur;           // carry forward
             continue;
         }
-        const int2 r = chunk_range[i];
-        const double rr_new = range_sum(chunk_part, r.x, r.y, red);
+        const bool replaced = (cur.flags & kSlotReplaced) != 0;
+        const int2 r = replaced ? repl_range[i] : chunk_range[i];
+        const double rr_new = range_sum(replaced ? repl_part : chunk_part, r.x, r.y, red);
         if (threadIdx.x == 0) {
+            const bool met = !(sqrt(rr_new) > st->stop);
+            const bool need_true = st->true_stop && !replaced;
             cg_slot nx;
             nx.rr = rr_new;
             nx.alpha = 0.0;
             nx.beta = rr_new / cur.rr;
             nx.pAp = 0.0;
             nx.it = cur.it + 1;
-            nx.done = (!(sqrt(rr_new) > st->stop) || nx.it >= st->maxit) ? 1 : 0;
-            nx.checked = 0;
-            nx.flags = 0;
+            nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
+            nx.checked = replaced ? 1 : 0;
+            nx.flags = (met && need_true) ? kSlotWantCheck : 0;
             st->s[slot ^ 1] = nx;
+            if (replaced) st->rr_ref = rr_new;
             if (!isfinite(rr_new)) st->err = 3;
+        }
+    }
+}
+
+// marks the listed instances' slot as replaced (r is b - A x now and the
+// replacement chunk sums hold its r.r); a no-op for finished instances
+__global__ __launch_bounds__(NT) void batch_mark_kernel(const int *__restrict__ list, const int *__restrict__ count,
+                                                        cg_state *states, int slot) {
+    const int n = *count;
+    for (int k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
+        cg_state *st = states + list[k];
+        cg_slot c = st->s[slot];
+        if (c.done | st->err) continue;
+        c.flags |= kSlotReplaced;
+        st->s[slot] = c;
+        st->nrep += 1;
+    }
+}
+
+// x += z; z = 0 over the listed chunks: the group update (van der Vorst &
+// Ye 2000) folds the increments accumulated since the last true residual
+template <typename T>
+__global__ __launch_bounds__(NT) void batch_fold_kernel(T *__restrict__ X, T *__restrict__ Z,
+                                                        const batch_chunk *__restrict__ chunks,
+                                                        const int *__restrict__ count) {
+    const int n = *count;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const batch_chunk c = chunks[k];
+        for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
+            X[i] += Z[i];
+            Z[i] = T(0);
         }
     }
 }
@@ -247,7 +292,7 @@ __global__ __launch_bounds__(NT) void batch_init_kernel(const int *__restrict__ 
                                                         const int2 *__restrict__ chunk_range,
                                                         const double *__restrict__ part_rr,
                                                         const double *__restrict__ part_bb, cg_state *states,
-                                                        double tol, int tol_mode, const int *__restrict__ maxit) {
+                                                        const batch_ctl *__restrict__ ctl) {
     __shared__ double red[NT / 64];
     const int n = *count;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
@@ -257,12 +302,13 @@ __global__ __launch_bounds__(NT) void batch_init_kernel(const int *__restrict__ 
         const double bb = range_sum(part_bb, r.x, r.y, red);
         if (threadIdx.x == 0) {
             cg_state *st = states + i;
-            const double stop = tol_mode == SIPG_TOL_REL ? tol * sqrt(bb) : tol;
+            const batch_ctl c = ctl[i];
+            const double stop = c.tol_mode == SIPG_TOL_REL ? c.tol * sqrt(bb) : c.tol;
             cg_slot s0;
             s0.rr = rr;
             s0.alpha = s0.beta = s0.pAp = 0.0;
             s0.it = 0;
-            s0.done = (sqrt(rr) > stop && maxit[i] > 0) ? 0 : 1;
+            s0.done = (sqrt(rr) > stop && c.maxit > 0) ? 0 : 1;
             s0.checked = 1;
             s0.flags = 0;
             st->s[0] = s0;
@@ -272,67 +318,11 @@ __global__ __launch_bounds__(NT) void batch_init_kernel(const int *__restrict__ 
             st->stop_true = stop;
             st->rr0 = rr;
             st->rr_ref = rr;
-            st->maxit = maxit[i];
+            st->maxit = c.maxit;
             st->nrep = 0;
             st->err = (!isfinite(rr) || !isfinite(bb)) ? 3 : 0;
             st->true_stop = 1;
         }
-    }
-}
-
-// After r = b - A x for the listed (finished) instances: confirm the finish
-// on the true residual, or resume the iteration from it.  A resumed
-// instance restarts from p = r (a direction built from the recursive
-// residual does not fit the replaced one) and its recursion must now reach
-// the threshold tightened by the gap just observed, so that the next check
-// lands below the true one.
-__global__ __launch_bounds__(NT) void batch_check_kernel(const int *__restrict__ list, const int *__restrict__ count,
-                                                         const int2 *__restrict__ chunk_range,
-                                                         const double *__restrict__ part_rr, cg_state *states,
-                                                         int slot, int *__restrict__ restart) {
-    __shared__ double red[NT / 64];
-    const int n = *count;
-    for (int k = blockIdx.x; k < n; k += gridDim.x) {
-        const int i = list[k];
-        const int2 r = chunk_range[i];
-        const double rr = range_sum(part_rr, r.x, r.y, red);
-        if (threadIdx.x == 0) {
-            cg_state *st = states + i;
-            cg_slot c = st->s[slot];
-            st->rr_ref = rr;
-            st->nrep += 1;
-            restart[i] = 0;
-            if (!isfinite(rr)) {
-                st->err = 3;
-            } else if (!(sqrt(rr) > st->stop_true) || c.it >= st->maxit) {
-                c.done = 1;
-                c.checked = 1;
-            } else {
-                c.done = 0;
-                c.checked = 0;
-                restart[i] = 1;
-                // recursion gap g = sqrt(rr / rr_rec) >= 1: aim the recursion
-                // at stop_true / g (at most 100x tighter per check)
-                const double g = sqrt(rr / fmax(c.rr, 1e-300));
-                st->stop = fmin(st->stop, st->stop_true / fmin(fmax(g, 1.0), 100.0));
-            }
-            c.rr = rr;
-            st->s[slot] = c;
-            st->s[slot ^ 1] = c;
-        }
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(NT) void batch_restart_p_kernel(const T *__restrict__ Rv, T *__restrict__ Pv,
-                                                             const batch_chunk *__restrict__ chunks,
-                                                             const int *__restrict__ count,
-                                                             const int *__restrict__ restart) {
-    const int n = *count;
-    for (int k = blockIdx.x; k < n; k += gridDim.x) {
-        const batch_chunk c = chunks[k];
-        if (!restart[c.inst]) continue;
-        for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) Pv[i] = Rv[i];
     }
 }
 
@@ -433,9 +423,10 @@ int launch_batch_update_xr(double *X, const double *Pv, double *Rv, const double
 }
 
 int launch_batch_beta(const int *list, const int *count, int max_inst, const int2 *chunk_range,
-                      const double *chunk_part, cg_state *states, int slot, hipStream_t s) {
+                      const double *chunk_part, const int2 *repl_range, const double *repl_part,
+                      cg_state *states, int slot, hipStream_t s) {
     hipLaunchKernelGGL(batch_beta_kernel, dim3(inst_blocks(max_inst)), dim3(NT), 0, s, list, count, chunk_range,
-                       chunk_part, states, slot);
+                       chunk_part, repl_range, repl_part, states, slot);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }
@@ -457,26 +448,25 @@ int launch_batch_residual(const double *B, const double *AX, double *Rv, double 
 }
 
 int launch_batch_init(const int *list, const int *count, int max_inst, const int2 *chunk_range,
-                      const double *part_rr, const double *part_bb, cg_state *states, double tol, int tol_mode,
-                      const int *maxit, hipStream_t s) {
+                      const double *part_rr, const double *part_bb, cg_state *states, const batch_ctl *ctl,
+                      hipStream_t s) {
     hipLaunchKernelGGL(batch_init_kernel, dim3(inst_blocks(max_inst)), dim3(NT), 0, s, list, count, chunk_range,
-                       part_rr, part_bb, states, tol, tol_mode, maxit);
+                       part_rr, part_bb, states, ctl);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }
 
-int launch_batch_check(const int *list, const int *count, int max_inst, const int2 *chunk_range,
-                       const double *part_rr, cg_state *states, int slot, int *restart, hipStream_t s) {
-    hipLaunchKernelGGL(batch_check_kernel, dim3(inst_blocks(max_inst)), dim3(NT), 0, s, list, count, chunk_range,
-                       part_rr, states, slot, restart);
+int launch_batch_fold(double *X, double *Z, const batch_chunk *chunks, const int *count, int max_chunks,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(batch_fold_kernel<double>, dim3(vec_blocks(max_chunks)), dim3(NT), 0, s, X, Z, chunks, count);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }
 
-int launch_batch_restart_p(const double *Rv, double *Pv, const batch_chunk *chunks, const int *count,
-                           int max_chunks, const int *restart, hipStream_t s) {
-    hipLaunchKernelGGL(batch_restart_p_kernel<double>, dim3(vec_blocks(max_chunks)), dim3(NT), 0, s, Rv, Pv, chunks,
-                       count, restart);
+int launch_batch_mark(const int *list, const int *count, int max_inst, cg_state *states, int slot,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(batch_mark_kernel, dim3(std::max(1, std::min((max_inst + NT - 1) / NT, 64))), dim3(NT), 0,
+                       s, list, count, states, slot);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

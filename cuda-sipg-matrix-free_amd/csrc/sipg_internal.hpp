@@ -88,6 +88,11 @@ struct batch_inst {
     int dimx, dimy;
     double pen;
 };
+// Per-instance solver parameters.
+struct batch_ctl {
+    double tol;
+    int tol_mode, maxit;
+};
 // A BX x BY block of an instance for the operator kernel.
 struct batch_item {
     int inst, x0, y0, pad;
@@ -160,19 +165,22 @@ int launch_batch_update_xr(double *X, const double *Pv, double *Rv, const double
                            const batch_chunk *chunks, const int *count, int max_chunks,
                            double *chunk_part, const cg_state *states, int slot, hipStream_t s);
 int launch_batch_beta(const int *list, const int *count, int max_inst, const int2 *chunk_range,
-                      const double *chunk_part, cg_state *states, int slot, hipStream_t s);
+                      const double *chunk_part, const int2 *repl_range, const double *repl_part,
+                      cg_state *states, int slot, hipStream_t s);
 int launch_batch_update_p(const double *Rv, double *Pv, const batch_chunk *chunks, const int *count,
                           int max_chunks, const cg_state *states, int slot, hipStream_t s);
 int launch_batch_residual(const double *B, const double *AX, double *Rv, double *Pv,
                           const batch_chunk *chunks, const int *count, int max_chunks, double *part_rr,
                           double *part_bb, hipStream_t s);
 int launch_batch_init(const int *list, const int *count, int max_inst, const int2 *chunk_range,
-                      const double *part_rr, const double *part_bb, cg_state *states, double tol,
-                      int tol_mode, const int *maxit, hipStream_t s);
-int launch_batch_check(const int *list, const int *count, int max_inst, const int2 *chunk_range,
-                       const double *part_rr, cg_state *states, int slot, int *restart, hipStream_t s);
-int launch_batch_restart_p(const double *Rv, double *Pv, const batch_chunk *chunks, const int *count,
-                           int max_chunks, const int *restart, hipStream_t s);
+                      const double *part_rr, const double *part_bb, cg_state *states, const batch_ctl *ctl,
+                      hipStream_t s);
+// x += z; z = 0 over the listed chunks (group update)
+int launch_batch_fold(double *X, double *Z, const batch_chunk *chunks, const int *count, int max_chunks,
+                      hipStream_t s);
+// sets kSlotReplaced on the listed instances (after their r = b - A x)
+int launch_batch_mark(const int *list, const int *count, int max_inst, cg_state *states, int slot,
+                      hipStream_t s);
 // reference order <-> element-major device order (to_device = 1: ref -> dev)
 int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t noe, int nq,
                   hipStream_t s);

@@ -117,6 +117,13 @@ def _declare(L):
         "sipg_batch_upload": (i, [_vp, i, _dp, _vp]),
         "sipg_batch_download": (i, [_vp, i, _vp, _dp]),
         "sipg_batch_cg": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
+        "sipg_server_create": (i, [ctypes.POINTER(_vp), d, _vp]),
+        "sipg_server_destroy": (i, [_vp]),
+        "sipg_server_set_shape": (i, [_vp, i, i, i]),
+        "sipg_server_submit": (i, [_vp, i, i, d, i, _dp, _ip, _ip, d, i, i, ctypes.POINTER(i64)]),
+        "sipg_server_progress": (i, [_vp, _ip]),
+        "sipg_server_take": (i, [_vp, ctypes.POINTER(i64), ctypes.POINTER(CgInfo), _dp, i64]),
+        "sipg_server_stats": (i, [_vp, _ip, _ip, _ip, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -417,8 +424,60 @@ class Batch:
         check(lib().sipg_batch_download(self.h, inst, dev.ptr, out.ctypes.data_as(_dp)), "batch_download")
         return out
 
-    def cg(self, x, b, tol, tol_mode=TOL_REL, maxit=0, true_stop=True):
-        o = CgOpts(tol, tol_mode, maxit, 0.0, 1 if true_stop else 0, 0)
+    def cg(self, x, b, tol, tol_mode=TOL_REL, maxit=0, true_stop=True, replace_factor=0.0):
+        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1 if true_stop else 0, 0)
         infos = (CgInfo * self.ninst)()
         st = lib().sipg_batch_cg(self.h, x.ptr, b.ptr, ctypes.byref(o), infos)
         return st, list(infos)
+
+
+class Server:
+    """Continuous batching of independent FP64 solves (sipg_server_*)."""
+
+    def __init__(self, hbm_budget_bytes, stream=None):
+        self.h = _vp()
+        check(lib().sipg_server_create(ctypes.byref(self.h), float(hbm_budget_bytes), _vp(stream) if stream else None),
+              "server_create")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.sipg_server_destroy(self.h)
+            self.h = None
+
+    def set_shape(self, p, bx, by):
+        return lib().sipg_server_set_shape(self.h, p, bx, by)
+
+    def submit(self, dim, p, pen, modes, tol, tol_mode=TOL_REL, maxit=0):
+        a, m, n = modes
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        rid = ctypes.c_int64()
+        check(lib().sipg_server_submit(self.h, dim, p, pen, len(a), a.ctypes.data_as(_dp), m.ctypes.data_as(_ip),
+                                       n.ctypes.data_as(_ip), tol, tol_mode, maxit, ctypes.byref(rid)),
+              "server_submit")
+        return rid.value
+
+    def progress(self):
+        nd = ctypes.c_int()
+        check(lib().sipg_server_progress(self.h, ctypes.byref(nd)), "server_progress")
+        return nd.value
+
+    def take(self, n_solution=0):
+        """(id, CgInfo, solution or None) of the oldest completed request, or None."""
+        rid = ctypes.c_int64()
+        info = CgInfo()
+        x = np.empty(n_solution) if n_solution else None
+        st = lib().sipg_server_take(self.h, ctypes.byref(rid), ctypes.byref(info),
+                                    x.ctypes.data_as(_dp) if x is not None else None, n_solution)
+        if st == ERR_NOT_FOUND:
+            return None
+        check(st, "server_take")
+        return rid.value, info, x
+
+    def stats(self):
+        q, r, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        b = ctypes.c_double()
+        check(lib().sipg_server_stats(self.h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(d), ctypes.byref(b)),
+              "server_stats")
+        return {"queued": q.value, "running": r.value, "done_held": d.value, "bytes_used": b.value}

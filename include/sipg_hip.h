@@ -227,12 +227,43 @@ int sipg_batch_rhs_fourier(sipg_batch *b, int inst, int nmodes, const double *a,
 int sipg_batch_upload(sipg_batch *b, int inst, const double *host, void *dev);
 int sipg_batch_download(sipg_batch *b, int inst, const void *dev, double *host);
 /* CG on every instance: x (in/out) and rhs are batch vectors.  Uses tol,
- * tol_mode, maxit (<= 0: n_i per instance) and true_residual_stop of opts
- * (an instance whose recursion meets the tolerance is confirmed on b - A x,
- * or resumes from it); replace_factor and restart are not used.  info, if
- * not null, receives one record per instance.  Returns SIPG_OK when every
+ * tol_mode, maxit (<= 0: n_i per instance), replace_factor and
+ * true_residual_stop of opts with the meaning of sipg_cg_ex (reliable
+ * updates keeping the search direction; restart is not used).  info, if not
+ * null, receives one record per instance.  Returns SIPG_OK when every
  * instance converged, else the first failing instance's status. */
 int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opts, sipg_cg_info *info);
+
+/* ---- serving: continuous batching of a request stream ------------------- */
+/* A server solves a stream of independent FP64 problems (square meshes).
+ * No counterpart in the reference.  Submitted requests wait in arrival
+ * order and are admitted as soon as a slot (up to 4096 on the device) and
+ * device memory within hbm_budget_bytes are free: a request of n unknowns
+ * needs 48 n bytes (solution, right-hand side and four CG vectors).  Each
+ * call of sipg_server_progress advances every admitted request by a few CG
+ * iterations (one replay of a captured hipGraph), confirms finished ones
+ * on their true residual, retires them and admits new ones; it blocks
+ * until that work is done.  A request's result does not depend on the other
+ * requests in flight.  Requests whose recursion meets the tolerance are
+ * checked on b - A x (as sipg_cg_ex with true_residual_stop). */
+typedef struct sipg_server sipg_server;
+int sipg_server_create(sipg_server **out, double hbm_budget_bytes, void *stream);
+int sipg_server_destroy(sipg_server *s);
+/* Block shape of the operator kernel for degree p (a sipg_batch_tune
+ * result; 8x4 by default), while no request of degree p is running. */
+int sipg_server_set_shape(sipg_server *s, int p, int bx, int by);
+/* Queue a request: dim x dim mesh of degree p, penalty pen, Fourier forcing
+ * (as sipg_rhs_fourier), tolerance (tol, tol_mode), maxit <= 0: n.
+ * SIPG_ERR_BUDGET if it can never fit the budget. */
+int sipg_server_submit(sipg_server *s, int dim, int p, double pen, int nmodes, const double *a,
+                       const int *m, const int *n, double tol, int tol_mode, int maxit, int64_t *id);
+/* One round (see above); *n_done = completed requests not yet taken. */
+int sipg_server_progress(sipg_server *s, int *n_done);
+/* Oldest completed request: its id and record, and (if x_host is not null,
+ * with room for n_cap values) its solution in the reference order; releases
+ * its memory.  SIPG_ERR_NOT_FOUND if none has completed. */
+int sipg_server_take(sipg_server *s, int64_t *id, sipg_cg_info *info, double *x_host, int64_t n_cap);
+int sipg_server_stats(const sipg_server *s, int *queued, int *running, int *done_held, double *bytes_used);
 
 /* ---- right-hand side and error norms (device) -------------------------- */
 /* Seeded Fourier forcing u = sum_k a_k sin(m_k pi x) sin(n_k pi y) drawn

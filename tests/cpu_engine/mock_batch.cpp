@@ -118,7 +118,7 @@ int launch_batch_update_xr(double *X, const double *Pv, double *Rv, const double
 }
 
 int launch_batch_beta(const int *list, const int *count, int, const int2 *chunk_range, const double *chunk_part,
-                      cg_state *states, int slot, hipStream_t) {
+                      const int2 *repl_range, const double *repl_part, cg_state *states, int slot, hipStream_t) {
     mock_run([=]() {
         for (int k = 0; k < *count; ++k) {
             cg_state *st = states + list[k];
@@ -127,16 +127,50 @@ int launch_batch_beta(const int *list, const int *count, int, const int2 *chunk_
                 st->s[slot ^ 1] = cur;
                 continue;
             }
-            const int2 r = chunk_range[list[k]];
+            const bool replaced = (cur.flags & kSlotReplaced) != 0;
+            const int2 r = replaced ? repl_range[list[k]] : chunk_range[list[k]];
+            const double *part = replaced ? repl_part : chunk_part;
             double rr = 0.0;
-            for (int i = r.x; i < r.y; ++i) rr += chunk_part[i];
+            for (int i = r.x; i < r.y; ++i) rr += part[i];
+            const bool met = !(std::sqrt(rr) > st->stop);
+            const bool need_true = st->true_stop && !replaced;
             cg_slot nx{};
             nx.rr = rr;
             nx.beta = rr / cur.rr;
             nx.it = cur.it + 1;
-            nx.done = (!(std::sqrt(rr) > st->stop) || nx.it >= st->maxit) ? 1 : 0;
+            nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
+            nx.checked = replaced ? 1 : 0;
+            nx.flags = (met && need_true) ? kSlotWantCheck : 0;
             st->s[slot ^ 1] = nx;
+            if (replaced) st->rr_ref = rr;
             if (!std::isfinite(rr)) st->err = 3;
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_fold(double *X, double *Z, const batch_chunk *chunks, const int *count, int max_chunks,
+                      hipStream_t) {
+    mock_run([=]() {
+        if (*count > max_chunks) std::abort();
+        for (int k = 0; k < *count; ++k)
+            for (int64_t i = chunks[k].begin; i < chunks[k].end; ++i) {
+                X[i] += Z[i];
+                Z[i] = 0.0;
+            }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_mark(const int *list, const int *count, int, cg_state *states, int slot, hipStream_t) {
+    mock_run([=]() {
+        for (int k = 0; k < *count; ++k) {
+            cg_state *st = states + list[k];
+            cg_slot c = st->s[slot];
+            if (c.done | st->err) continue;
+            c.flags |= kSlotReplaced;
+            st->s[slot] = c;
+            st->nrep += 1;
         }
     });
     return SIPG_OK;
@@ -178,8 +212,7 @@ int launch_batch_residual(const double *B, const double *AX, double *Rv, double 
 }
 
 int launch_batch_init(const int *list, const int *count, int, const int2 *chunk_range, const double *part_rr,
-                      const double *part_bb, cg_state *states, double tol, int tol_mode, const int *maxit,
-                      hipStream_t) {
+                      const double *part_bb, cg_state *states, const batch_ctl *ctl, hipStream_t) {
     mock_run([=]() {
         for (int k = 0; k < *count; ++k) {
             const int i = list[k];
@@ -190,10 +223,12 @@ int launch_batch_init(const int *list, const int *count, int, const int2 *chunk_
                 bb += part_bb[c];
             }
             cg_state *st = states + i;
-            const double stop = tol_mode == SIPG_TOL_REL ? tol * std::sqrt(bb) : tol;
+            const double tol = ctl[i].tol;
+            const int maxit_i = ctl[i].maxit;
+            const double stop = ctl[i].tol_mode == SIPG_TOL_REL ? tol * std::sqrt(bb) : tol;
             cg_slot s0{};
             s0.rr = rr;
-            s0.done = (std::sqrt(rr) > stop && maxit[i] > 0) ? 0 : 1;
+            s0.done = (std::sqrt(rr) > stop && maxit_i > 0) ? 0 : 1;
             s0.checked = 1;
             st->s[0] = s0;
             st->s[1] = s0;
@@ -202,55 +237,10 @@ int launch_batch_init(const int *list, const int *count, int, const int2 *chunk_
             st->stop_true = stop;
             st->rr0 = rr;
             st->rr_ref = rr;
-            st->maxit = maxit[i];
+            st->maxit = maxit_i;
             st->nrep = 0;
             st->err = (!std::isfinite(rr) || !std::isfinite(bb)) ? 3 : 0;
             st->true_stop = 1;
-        }
-    });
-    return SIPG_OK;
-}
-
-int launch_batch_check(const int *list, const int *count, int, const int2 *chunk_range, const double *part_rr,
-                       cg_state *states, int slot, int *restart, hipStream_t) {
-    mock_run([=]() {
-        for (int k = 0; k < *count; ++k) {
-            const int i = list[k];
-            const int2 r = chunk_range[i];
-            double rr = 0.0;
-            for (int c = r.x; c < r.y; ++c) rr += part_rr[c];
-            cg_state *st = states + i;
-            cg_slot c = st->s[slot];
-            st->rr_ref = rr;
-            st->nrep += 1;
-            restart[i] = 0;
-            if (!std::isfinite(rr)) {
-                st->err = 3;
-            } else if (!(std::sqrt(rr) > st->stop_true) || c.it >= st->maxit) {
-                c.done = 1;
-                c.checked = 1;
-            } else {
-                c.done = 0;
-                c.checked = 0;
-                restart[i] = 1;
-                const double g = std::sqrt(rr / std::fmax(c.rr, 1e-300));
-                st->stop = std::fmin(st->stop, st->stop_true / std::fmin(std::fmax(g, 1.0), 100.0));
-            }
-            c.rr = rr;
-            st->s[slot] = c;
-            st->s[slot ^ 1] = c;
-        }
-    });
-    return SIPG_OK;
-}
-
-int launch_batch_restart_p(const double *Rv, double *Pv, const batch_chunk *chunks, const int *count, int,
-                           const int *restart, hipStream_t) {
-    mock_run([=]() {
-        for (int k = 0; k < *count; ++k) {
-            const batch_chunk c = chunks[k];
-            if (!restart[c.inst]) continue;
-            for (int64_t i = c.begin; i < c.end; ++i) Pv[i] = Rv[i];
         }
     });
     return SIPG_OK;

@@ -11,6 +11,8 @@
 #include <cstring>
 #include <vector>
 
+#include "../../oracle/sipg_oracle.h"
+
 extern "C" {
 void orc_apply(int dimx, int dimy, int p, double pen, const double *x, double *y);
 void orc_apply_f32(int dimx, int dimy, int p, float pen, const float *x, float *y);
@@ -270,9 +272,18 @@ int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t 
     return SIPG_OK;
 }
 
-int launch_rhs_fourier(int, int, int, const void *, int, const double *, const int *, const int *, void *,
-                       hipStream_t) {
-    return SIPG_ERR_UNSUPPORTED;
+// the oracle's right-hand side of the Fourier forcing (g = 0), transposed
+// to the element-major order; FP64 only
+int launch_rhs_fourier(int dtype, int p, int dim, const void *, int nmodes, const double *a, const int *m,
+                       const int *n, void *b, hipStream_t) {
+    if (dtype != SIPG_F64) return SIPG_ERR_UNSUPPORTED;
+    orc_problem pb{0, nmodes, a, m, n};
+    const int64_t noe = (int64_t)dim * dim, nq = (int64_t)(p + 1) * (p + 1);
+    std::vector<double> br(noe * nq);
+    orc_rhs(dim, p, 5.0 * (p + 1) * (p + 1), &pb, br.data());   // pen does not enter with g = 0
+    for (int64_t e = 0; e < noe; ++e)
+        for (int64_t q = 0; q < nq; ++q) tp<double>(b)[e * nq + q] = br[q * noe + e];
+    return SIPG_OK;
 }
 
 }  // namespace sipg_internal

@@ -1,0 +1,105 @@
+// CPU test driver for the serving loop (serve.cpp against the mock runtime
+// and the CPU stand-ins of the batched launchers).  Prints JSON lines for
+// tests/test_cpu_engine.py.
+#include "../../include/sipg_hip.h"
+
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <vector>
+
+
+struct Req {
+    int dim, p;
+    double tol;
+    int nm;
+    double a[8];
+    int m[8], n[8];
+};
+
+static std::vector<Req> make_requests(int count) {
+    std::vector<Req> out;
+    unsigned long long s = 99;
+    auto next = [&]() { s = s * 6364136223846793005ULL + 1442695040888963407ULL; return s >> 33; };
+    for (int i = 0; i < count; ++i) {
+        Req r{};
+        r.dim = 3 + (int)(next() % 10);
+        r.p = 2 + (int)(next() % 5);
+        r.tol = (next() % 2) ? 1e-10 : 1e-8;
+        sipg_forcing_draw(1000 + i, 1, 4, 4, &r.nm, r.a, r.m, r.n);
+        out.push_back(r);
+    }
+    return out;
+}
+
+// solves `reqs` on one server; returns solutions by request index
+static std::map<int, std::vector<double>> serve(const std::vector<Req> &reqs, double budget, int *max_queued,
+                                                int *rc_out, std::vector<sipg_cg_info> *infos) {
+    std::map<int, std::vector<double>> sol;
+    sipg_server *S = nullptr;
+    int rc = sipg_server_create(&S, budget, nullptr);
+    if (rc) { *rc_out = rc; return sol; }
+    std::map<int64_t, int> index;
+    for (size_t i = 0; i < reqs.size(); ++i) {
+        const Req &r = reqs[i];
+        int64_t id = 0;
+        rc = sipg_server_submit(S, r.dim, r.p, 5.0 * (r.p + 1) * (r.p + 1), r.nm, r.a, r.m, r.n, r.tol,
+                                SIPG_TOL_REL, 0, &id);
+        if (rc) { *rc_out = rc; sipg_server_destroy(S); return sol; }
+        index[id] = (int)i;
+    }
+    *max_queued = 0;
+    if (infos) infos->assign(reqs.size(), sipg_cg_info{});
+    for (int round = 0; round < 100000 && sol.size() < reqs.size(); ++round) {
+        int nd = 0;
+        if ((rc = sipg_server_progress(S, &nd))) break;
+        int q = 0;
+        sipg_server_stats(S, &q, nullptr, nullptr, nullptr);
+        if (q > *max_queued) *max_queued = q;
+        for (;;) {
+            int64_t id = 0;
+            sipg_cg_info info{};
+            std::vector<double> x(64 * 64 * 49);
+            const int t = sipg_server_take(S, &id, &info, x.data(), (int64_t)x.size());
+            if (t == SIPG_ERR_NOT_FOUND) break;
+            if (t) { rc = t; break; }
+            const int k = index[id];
+            x.resize((size_t)reqs[k].dim * reqs[k].dim * (reqs[k].p + 1) * (reqs[k].p + 1));
+            sol[k] = x;
+            if (infos) (*infos)[k] = info;
+        }
+        if (rc) break;
+    }
+    *rc_out = rc;
+    sipg_server_destroy(S);
+    return sol;
+}
+
+int main() {
+    const std::vector<Req> reqs = make_requests(24);
+    int64_t demand = 0;
+    for (auto &r : reqs) demand += (int64_t)r.dim * r.dim * (r.p + 1) * (r.p + 1);
+    // budget for about a third of the total demand (48 bytes per unknown)
+    const double budget = 48.0 * demand / 3.0;
+    int maxq = 0, rc = 0;
+    std::vector<sipg_cg_info> infos;
+    auto mix = serve(reqs, budget, &maxq, &rc, &infos);
+    printf("{\"case\":\"serve_mix\",\"rc\":%d,\"done\":%zu,\"total\":%zu,\"max_queued\":%d}\n", rc, mix.size(),
+           reqs.size(), maxq);
+    for (size_t k = 0; k < reqs.size(); ++k) {
+        const Req &r = reqs[k];
+        std::vector<Req> one{r};
+        int mq = 0, rc1 = 0;
+        auto alone = serve(one, budget, &mq, &rc1, nullptr);
+        double same = -1.0;
+        if (mix.count(k) && alone.count(0)) {
+            same = 0.0;
+            for (size_t i = 0; i < mix[k].size(); ++i) same = std::fmax(same, std::fabs(mix[k][i] - alone[0][i]));
+        }
+        printf("{\"case\":\"serve_one\",\"k\":%zu,\"rc\":%d,\"status\":%d,\"iters\":%d,\"tol\":%.0e,"
+               "\"true_rel\":%.3e,\"alone_maxdiff\":%.3e}\n",
+               k, rc1, infos.size() > k ? infos[k].status : -1, infos.size() > k ? infos[k].iters : -1, r.tol,
+               infos.size() > k && infos[k].b_norm > 0 ? infos[k].true_res_norm / infos[k].b_norm : -1.0, same);
+    }
+    return 0;
+}

@@ -93,8 +93,10 @@ def test_batch_engine_solves_every_instance(batch_results):
     # the batched engine (batch.cpp); each instance is the reference CG
     for r in batch_results["batch"]:
         assert r["rc"] == 0 and r["status"] == 0 and r["relres"] <= 1e-10, r
-        assert abs(r["iters"] - r["oracle_iters"]) <= 2, r
-        assert r["diff"] <= 1e-9 and r["reps"] == 1 and r["apply_err"] == 0.0, r
+        # the finish is confirmed on a true residual inserted a replay or
+        # two (8 iterations each) after the recursion met the tolerance
+        assert r["oracle_iters"] - 2 <= r["iters"] <= r["oracle_iters"] + 40, r
+        assert r["diff"] <= 1e-9 and r["reps"] >= 1 and r["apply_err"] == 0.0, r
 
 
 def test_batch_engine_iteration_cap(batch_results):
@@ -107,3 +109,26 @@ def test_batch_engine_breakdown_is_per_instance(batch_results):
     assert all(r["rc"] == 8 for r in rows)
     assert [r["status"] for r in rows] == [0, 8, 0]
     assert rows[0]["relres"] <= 1e-10 and rows[2]["relres"] <= 1e-10
+
+
+@pytest.fixture(scope="module")
+def serve_results(results):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="halt_on_error=1")
+    out = subprocess.run([os.path.join(ENG, "_build", "serve_driver")], check=True, capture_output=True,
+                         text=True, env=env, timeout=600)
+    assert "runtime error" not in out.stderr and "AddressSanitizer" not in out.stderr, out.stderr
+    return [json.loads(l) for l in out.stdout.strip().splitlines()]
+
+
+def test_server_admits_within_budget_and_finishes_all(serve_results):
+    mix = serve_results[0]
+    # the budget holds about a third of the demand: requests had to wait
+    assert mix["case"] == "serve_mix" and mix["rc"] == 0 and mix["done"] == mix["total"] == 24
+    assert mix["max_queued"] > 0
+
+
+def test_server_result_same_alone_and_in_the_mix(serve_results):
+    for r in serve_results[1:]:
+        assert r["rc"] == 0 and r["status"] == 0, r
+        assert 0 <= r["true_rel"] <= r["tol"], r
+        assert r["alone_maxdiff"] == 0.0, r

@@ -64,7 +64,7 @@ def test_batch_cg_reaches_tolerance_like_single_solves(sipg):
     B, b = _fourier_batch(sipg, meshes)
     x = B.vector()
     x.zero()
-    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL)
+    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL, replace_factor=3e-3)
     assert st == sipg.OK, sipg.strerror(st)
     for i, (d, _, p) in enumerate(meshes):
         op = sipg.Operator(d, d, p, sipg.pen_sigma(p), sipg.F64)
@@ -77,7 +77,7 @@ def test_batch_cg_reaches_tolerance_like_single_solves(sipg):
         xs.zero()
         st1, it1, _ = op.cg(xs, bi, 1e-10, sipg.TOL_REL)
         assert st1 == sipg.OK
-        assert abs(infos[i].iters - it1) <= max(3, 0.03 * it1), (i, infos[i].iters, it1)
+        assert it1 - 3 <= infos[i].iters <= 1.25 * it1 + 64, (i, infos[i].iters, it1)
         ref = op.download(xs)
         got = B.download(i, x)
         assert np.linalg.norm(got - ref) <= 1e-7 * np.linalg.norm(ref), i
@@ -92,7 +92,7 @@ def test_batch_cg_every_shape(sipg, shape):
         assert B.set_shape(p, *shape) == sipg.OK
     x = B.vector()
     x.zero()
-    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL, maxit=5000)
+    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL, maxit=5000, replace_factor=3e-3)
     assert st == sipg.OK, [(k.status, k.iters) for k in infos]
     for i, (d, _, p) in enumerate(meshes):
         op = sipg.Operator(d, d, p, sipg.pen_sigma(p), sipg.F64)
@@ -100,7 +100,29 @@ def test_batch_cg_every_shape(sipg, shape):
         xs = op.vector()
         xs.zero()
         st1, it1, _ = op.cg(xs, bi, 1e-10, sipg.TOL_REL)
-        assert abs(infos[i].iters - it1) <= max(3, 0.03 * it1), (i, shape, infos[i].iters, it1)
+        assert it1 - 3 <= infos[i].iters <= 1.25 * it1 + 64, (i, shape, infos[i].iters, it1)
+
+
+def test_batch_skips_finished_instances_in_partial_waves(sipg):
+    # p = 8 on 8x2 blocks: 144 threads, the last wave has 16 lanes; small
+    # instances finish first, so the workgroups' item ranges cross finished
+    # instances and every wave must skip to the same next item
+    dims = [4, 30, 5, 26, 6, 20, 7, 17, 4, 24, 9, 28, 5, 19, 8, 22]
+    meshes = [(d, d, 8) for d in dims]
+    B, b = _fourier_batch(sipg, meshes)
+    assert B.set_shape(8, 8, 2) == sipg.OK
+    x = B.vector()
+    x.zero()
+    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL, maxit=5000, replace_factor=3e-3)
+    assert st == sipg.OK, [(k.status, k.iters) for k in infos]
+    for i in (1, 3, 11):
+        d = dims[i]
+        op = sipg.Operator(d, d, 8, sipg.pen_sigma(8), sipg.F64)
+        bi = op.to_device(B.download(i, b))
+        xs = op.vector()
+        xs.zero()
+        _, it1, _ = op.cg(xs, bi, 1e-10, sipg.TOL_REL)
+        assert it1 - 3 <= infos[i].iters <= 1.25 * it1 + 64, (i, infos[i].iters, it1)
 
 
 def test_batch_cg_deterministic(sipg):
@@ -135,7 +157,7 @@ def test_batch_breakdown_stays_in_its_instance(sipg):
         B.rhs_fourier(i, sipg.forcing_draw(i, 1, 4, 4), b)
     x = B.vector()
     x.zero()
-    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL)
+    st, infos = B.cg(x, b, 1e-10, sipg.TOL_REL, replace_factor=3e-3)
     assert st == sipg.ERR_BREAKDOWN
     assert infos[1].status == sipg.ERR_BREAKDOWN
     assert infos[0].status == sipg.OK and infos[2].status == sipg.OK
