@@ -218,7 +218,11 @@ def serving_leg(sipg, seconds, rate):
             log(f"serving: {len(done)}/{K} done, {st['running']} running, {st['queued']} queued")
     wall = time.perf_counter() - t0
     statuses = [done[k].status for k in range(K)]
-    rel_ok = all(done[k].true_res_norm <= tols[k] * done[k].b_norm * 1.0000001 for k in range(K))
+    rel_ok = all(0.0 <= done[k].true_res_norm <= tols[k] * done[k].b_norm * 1.0000001 for k in range(K))
+    failed = [{"dim": int(dims[k]), "p": int(ps[k]), "tol": float(tols[k]), "iters": done[k].iters,
+               "status": sipg.strerror(done[k].status),
+               "true_relres": done[k].true_res_norm / done[k].b_norm if done[k].true_res_norm >= 0 else None}
+              for k in range(K) if done[k].status != sipg.OK]
     dof = int(sum(int(dims[k]) ** 2 * (int(ps[k]) + 1) ** 2 for k in range(K)))
     out = {"arrival_rate_per_s": rate, "arrival_window_s": seconds, "requests": K,
            "solves_per_s": K / wall, "dof_per_s": dof / wall, "wall_s": wall,
@@ -226,6 +230,7 @@ def serving_leg(sipg, seconds, rate):
                          "max": float(lat.max())},
            "max_queued": max_queued, "max_running": max_running,
            "all_converged": all(s == sipg.OK for s in statuses), "all_meet_tolerance": rel_ok,
+           "n_failed": len(failed), "failed": failed[:10],
            "mixed_marked": int(mixed.sum()),
            "note": ("requests marked mixed precision are served by the FP64 path: FP32 inner CG "
                     "needed 5-6x the FP64 time on these operators (eps32 * kappa ~ 1)")}

@@ -640,6 +640,12 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
         if (phase[i] != FINISHED) open_work += B.off[i + 1] - B.off[i];
     int64_t done_work = 0;
     std::vector<int> nrep_req(ni, 0);
+    // final checks: the true r.r of the last failed one, and whether one is
+    // in flight; a failed check that gains less than 10% on the previous
+    // one means the FP64 floor -- the instance ends there, not converged
+    std::vector<double> check_ref(ni, -1.0);
+    std::vector<char> check_req(ni, 0);
+    bool stopped = false;
     const double rf2 = opts->replace_factor * opts->replace_factor;
     auto launch_replay = [&]() -> int {
         SIPG_HIP_TRY(hipGraphLaunch(exec, B.stream));
@@ -677,12 +683,25 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
                 // the polled state; the final one once the recursion has
                 // fallen 4x below the last true residual (as in sipg_cg_ex)
                 if (hs[i].nrep != nrep_req[i]) continue;
+                if (check_req[i]) {
+                    check_req[i] = 0;
+                    if (check_ref[i] >= 0.0 && hs[i].rr_ref > 0.81 * check_ref[i]) {
+                        phase[i] = FINISHED;
+                        stopped = true;
+                        --n_open;
+                        ++finished_since;
+                        done_work += B.off[i + 1] - B.off[i];
+                        continue;
+                    }
+                    check_ref[i] = hs[i].rr_ref;
+                }
                 const bool check = true_stop && (c.flags & kSlotWantCheck) && c.rr <= 0.0625 * hs[i].rr_ref;
                 const bool periodic = rf2 > 0.0 && c.rr <= rf2 * hs[i].rr_ref;
                 if (check || periodic) {
                     repl[i] = 1;
                     any_repl = true;
                     ++nrep_req[i];
+                    check_req[i] = check ? 1 : 0;
                 }
             }
             // the replacement set of replay j+2 (empty most of the time)
@@ -690,8 +709,10 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
                 build_lists(B, repl, Q);
                 if ((rc = upload_lists(B, SET_REPL, Q))) break;
             }
-            // shorter lists once a tenth of the open work has finished
-            if (finished_since > 0 && n_open > 0 && 10 * done_work >= open_work) {
+            // shorter lists once a tenth of the open work has finished (at
+            // once for stopped instances, which are not finished on the device)
+            if (finished_since > 0 && n_open > 0 && (stopped || 10 * done_work >= open_work)) {
+                stopped = false;
                 if ((rc = upload_active())) break;
                 finished_since = 0;
                 open_work -= done_work;
@@ -729,7 +750,7 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
             info[i].replacements = f.nrep;
             info[i].pad = 0;
             info[i].res_norm = std::sqrt(c.rr);
-            info[i].true_res_norm = c.checked ? std::sqrt(f.rr_ref) : -1.0;
+            info[i].true_res_norm = (c.checked || f.nrep > 0) ? std::sqrt(f.rr_ref) : -1.0;
             info[i].b_norm = std::sqrt(f.bb);
         }
         if (st != SIPG_OK && status == SIPG_OK) status = st;

@@ -56,6 +56,8 @@ struct Request {
     int slot = -1;
     int64_t off = -1, len = 0;
     int nrep_req = 0;                // replacements requested so far
+    bool check_req = false;          // the last one requested is a final check
+    double check_ref = -1.0;         // true r.r of the last failed final check
     sipg_cg_info info{};
 };
 
@@ -370,7 +372,7 @@ void finish(sipg_server &S, Request &R, const cg_state &st) {
     R.info.replacements = st.nrep;
     R.info.pad = 0;
     R.info.res_norm = std::sqrt(c.rr);
-    R.info.true_res_norm = c.checked ? std::sqrt(st.rr_ref) : -1.0;
+    R.info.true_res_norm = (c.checked || st.nrep > 0) ? std::sqrt(st.rr_ref) : -1.0;
     R.info.b_norm = std::sqrt(st.bb);
     R.phase = DONE;
     S.slot_req[R.slot] = -1;
@@ -531,13 +533,28 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
                 lists_changed = true;
                 continue;
             }
+            if (st.nrep != R.nrep_req) {                  // the last one is not in yet
+                still.push_back(id);
+                continue;
+            }
+            if (R.check_req) {
+                // a failed final check that gains less than 10% on the
+                // previous one means the FP64 floor: the request ends there
+                R.check_req = false;
+                if (R.check_ref >= 0.0 && st.rr_ref > 0.81 * R.check_ref) {
+                    finish(*S, R, st);
+                    lists_changed = true;
+                    continue;
+                }
+                R.check_ref = st.rr_ref;
+            }
             still.push_back(id);
-            if (st.nrep != R.nrep_req) continue;          // the last one is not in yet
             const bool check = (c.flags & kSlotWantCheck) && c.rr <= 0.0625 * st.rr_ref;
             const bool periodic = c.rr <= kServeReplace * kServeReplace * st.rr_ref;
             if (check || periodic) {
                 repl.push_back(id);
                 ++R.nrep_req;
+                R.check_req = check;
             }
         }
         S->active.swap(still);
