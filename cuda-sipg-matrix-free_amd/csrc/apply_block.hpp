@@ -154,6 +154,79 @@ __device__ __forceinline__ void block_prefetch(BlockRegs<T, P, BX, BY> &R, const
     }
 }
 
+// Direct global -> LDS staging (global_load_lds_dword): used where the LDS
+// image of a block is the concatenation of its row runs and halo-column
+// tiles (no row padding, TS == NQ) and two images fit in LDS, so the next
+// block streams in while the current one is computed.
+template <typename T, int P, int BX, int BY>
+struct BlockDma {
+    using G = BlockGeo<T, P, BX, BY>;
+    static constexpr bool ok = G::TS == G::NQ && G::NT >= 64 && 2 * G::LDS * sizeof(T) <= 160 * 1024;
+};
+
+typedef __attribute__((address_space(3))) unsigned lds_dword;
+
+// Issues the loads of block (x0, y0) into the LDS image `lds`; returns
+// without waiting.  Every lane of a load instruction writes the dword at
+// (LDS base + 4 * lane), so an instruction moves 64 consecutive dwords of
+// one row run or halo tile, and only full waves issue them.  Work items
+// (BY + 2 row runs, then 2 BY halo tiles) go round-robin over the waves;
+// runs and tiles outside the mesh are skipped (their slots are never read
+// by a valid element), and a run cut by the right mesh edge loads only the
+// elements that exist.
+template <typename T, int P, int BX, int BY>
+__device__ __forceinline__ void block_issue(T *lds, const T *__restrict__ x, int dimx, int dimy, int x0, int y0)
+{
+    using G = BlockGeo<T, P, BX, BY>;
+    static_assert(BlockDma<T, P, BX, BY>::ok, "direct staging needs unpadded tiles and full waves");
+    constexpr int WD = sizeof(T) / 4;                   // dwords per value
+    constexpr int NQD = G::NQ * WD, RUND = G::RUN * WD;
+    constexpr int CHR = (RUND + 63) / 64, CHT = (NQD + 63) / 64;
+    constexpr int NW = G::NT / 64;
+    constexpr int NITEM = BY + 2 + 2 * BY;
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wid >= NW) return;
+    const unsigned *xd = reinterpret_cast<const unsigned *>(x);
+    lds_dword *ld = (lds_dword *)lds;
+    const int ncol = dimx - x0 < BX ? dimx - x0 : BX;
+    for (int it = wid; it < NITEM; it += NW) {
+        if (it < BY + 2) {
+            const int gy = y0 - 1 + it;
+            if (gy < 0 || gy >= dimy) continue;
+            const unsigned *src = xd + ((size_t)gy * dimx + x0) * NQD + lane;
+            lds_dword *dst = ld + it * RUND;
+            if (ncol == BX) {
+#pragma unroll
+                for (int c = 0; c < CHR; ++c)
+                    if ((c + 1) * 64 <= RUND || c * 64 + lane < RUND)
+                        __builtin_amdgcn_global_load_lds(src + c * 64, dst + c * 64, 4, 0, 0);
+            } else {
+                const int nd = ncol * NQD;
+#pragma unroll
+                for (int c = 0; c < CHR; ++c)
+                    if (c * 64 + lane < nd)
+                        __builtin_amdgcn_global_load_lds(src + c * 64, dst + c * 64, 4, 0, 0);
+            }
+        } else {
+            const int k = it - (BY + 2);                // halo tile: left k < BY, right k >= BY
+            const bool right = k >= BY;
+            const int gy = y0 + (right ? k - BY : k), gx = right ? x0 + BX : x0 - 1;
+            if (gy >= dimy || gx < 0 || gx >= dimx) continue;
+            const unsigned *src = xd + ((size_t)gy * dimx + gx) * NQD + lane;
+            lds_dword *dst = ld + (G::SL_COL + k) * NQD;
+#pragma unroll
+            for (int c = 0; c < CHT; ++c)
+                if ((c + 1) * 64 <= NQD || c * 64 + lane < NQD)
+                    __builtin_amdgcn_global_load_lds(src + c * 64, dst + c * 64, 4, 0, 0);
+        }
+    }
+}
+
+// Waits for this wave's outstanding vector-memory operations, the direct
+// LDS loads included; a barrier after it publishes the staged block.
+__device__ __forceinline__ void block_issue_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 template <typename T, int P, int BX, int BY>
 __device__ __forceinline__ void block_stage(T *lds, const BlockRegs<T, P, BX, BY> &R)
 {
@@ -176,7 +249,9 @@ __device__ __forceinline__ void block_stage(T *lds, const BlockRegs<T, P, BX, BY
 
 // The operator on the staged block (x0, y0), result written to y; returns
 // this thread's share of x.y over valid elements when DOT.  Contains the
-// block's barriers and leaves LDS free for the next stage.
+// block's barriers; with TAIL it ends with one more, after which the LDS
+// image may be overwritten (not needed when the next block is staged in
+// another image and a barrier precedes the reuse of this one).
 //
 // Thread (s, j): element s of the block, line j.  Column j of the element
 // (x-direction, flux_term6a) and row j (y-direction, flux_term6b) come from
@@ -184,7 +259,7 @@ __device__ __forceinline__ void block_stage(T *lds, const BlockRegs<T, P, BX, BY
 // Yy[a] = (U K1)[j][a] (K1 symmetric) in even-odd form, the face terms, and
 // the assembly Y[a][b] = Yx (column j) + Yy (row j) over the element's own
 // tile before the block's rows are copied out as contiguous runs.
-template <typename T, int P, int BX, int BY, bool DOT>
+template <typename T, int P, int BX, int BY, bool DOT, bool TAIL = true>
 __device__ __forceinline__ double block_compute(T *lds, T *__restrict__ y, int dimx, int dimy,
                                                 int x0, int y0, T pen)
 {
@@ -363,7 +438,7 @@ __device__ __forceinline__ double block_compute(T *lds, T *__restrict__ y, int d
             }
         }
     }
-    __syncthreads();                                   // LDS free for the next block
+    if (TAIL) __syncthreads();                         // LDS free for the next block
     return dot;
 }
 

@@ -640,10 +640,12 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
         if (phase[i] != FINISHED) open_work += B.off[i + 1] - B.off[i];
     int64_t done_work = 0;
     std::vector<int> nrep_req(ni, 0);
-    // final checks: the true r.r of the last failed one, and whether one is
-    // in flight; a failed check that gains less than 10% on the previous
-    // one means the FP64 floor -- the instance ends there, not converged
+    // final checks: the smallest true r.r of the failed ones, how many failed
+    // checks gained less than 10% on it, and whether one is in flight;
+    // kStallChecks such checks mean the FP64 floor -- the instance ends
+    // there, not converged
     std::vector<double> check_ref(ni, -1.0);
+    std::vector<int> check_stale(ni, 0);
     std::vector<char> check_req(ni, 0);
     bool stopped = false;
     const double rf2 = opts->replace_factor * opts->replace_factor;
@@ -685,7 +687,9 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
                 if (hs[i].nrep != nrep_req[i]) continue;
                 if (check_req[i]) {
                     check_req[i] = 0;
-                    if (check_ref[i] >= 0.0 && hs[i].rr_ref > 0.81 * check_ref[i]) {
+                    const bool gain = check_ref[i] < 0.0 || hs[i].rr_ref <= 0.81 * check_ref[i];
+                    if (!gain) ++check_stale[i];
+                    if (check_stale[i] >= kStallChecks) {
                         phase[i] = FINISHED;
                         stopped = true;
                         --n_open;
@@ -693,7 +697,7 @@ int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *o
                         done_work += B.off[i + 1] - B.off[i];
                         continue;
                     }
-                    check_ref[i] = hs[i].rr_ref;
+                    if (gain) check_ref[i] = hs[i].rr_ref;
                 }
                 const bool check = true_stop && (c.flags & kSlotWantCheck) && c.rr <= 0.0625 * hs[i].rr_ref;
                 const bool periodic = rf2 > 0.0 && c.rr <= rf2 * hs[i].rr_ref;

@@ -78,7 +78,62 @@ void apply_kernel(const T *__restrict__ x, T *__restrict__ y, int dimx, int dimy
     }
 }
 
+// The same walk with direct global -> LDS staging into two LDS images:
+// block b+1 streams into one while block b is computed from the other, with
+// no registers held for it.  The barrier at the top of each pass (after the
+// wait for this wave's loads) both publishes the staged block and tells the
+// issuing waves that every wave is done with the image they refill next.
+template <typename T, int P, int BX, int BY, bool DOT>
+__global__ __launch_bounds__(BX * BY * (P + 1))
+void apply_kernel_lds(const T *__restrict__ x, T *__restrict__ y, int dimx, int dimy, int nbx,
+                      int nblk, T pen, double *__restrict__ partials,
+                      const cg_state *__restrict__ st, int slot)
+{
+    using G = BlockGeo<T, P, BX, BY>;
+    __shared__ T lds[2 * G::LDS];
+    __shared__ double red[(G::NT + 63) / 64];
+
+    if (DOT) {
+        if (st->s[slot].done | st->err) return;
+    }
+    const int NG = gridDim.x;
+    const int lw = xcd_remap(blockIdx.x, NG);
+    const int bbeg = (int)((int64_t)lw * nblk / NG);
+    const int bend = (int)((int64_t)(lw + 1) * nblk / NG);
+
+    double dot = 0.0;
+    if (bbeg < bend)
+        block_issue<T, P, BX, BY>(lds, x, dimx, dimy, (bbeg % nbx) * BX, (bbeg / nbx) * BY);
+    for (int b = bbeg; b < bend; ++b) {
+        const int x0 = (b % nbx) * BX, y0 = (b / nbx) * BY;
+        const int cur = (b - bbeg) & 1;
+        block_issue_wait();
+        __syncthreads();
+        if (b + 1 < bend)
+            block_issue<T, P, BX, BY>(lds + (cur ^ 1) * G::LDS, x, dimx, dimy, ((b + 1) % nbx) * BX,
+                                      ((b + 1) / nbx) * BY);
+        dot += block_compute<T, P, BX, BY, DOT, false>(lds + cur * G::LDS, y, dimx, dimy, x0, y0, pen);
+    }
+    if (DOT) {
+        const double bs = block_sum<G::NT>(dot, red);
+        if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+    }
+}
+
 namespace {
+
+// The apply kernel of one degree: direct LDS staging where the block's LDS
+// image is unpadded and two images fit, register staging otherwise.
+template <typename T, int P, bool DOT>
+struct ApplyKernel {
+    using S = ApplyShape<T, P>;
+    static constexpr bool lds_staged = BlockDma<T, P, S::BX, S::BY>::ok;
+    using Fn = void (*)(const T *, T *, int, int, int, int, T, double *, const cg_state *, int);
+    static Fn fn() {
+        if constexpr (lds_staged) return &apply_kernel_lds<T, P, S::BX, S::BY, DOT>;
+        else return &apply_kernel<T, P, S::BX, S::BY, DOT>;
+    }
+};
 
 // Workgroups of one kernel that the device holds at once: the persistent
 // grid.  Cached per device (occupancy depends only on the kernel).
@@ -94,7 +149,7 @@ int resident_groups() {
     int cus = 0, per = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, reinterpret_cast<const void *>(&apply_kernel<T, P, S::BX, S::BY, DOT>),
+            &per, reinterpret_cast<const void *>(ApplyKernel<T, P, DOT>::fn()),
             S::BX * S::BY * (P + 1), 0) != hipSuccess)
         per = 0;
     g = (cus > 0 ? cus : 1) * (per > 0 ? per : 1);
@@ -113,12 +168,12 @@ int launch_one(const void *x, void *y, const void *tab, int dimx, int dimy, doub
     dim3 block(BX * BY * (P + 1));
     if (partials) {
         dim3 grid(std::min(nblk, resident_groups<T, P, true>()));
-        hipLaunchKernelGGL((apply_kernel<T, P, BX, BY, true>), grid, block, 0, s,
+        hipLaunchKernelGGL((ApplyKernel<T, P, true>::fn()), grid, block, 0, s,
                            (const T *)x, (T *)y, dimx, dimy, nbx, nblk, (T)pen,
                            partials, st, slot);
     } else {
         dim3 grid(std::min(nblk, resident_groups<T, P, false>()));
-        hipLaunchKernelGGL((apply_kernel<T, P, BX, BY, false>), grid, block, 0, s,
+        hipLaunchKernelGGL((ApplyKernel<T, P, false>::fn()), grid, block, 0, s,
                            (const T *)x, (T *)y, dimx, dimy, nbx, nblk, (T)pen,
                            partials, st, slot);
     }

@@ -57,7 +57,8 @@ struct Request {
     int64_t off = -1, len = 0;
     int nrep_req = 0;                // replacements requested so far
     bool check_req = false;          // the last one requested is a final check
-    double check_ref = -1.0;         // true r.r of the last failed final check
+    double check_ref = -1.0;         // smallest true r.r of the failed final checks
+    int check_stale = 0;             // failed checks without a 10% gain on it
     sipg_cg_info info{};
 };
 
@@ -538,15 +539,18 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
                 continue;
             }
             if (R.check_req) {
-                // a failed final check that gains less than 10% on the
-                // previous one means the FP64 floor: the request ends there
+                // kStallChecks failed final checks that gain less than 10%
+                // on the best one mean the FP64 floor: the request ends
+                // there
                 R.check_req = false;
-                if (R.check_ref >= 0.0 && st.rr_ref > 0.81 * R.check_ref) {
+                const bool gain = R.check_ref < 0.0 || st.rr_ref <= 0.81 * R.check_ref;
+                if (!gain) ++R.check_stale;
+                if (R.check_stale >= kStallChecks) {
                     finish(*S, R, st);
                     lists_changed = true;
                     continue;
                 }
-                R.check_ref = st.rr_ref;
+                if (gain) R.check_ref = st.rr_ref;
             }
             still.push_back(id);
             const bool check = (c.flags & kSlotWantCheck) && c.rr <= 0.0625 * st.rr_ref;

@@ -66,6 +66,13 @@ struct cg_slot {
 constexpr int kSlotReplaced = 1;
 constexpr int kSlotRestart = 2;
 constexpr int kSlotWantCheck = 4;
+// Batched and served solves: failed final true-residual checks that gain
+// less than 10% on the best failed one (counted over the whole solve) before
+// the solve is declared stuck at the floor of FP64 CG and ends, not
+// converged.  Near the floor the true residual of successive checks
+// scatters around a fixed level, so a lucky low one does not reset the
+// count.
+constexpr int kStallChecks = 4;
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b
