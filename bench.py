@@ -422,7 +422,7 @@ def main():
             "frac": apply_gbs / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_note": traffic_note,
-            "kernel": "apply_kernel<double,8,16,4> (fused volume + flux_term6a + flux_term6b)",
+            "kernel": "apply_kernel_lds<double,8,16,4,false> (fused volume + flux_term6a + flux_term6b; blocks staged into LDS by global_load_lds, double-buffered)",
             "algorithmic_bytes_per_launch": apply_bytes,
         },
     }

@@ -6,17 +6,21 @@
 #include "sipg_internal.hpp"
 #include "basis_constexpr.hpp"
 
+#include <utility>
+
 namespace sipg_internal {
 
 // BX x BY elements per workgroup, NE = BX*BY, threads = NE*(P+1) <= 1024,
-// LDS = (P+1)^2 * (NE + 2BX + 2BY) * sizeof(T) <= 160 KiB.
+// LDS = (P+1)^2 * (NE + 2BX + 2BY) * sizeof(T) <= 160 KiB.  FP64 at p >= 9
+// takes 8-wide blocks and at p >= 15 8 x 2: fewer waves per workgroup leave
+// each thread the registers that its (p+1)-long lines need without spills.
 template <typename T, int P>
 struct ApplyShape {
     static constexpr int N1 = P + 1;
-    static constexpr bool big64 = sizeof(T) == 8 && P >= 13;
+    static constexpr bool big64 = sizeof(T) == 8 && P >= 9;
     static constexpr bool small = N1 > 16;
     static constexpr int BX = (big64 || small) ? 8 : 16;
-    static constexpr int BY = 4;
+    static constexpr int BY = (sizeof(T) == 8 && P >= 15) ? 2 : 4;
 };
 
 __device__ __forceinline__ int xcd_remap(int b, int nblk) {
@@ -166,6 +170,40 @@ struct BlockDma {
 
 typedef __attribute__((address_space(3))) unsigned lds_dword;
 
+// Direct loads of chunks C... of 64 dwords: lane l moves dword 64C + l of
+// src to dword 64C + l of dst.  The chunk offset is the instruction's
+// immediate (applied to both addresses), so a group costs one M0 write.
+template <int... C>
+__device__ __forceinline__ void dma_group(const unsigned *src, lds_dword *dst, std::integer_sequence<int, C...>)
+{
+    (__builtin_amdgcn_global_load_lds(src, dst, 4, C * 256, 0), ...);
+}
+
+// The first N full 64-dword chunks from src (per-lane pointer: base + lane)
+// to dst (wave-uniform base), in groups of 16 (immediate offsets < 4 KiB).
+template <int N>
+__device__ __forceinline__ void dma_chunks(const unsigned *src, lds_dword *dst)
+{
+    if constexpr (N >= 16) {
+        dma_group(src, dst, std::make_integer_sequence<int, 16>{});
+        dma_chunks<N - 16>(src + 1024, dst + 1024);
+    } else if constexpr (N > 0) {
+        dma_group(src, dst, std::make_integer_sequence<int, N>{});
+    }
+}
+
+// A run of exactly ND dwords: its full chunks, then the lanes of the last
+// partial chunk that fall inside the run.
+template <int ND>
+__device__ __forceinline__ void dma_run(const unsigned *src, lds_dword *dst, int lane)
+{
+    dma_chunks<ND / 64>(src, dst);
+    if constexpr (ND % 64 != 0) {
+        if (lane < ND % 64)
+            __builtin_amdgcn_global_load_lds(src + (ND / 64) * 64, dst + (ND / 64) * 64, 4, 0, 0);
+    }
+}
+
 // Issues the loads of block (x0, y0) into the LDS image `lds`; returns
 // without waiting.  Every lane of a load instruction writes the dword at
 // (LDS base + 4 * lane), so an instruction moves 64 consecutive dwords of
@@ -181,7 +219,7 @@ __device__ __forceinline__ void block_issue(T *lds, const T *__restrict__ x, int
     static_assert(BlockDma<T, P, BX, BY>::ok, "direct staging needs unpadded tiles and full waves");
     constexpr int WD = sizeof(T) / 4;                   // dwords per value
     constexpr int NQD = G::NQ * WD, RUND = G::RUN * WD;
-    constexpr int CHR = (RUND + 63) / 64, CHT = (NQD + 63) / 64;
+    constexpr int CHR = (RUND + 63) / 64;
     constexpr int NW = G::NT / 64;
     constexpr int NITEM = BY + 2 + 2 * BY;
     const int lane = threadIdx.x & 63;
@@ -197,10 +235,7 @@ __device__ __forceinline__ void block_issue(T *lds, const T *__restrict__ x, int
             const unsigned *src = xd + ((size_t)gy * dimx + x0) * NQD + lane;
             lds_dword *dst = ld + it * RUND;
             if (ncol == BX) {
-#pragma unroll
-                for (int c = 0; c < CHR; ++c)
-                    if ((c + 1) * 64 <= RUND || c * 64 + lane < RUND)
-                        __builtin_amdgcn_global_load_lds(src + c * 64, dst + c * 64, 4, 0, 0);
+                dma_run<RUND>(src, dst, lane);
             } else {
                 const int nd = ncol * NQD;
 #pragma unroll
@@ -213,12 +248,7 @@ __device__ __forceinline__ void block_issue(T *lds, const T *__restrict__ x, int
             const bool right = k >= BY;
             const int gy = y0 + (right ? k - BY : k), gx = right ? x0 + BX : x0 - 1;
             if (gy >= dimy || gx < 0 || gx >= dimx) continue;
-            const unsigned *src = xd + ((size_t)gy * dimx + gx) * NQD + lane;
-            lds_dword *dst = ld + (G::SL_COL + k) * NQD;
-#pragma unroll
-            for (int c = 0; c < CHT; ++c)
-                if ((c + 1) * 64 <= NQD || c * 64 + lane < NQD)
-                    __builtin_amdgcn_global_load_lds(src + c * 64, dst + c * 64, 4, 0, 0);
+            dma_run<NQD>(xd + ((size_t)gy * dimx + gx) * NQD + lane, ld + (G::SL_COL + k) * NQD, lane);
         }
     }
 }
@@ -298,6 +328,45 @@ __device__ __forceinline__ double block_compute(T *lds, T *__restrict__ y, int d
         u[i] = tile[i * N1 + j];
         v[i] = tile[j * N1 + i];
     }
+
+    // even-odd form of both lines: K1 is centrosymmetric and the derivative
+    // table antisymmetric (DP[i] = -D0[p-i], exactly), so the contraction,
+    // the element's own face derivatives and the face terms all act on the
+    // pair sums ue = u[i] + u[p-i] and differences uo = u[i] - u[p-i]
+    constexpr cx::EvenOdd<P> EO = cx::make_even_odd<P>();
+    constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
+    T ue[ME], uo[H > 0 ? H : 1], ve[ME], vo[H > 0 ? H : 1];
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        ue[i] = u[i] + u[P - i];
+        uo[i] = u[i] - u[P - i];
+        ve[i] = v[i] + v[P - i];
+        vo[i] = v[i] - v[P - i];
+    }
+    if (ME > H) { ue[ME - 1] = u[H]; ve[ME - 1] = v[H]; }
+
+    // own face derivatives: dL = sum D0[i] u[i] = S + B, dR = sum DP[i] u[i]
+    // = B - S with S = sum DA[i] ue[i] (+ D0[h] u[h]), B = sum DB[i] uo[i]
+    T dL, dR, dD, dU;
+    {
+        T su = 0, bu = 0, sv = 0, bv = 0;
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            su += (T)EO.DA[i] * ue[i];
+            bu += (T)EO.DB[i] * uo[i];
+            sv += (T)EO.DA[i] * ve[i];
+            bv += (T)EO.DB[i] * vo[i];
+        }
+        if (ME > H) {
+            su += (T)EO.DM * ue[ME - 1];
+            sv += (T)EO.DM * ve[ME - 1];
+        }
+        dL = su + bu;
+        dR = bu - su;
+        dD = sv + bv;
+        dU = bv - sv;
+    }
+
     T ev, fv, Ax, Bx, g, h, Cy, Dy;
     {
         T nL = 0, tL = 0, nR = 0, tR = 0;
@@ -316,12 +385,6 @@ __device__ __forceinline__ double block_compute(T *lds, T *__restrict__ y, int d
                 nR += D0[i] * t;
                 if (i == 0) tR = t;
             }
-        }
-        T dL = 0, dR = 0;
-#pragma unroll
-        for (int i = 0; i < N1; ++i) {
-            dL += D0[i] * u[i];
-            dR += DP[i] * u[i];
         }
         ev = wj * (Lb ? u[0] : T(0.5) * (u[0] - tL));
         fv = wj * (Rb ? -u[P] : T(0.5) * (tR - u[P]));
@@ -346,33 +409,19 @@ __device__ __forceinline__ double block_compute(T *lds, T *__restrict__ y, int d
                 if (k == 0) tU = t;
             }
         }
-        T dD = 0, dU = 0;
-#pragma unroll
-        for (int k = 0; k < N1; ++k) {
-            dD += D0[k] * v[k];
-            dU += DP[k] * v[k];
-        }
         g = wj * (Db ? v[0] : T(0.5) * (v[0] - tD));
         h = wj * (Ub ? -v[P] : T(0.5) * (tU - v[P]));
         Cy = Db ? dD : T(0.5) * (dD + nD);
         Dy = Ub ? -dU : T(-0.5) * (dU + nU);
     }
 
-    // even-odd form: each constant of the half-size E/O blocks serves both
-    // directions while it is live
+    // Yx[a] = wj (K1 u)[a] + D0[a] ev + DP[a] fv, and the same for Yy with
+    // (v, g, h).  In even-odd form the face part of pair (a, p-a) is
+    // DA[a] (ev - fv) +/- DB[a] (ev + fv), and D0[h] (ev - fv) at the middle
+    // node; each constant of the E/O blocks serves both directions while live.
     T Yx[N1], Yy[N1];
     {
-        constexpr cx::EvenOdd<P> EO = cx::make_even_odd<P>();
-        constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
-        T ue[ME], uo[H > 0 ? H : 1], ve[ME], vo[H > 0 ? H : 1];
-#pragma unroll
-        for (int i = 0; i < H; ++i) {
-            ue[i] = u[i] + u[P - i];
-            uo[i] = u[i] - u[P - i];
-            ve[i] = v[i] + v[P - i];
-            vo[i] = v[i] - v[P - i];
-        }
-        if (ME > H) { ue[ME - 1] = u[H]; ve[ME - 1] = v[H]; }
+        const T s0x = ev - fv, s1x = ev + fv, s0y = g - h, s1y = g + h;
 #pragma unroll
         for (int a = 0; a < ME; ++a) {
             T ex_ = 0, ey_ = 0, ox_ = 0, oy_ = 0;
@@ -389,20 +438,17 @@ __device__ __forceinline__ double block_compute(T *lds, T *__restrict__ y, int d
                     ox_ += c * uo[i];
                     oy_ += c * vo[i];
                 }
-                Yx[a] = ex_ + ox_;
-                Yx[P - a] = ex_ - ox_;
-                Yy[a] = ey_ + oy_;
-                Yy[P - a] = ey_ - oy_;
+                const T e1x = wj * ex_ + (T)EO.DA[a] * s0x, o1x = wj * ox_ + (T)EO.DB[a] * s1x;
+                const T e1y = wj * ey_ + (T)EO.DA[a] * s0y, o1y = wj * oy_ + (T)EO.DB[a] * s1y;
+                Yx[a] = e1x + o1x;
+                Yx[P - a] = e1x - o1x;
+                Yy[a] = e1y + o1y;
+                Yy[P - a] = e1y - o1y;
             } else {
-                Yx[a] = ex_;
-                Yy[a] = ey_;
+                Yx[a] = wj * ex_ + (T)EO.DM * s0x;
+                Yy[a] = wj * ey_ + (T)EO.DM * s0y;
             }
         }
-    }
-#pragma unroll
-    for (int a = 0; a < N1; ++a) {
-        Yx[a] = wj * Yx[a] + D0[a] * ev + DP[a] * fv;
-        Yy[a] = wj * Yy[a] + D0[a] * g + DP[a] * h;
     }
     Yx[0] += wj * Ax + (Lb ? pen : pen2) * ev;
     Yx[P] += wj * Bx - (Rb ? pen : pen2) * fv;

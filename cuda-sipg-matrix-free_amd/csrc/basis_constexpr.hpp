@@ -112,6 +112,14 @@ struct EvenOdd {
     static constexpr int ME = (N1 + 1) / 2;     // even size (H, plus the middle node)
     double E[ME][ME] = {};
     double O[H > 0 ? H : 1][H > 0 ? H : 1] = {};
+    // face derivative at node 0, D0[i] = D[i][0], split the same way:
+    // DA[i] = (D0[i] + D0[p-i]) / 2, DB[i] = (D0[i] - D0[p-i]) / 2, and the
+    // middle node's D0[p/2] when p is even.  With DP[i] = -D0[p-i] these
+    // give sum D0 u = S + B and sum DP u = B - S (S over pair sums, B over
+    // pair differences).
+    double DA[H > 0 ? H : 1] = {};
+    double DB[H > 0 ? H : 1] = {};
+    double DM = 0.0;
 };
 
 template <int P>
@@ -126,6 +134,11 @@ constexpr EvenOdd<P> make_even_odd() {
         }
         if (ME > H) eo.E[a][H] = t.K1[a][H];
     }
+    for (int i = 0; i < H; ++i) {
+        eo.DA[i] = 0.5 * (t.D[i][0] + t.D[N - i][0]);
+        eo.DB[i] = 0.5 * (t.D[i][0] - t.D[N - i][0]);
+    }
+    if (ME > H) eo.DM = t.D[H][0];
     return eo;
 }
 

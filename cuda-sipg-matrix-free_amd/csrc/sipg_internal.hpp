@@ -72,7 +72,7 @@ constexpr int kSlotWantCheck = 4;
 // converged.  Near the floor the true residual of successive checks
 // scatters around a fixed level, so a lucky low one does not reset the
 // count.
-constexpr int kStallChecks = 4;
+constexpr int kStallChecks = 2;
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b
