@@ -37,6 +37,7 @@ sipg_internal::parts sipg_internal::part_layout(sipg_op *op) {
     p.rr = op->d_part + std::max(p.napply, kMaxPartials);
     p.bb = p.rr + kMaxPartials;
     p.misc = p.bb + kMaxPartials;
+    p.rz = p.misc + kMaxPartials;
     return p;
 }
 
@@ -159,7 +160,7 @@ int sipg_op_create(sipg_op **out, int dimx, int dimy, int p, double pen, int dty
     }
     const int napply = apply_num_blocks(dtype, p, dimx, dimy);
     if (napply < 1) return fail(SIPG_ERR_UNSUPPORTED);
-    const size_t npart = (size_t)std::max(napply, kMaxPartials) + 3 * (size_t)kMaxPartials;
+    const size_t npart = (size_t)std::max(napply, kMaxPartials) + 4 * (size_t)kMaxPartials;
     if ((e = hipMalloc(&op->d_part, npart * sizeof(double))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
     if ((e = hipMalloc(&op->d_scalar, 256 * sizeof(double))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
     if ((e = hipMalloc(&op->d_state, sizeof(cg_state))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
@@ -180,6 +181,7 @@ int sipg_op_destroy(sipg_op *op) {
     if (op->d_state) (void)hipFree(op->d_state);
     if (op->h_state) (void)hipHostFree(op->h_state);
     if (op->ws) (void)hipFree(op->ws);
+    if (op->d_prec) (void)hipFree(op->d_prec);
     if (op->twin) sipg_op_destroy(op->twin);
     delete op;
     return SIPG_OK;

@@ -142,6 +142,91 @@ constexpr EvenOdd<P> make_even_odd() {
     return eo;
 }
 
+constexpr double csqrt(double x) {
+    if (!(x > 0.0)) return 0.0;
+    double g = x > 1.0 ? x : 1.0;
+    for (int i = 0; i < 200; ++i) {
+        const double ng = 0.5 * (g + x / g);
+        if (ng == g) break;
+        g = ng;
+    }
+    return g;
+}
+
+// Block-Jacobi tables for the standard penalty pen = 5 (p+1)^2, the same
+// construction as the host's build_block_jacobi (precond.cpp): S with
+// K S = W S diag(lambda), S^T W S = I, for K = K1 + the interior element's
+// own face terms, and Linv[a][c] = 1 / (lambda_a + lambda_c).  Constants let
+// the preconditioner kernel take its coefficients as immediates.
+template <int P>
+struct PrecTables {
+    double S[P + 1][P + 1] = {};
+    double Linv[P + 1][P + 1] = {};
+};
+
+template <int P>
+constexpr PrecTables<P> make_prec() {
+    const Tables<P> t = make_tables<P>();
+    constexpr int n = P + 1, N = P;
+    const double pen = 5.0 * n * n;
+    double K[n][n] = {};
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) K[i][k] = t.K1[i][k];
+    for (int i = 0; i < n; ++i) {
+        K[i][0] += 0.5 * t.D[i][0];
+        K[i][N] -= 0.5 * t.D[i][N];
+    }
+    for (int k = 0; k < n; ++k) {
+        K[0][k] += 0.5 * t.D[k][0];
+        K[N][k] -= 0.5 * t.D[k][N];
+    }
+    K[0][0] += pen;
+    K[N][N] += pen;
+    double A[n][n] = {}, V[n][n] = {};
+    double scale = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) {
+            A[i][k] = 0.5 * (K[i][k] + K[k][i]) / csqrt(t.w[i] * t.w[k]);
+            scale += A[i][k] * A[i][k];
+        }
+    for (int i = 0; i < n; ++i) V[i][i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {          // cyclic Jacobi rotations
+        double off = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) off += A[i][j] * A[i][j];
+        if (off <= 1e-34 * scale) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p][q];
+                if (cabs(apq) < 1e-300) continue;
+                const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+                const double tt = (theta >= 0 ? 1.0 : -1.0) / (cabs(theta) + csqrt(theta * theta + 1.0));
+                const double c = 1.0 / csqrt(tt * tt + 1.0), s = tt * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    PrecTables<P> r{};
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < n; ++a) r.S[i][a] = V[i][a] / csqrt(t.w[i]);
+    for (int a = 0; a < n; ++a)
+        for (int c = 0; c < n; ++c) r.Linv[a][c] = 1.0 / (A[a][a] + A[c][c]);
+    return r;
+}
+
 // One instance per degree, usable inside device code as constants.
 template <int P>
 struct Basis {

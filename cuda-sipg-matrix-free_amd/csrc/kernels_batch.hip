@@ -193,7 +193,7 @@ __global__ __launch_bounds__(NT) void batch_beta_kernel(const int *__restrict__ 
         if (threadIdx.x == 0) {
             const bool met = !(sqrt(rr_new) > st->stop);
             const bool need_true = st->true_stop && !replaced;
-            cg_slot nx;
+            cg_slot nx{};
             nx.rr = rr_new;
             nx.alpha = 0.0;
             nx.beta = rr_new / cur.rr;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(NT) void batch_init_kernel(const int *__restrict__ 
             cg_state *st = states + i;
             const batch_ctl c = ctl[i];
             const double stop = c.tol_mode == SIPG_TOL_REL ? c.tol * sqrt(bb) : c.tol;
-            cg_slot s0;
+            cg_slot s0{};
             s0.rr = rr;
             s0.alpha = s0.beta = s0.pAp = 0.0;
             s0.it = 0;

@@ -1,0 +1,482 @@
+// kernels_pcg.hip -- block-Jacobi preconditioner and the PCG pieces.
+//
+// The reference's preconditioned CG (conjugate_gradient_gpu.hpp:116-213)
+// applies a block-diagonal preconditioner built from the element Laplacian
+// (laplacian_operator.hpp:95-186, mode_matrix.hpp:474-521): a dense block
+// per element, with pen truncated to an integer and only the Laplacian-
+// pattern entries of the inverse kept.  Here the preconditioner is the
+// exact inverse of an interior element's diagonal block of A, applied to
+// every element:
+//
+//   A_ee U = K U W + W U K,   K = K1 + (the element's own face terms in one
+//                             direction; the same matrix in x and y for an
+//                             interior element),  W = diag(w)
+//
+// a Kronecker sum, so with K S = W S diag(lambda), S^T W S = I (host side,
+// precond.cpp) its inverse is
+//
+//   U = S [ (S^T R S) ./ (lambda_a + lambda_c) ] S^T
+//
+// four (p+1)-long contractions per node -- the cost of a volume apply, with
+// no neighbour data.  Elements on the mesh boundary get the interior block
+// too (still SPD; about 11 % more iterations than their own blocks, in
+// exchange for coefficients that are uniform across every wave).
+#include "apply_block.hpp"
+
+#include <algorithm>
+#include <atomic>
+
+namespace sipg_internal {
+
+namespace {
+
+// elements per workgroup group; thread (s, j) owns line j of element s
+template <int P>
+struct PcgShape {
+    static constexpr int N1 = P + 1;
+    static constexpr int NE = N1 > 16 ? 32 : 64;
+    static constexpr int NT = NE * N1;
+    static constexpr int NQ = N1 * N1;
+    static constexpr int TS = tile_stride(NQ);
+    static constexpr int NV = NE * NQ;                  // values per group
+    static constexpr int KR = (NV + NT - 1) / NT;       // = N1
+};
+
+// Sum over the workgroup, returned to every thread (fixed order).
+template <int NT>
+__device__ __forceinline__ double group_sum_all(double v, double *red, double *total) {
+    const double s = block_sum<NT>(v, red);
+    if (threadIdx.x == 0) *total = s;
+    __syncthreads();
+    const double out = *total;
+    __syncthreads();
+    return out;
+}
+
+// z = M^-1 r over every element, with the block sums of r.r and r.z.
+//   UPDATE: first r -= alpha Ap with alpha = slot.rz / p.Ap (the p.Ap block
+//   sums of this iteration's apply), r written back -- the r half of the
+//   reference's alpha step; returns at once when the slot is finished.
+//   COPYP:  also p = z (the start of the iteration).
+// prec = [S (row-major, S[i][a] = eigenvector a at node i), Linv[a][c] =
+// 1 / (lambda_a + lambda_c)], in T.
+//   CX: the coefficients are the compile-time tables of the standard penalty
+//   pen = 5 (p+1)^2 (prec is then unused); otherwise they are read from prec.
+template <typename T, int P, bool UPDATE, bool COPYP, bool CX>
+__global__ __launch_bounds__(PcgShape<P>::NT)
+void pcg_rz_kernel(T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ z,
+                   T *__restrict__ pcopy, int64_t noe, const T *__restrict__ prec,
+                   const double *__restrict__ ppAp, int np, double *__restrict__ prr,
+                   double *__restrict__ prz, cg_state *st, int slot)
+{
+    using G = PcgShape<P>;
+    constexpr int N1 = G::N1, NQ = G::NQ, TS = G::TS, NE = G::NE, NT = G::NT, KR = G::KR;
+    __shared__ T lds[NE * TS];
+    __shared__ double red[(NT + 63) / 64];
+    __shared__ double total;
+
+    T alpha = T(0);
+    if (UPDATE) {
+        if (st->s[slot].done | st->err) return;
+        double s = 0.0;
+        for (int i = threadIdx.x; i < np; i += NT) s += ppAp[i];
+        const double pAp = group_sum_all<NT>(s, red, &total);
+        const double a = st->s[slot].rz / pAp;
+        if (!(pAp > 0.0) || !isfinite(a)) {
+            // every workgroup takes this branch (same p.Ap); the first
+            // records it in a word that the p update reads afterwards
+            if (blockIdx.x == 0 && threadIdx.x == 0) st->err = isfinite(pAp) ? 2 : 3;
+            return;
+        }
+        alpha = (T)a;
+    }
+    constexpr cx::PrecTables<P> PT = cx::make_prec<P>();
+    struct CS {
+        const T *p;
+        __device__ T operator()(int i, int a) const { return CX ? (T)PT.S[i][a] : p[i * N1 + a]; }
+    } S{prec};
+    struct CL {
+        const T *p;
+        __device__ T operator()(int a, int c) const { return CX ? (T)PT.Linv[a][c] : p[a * N1 + c]; }
+    } Linv{prec + NQ};
+    const int tid = threadIdx.x;
+    const int s_el = tid % NE;
+    int j = tid / NE;
+    if (NE % 64 == 0) j = __builtin_amdgcn_readfirstlane(j);
+    const int64_t nval = noe * NQ;
+    const int64_t ngroups = (noe + NE - 1) / NE;
+    double srr = 0.0, srz = 0.0;
+
+    for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const int64_t base = g * (int64_t)(NE * NQ);
+        T rv[KR];
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int idx = tid + k * NT;
+            const int64_t gi = base + idx;
+            T v = T(0);
+            if (idx < G::NV && gi < nval) {
+                v = r[gi];
+                if (UPDATE) {
+                    v = v - alpha * Ap[gi];
+                    r[gi] = v;
+                }
+                srr += (double)v * (double)v;
+            }
+            rv[k] = v;
+            if (idx < G::NV) lds[lds_index<NQ, TS>(idx)] = v;
+        }
+        __syncthreads();
+        T *tile = lds + s_el * TS;
+        // 1: column j of S^T R
+        {
+            T c[N1], t[N1];
+#pragma unroll
+            for (int i = 0; i < N1; ++i) c[i] = tile[i * N1 + j];
+#pragma unroll
+            for (int a = 0; a < N1; ++a) {
+                T acc = T(0);
+#pragma unroll
+                for (int i = 0; i < N1; ++i) acc += S(i, a) * c[i];
+                t[a] = acc;
+            }
+#pragma unroll
+            for (int a = 0; a < N1; ++a) tile[a * N1 + j] = t[a];
+        }
+        __syncthreads();
+        // 2: row j of ((S^T R S) ./ L) S^T
+        {
+            T w[N1], v[N1];
+#pragma unroll
+            for (int b = 0; b < N1; ++b) w[b] = tile[j * N1 + b];
+#pragma unroll
+            for (int c = 0; c < N1; ++c) {
+                T acc = T(0);
+#pragma unroll
+                for (int b = 0; b < N1; ++b) acc += w[b] * S(b, c);
+                v[c] = acc * Linv(j, c);
+            }
+#pragma unroll
+            for (int d = 0; d < N1; ++d) {
+                T acc = T(0);
+#pragma unroll
+                for (int c = 0; c < N1; ++c) acc += v[c] * S(d, c);
+                w[d] = acc;
+            }
+#pragma unroll
+            for (int d = 0; d < N1; ++d) tile[j * N1 + d] = w[d];
+        }
+        __syncthreads();
+        // 3: column j of S (...)
+        {
+            T c[N1], t[N1];
+#pragma unroll
+            for (int i = 0; i < N1; ++i) c[i] = tile[i * N1 + j];
+#pragma unroll
+            for (int a = 0; a < N1; ++a) {
+                T acc = T(0);
+#pragma unroll
+                for (int i = 0; i < N1; ++i) acc += S(a, i) * c[i];
+                t[a] = acc;
+            }
+#pragma unroll
+            for (int a = 0; a < N1; ++a) tile[a * N1 + j] = t[a];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int idx = tid + k * NT;
+            const int64_t gi = base + idx;
+            if (idx < G::NV && gi < nval) {
+                const T zv = lds[lds_index<NQ, TS>(idx)];
+                z[gi] = zv;
+                if (COPYP) pcopy[gi] = zv;
+                srz += (double)rv[k] * (double)zv;
+            }
+        }
+        __syncthreads();                               // LDS free for the next group
+    }
+    const double a = block_sum<NT>(srr, red);
+    __syncthreads();
+    const double b = block_sum<NT>(srz, red);
+    if (threadIdx.x == 0) {
+        prr[blockIdx.x] = a;
+        prz[blockIdx.x] = b;
+    }
+}
+
+// Workgroups of one pcg_rz kernel that the device holds at once, cached per
+// device; the grid is min(groups, that, kMaxPartials).
+template <typename T, int P, bool UPDATE, bool COPYP, bool CX>
+int pcg_resident() {
+    constexpr int kMaxDev = 64;
+    static std::atomic<int> cache[kMaxDev];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) dev = 0;
+    int g = cache[dev].load(std::memory_order_relaxed);
+    if (g > 0) return g;
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, reinterpret_cast<const void *>(&pcg_rz_kernel<T, P, UPDATE, COPYP, CX>), PcgShape<P>::NT, 0) !=
+        hipSuccess)
+        per = 0;
+    g = (cus > 0 ? cus : 1) * (per > 0 ? per : 1);
+    cache[dev].store(g, std::memory_order_relaxed);
+    return g;
+}
+
+template <typename T, int P, bool UPDATE, bool COPYP, bool CX>
+int pcg_grid(int64_t noe) {
+    const int64_t groups = (noe + PcgShape<P>::NE - 1) / PcgShape<P>::NE;
+    return (int)std::min<int64_t>(std::min<int64_t>(groups, pcg_resident<T, P, UPDATE, COPYP, CX>()),
+                                  kMaxPartials);
+}
+
+template <typename T, int P, bool UPDATE, bool COPYP, bool CX>
+void rz_launch(dim3 grid, void *r, const void *Ap, void *z, void *pcopy, int64_t noe, const void *prec,
+               const double *ppAp, int np, double *prr, double *prz, cg_state *st, int slot, hipStream_t s) {
+    hipLaunchKernelGGL((pcg_rz_kernel<T, P, UPDATE, COPYP, CX>), grid, dim3(PcgShape<P>::NT), 0, s, (T *)r,
+                       (const T *)Ap, (T *)z, (T *)pcopy, noe, (const T *)prec, ppAp, np, prr, prz, st, slot);
+}
+
+// The standard penalty 5 (p+1)^2 takes the kernel with compile-time
+// coefficients; any other the one that reads the op's tables.
+inline bool standard_pen(int p, double pen) { return pen == 5.0 * (p + 1) * (p + 1); }
+
+template <typename T, int P>
+int launch_rz_one(int mode, bool cx, void *r, const void *Ap, void *z, void *pcopy, int64_t noe,
+                  const void *prec, const double *ppAp, int np, double *prr, double *prz, cg_state *st,
+                  int slot, hipStream_t s)
+{
+    if (cx) {
+        if (mode == 1)
+            rz_launch<T, P, true, false, true>(dim3(pcg_grid<T, P, true, false, true>(noe)), r, Ap, z, pcopy,
+                                               noe, prec, ppAp, np, prr, prz, st, slot, s);
+        else if (mode == 2)
+            rz_launch<T, P, false, true, true>(dim3(pcg_grid<T, P, false, true, true>(noe)), r, Ap, z, pcopy,
+                                               noe, prec, ppAp, np, prr, prz, st, slot, s);
+        else
+            rz_launch<T, P, false, false, true>(dim3(pcg_grid<T, P, false, false, true>(noe)), r, Ap, z,
+                                                pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+    } else {
+        if (mode == 1)
+            rz_launch<T, P, true, false, false>(dim3(pcg_grid<T, P, true, false, false>(noe)), r, Ap, z,
+                                                pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+        else if (mode == 2)
+            rz_launch<T, P, false, true, false>(dim3(pcg_grid<T, P, false, true, false>(noe)), r, Ap, z,
+                                                pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+        else
+            rz_launch<T, P, false, false, false>(dim3(pcg_grid<T, P, false, false, false>(noe)), r, Ap, z,
+                                                 pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+    }
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+template <typename T, int P>
+int grid_one(int mode, bool cx, int64_t noe) {
+    if (cx)
+        return mode == 1 ? pcg_grid<T, P, true, false, true>(noe)
+                         : mode == 2 ? pcg_grid<T, P, false, true, true>(noe)
+                                     : pcg_grid<T, P, false, false, true>(noe);
+    return mode == 1 ? pcg_grid<T, P, true, false, false>(noe)
+                     : mode == 2 ? pcg_grid<T, P, false, true, false>(noe)
+                                 : pcg_grid<T, P, false, false, false>(noe);
+}
+
+template <typename T, int... Ps>
+struct PcgDispatch;
+
+template <typename T>
+struct PcgDispatch<T> {
+    static int launch(int, int, bool, void *, const void *, void *, void *, int64_t, const void *,
+                      const double *, int, double *, double *, cg_state *, int, hipStream_t) {
+        return SIPG_ERR_UNSUPPORTED;
+    }
+    static int grid(int, int, bool, int64_t) { return -1; }
+};
+
+template <typename T, int P0, int... Ps>
+struct PcgDispatch<T, P0, Ps...> {
+    static int launch(int p, int mode, bool cx, void *r, const void *Ap, void *z, void *pcopy, int64_t noe,
+                      const void *prec, const double *ppAp, int np, double *prr, double *prz, cg_state *st,
+                      int slot, hipStream_t s) {
+        if (p == P0)
+            return launch_rz_one<T, P0>(mode, cx, r, Ap, z, pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+        return PcgDispatch<T, Ps...>::launch(p, mode, cx, r, Ap, z, pcopy, noe, prec, ppAp, np, prr, prz, st,
+                                             slot, s);
+    }
+    static int grid(int p, int mode, bool cx, int64_t noe) {
+        if (p == P0) return grid_one<T, P0>(mode, cx, noe);
+        return PcgDispatch<T, Ps...>::grid(p, mode, cx, noe);
+    }
+};
+
+template <typename T>
+using PcgDegrees = PcgDispatch<T, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16>;
+
+constexpr int VT = kVecThreads;
+
+__device__ __forceinline__ double vsum_all(double v) {
+    __shared__ double red[VT / 64];
+    __shared__ double total;
+    return group_sum_all<VT>(v, red, &total);
+}
+
+__device__ __forceinline__ double vreduce(const double *__restrict__ part, int np) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += VT) s += part[i];
+    return vsum_all(s);
+}
+
+// The step after z = M^-1 r: beta = r.z(new) / r.z(old) and p = z + beta p;
+// unless this iteration's r was replaced by the true residual (whose x
+// update ran before it), also x += alpha p with the old p.  Block 0 writes
+// the next slot: rr = r.r (the stopping test and the host's decisions),
+// rz = r.z.  Flag semantics are those of the CG p update.
+template <typename T>
+__global__ __launch_bounds__(VT) void pcg_update_p_kernel(T *__restrict__ x, T *__restrict__ p,
+                                                          const T *__restrict__ z, int64_t n,
+                                                          const double *__restrict__ ppAp, int np_pAp,
+                                                          const double *__restrict__ prr,
+                                                          const double *__restrict__ prz, int np,
+                                                          cg_state *st, int slot) {
+    const cg_slot cur = st->s[slot];
+    if (cur.done | st->err) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->s[slot ^ 1] = cur;   // carry forward
+        return;
+    }
+    const double rr_new = vreduce(prr, np);
+    const double rz_new = vreduce(prz, np);
+    const double pAp = vreduce(ppAp, np_pAp);
+    const bool replaced = (cur.flags & kSlotReplaced) != 0;
+    const double beta_d = (cur.flags & kSlotRestart) ? 0.0 : rz_new / cur.rz;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const bool met = !(sqrt(rr_new) > st->stop);
+        const bool need_true = st->true_stop && !replaced;
+        cg_slot nx;
+        nx.rr = rr_new;
+        nx.alpha = 0.0;
+        nx.beta = beta_d;
+        nx.pAp = 0.0;
+        nx.rz = rz_new;
+        nx.it = cur.it + 1;
+        nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
+        nx.checked = replaced ? 1 : 0;
+        nx.flags = (met && need_true) ? kSlotWantCheck : 0;
+        st->s[slot ^ 1] = nx;
+        if (replaced) st->rr_ref = rr_new;
+        if (!isfinite(rr_new) || !isfinite(rz_new)) st->err = 3;
+        else if (!(rz_new > 0.0) && rr_new > 0.0) st->err = 2;   // M not positive on r
+    }
+    const T alpha = (T)(cur.rz / pAp);
+    const T beta = (T)beta_d;
+    for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < n; i += (int64_t)gridDim.x * VT) {
+        const T pi = p[i];
+        if (!replaced) x[i] = x[i] + alpha * pi;
+        p[i] = z[i] + beta * pi;
+    }
+}
+
+// x += alpha p (alpha = slot.rz / p.Ap): the x half of the step, run before
+// a true residual replaces r in the same iteration.
+template <typename T>
+__global__ __launch_bounds__(VT) void pcg_x_kernel(T *__restrict__ x, const T *__restrict__ p, int64_t n,
+                                                   const double *__restrict__ ppAp, int np,
+                                                   const cg_state *st, int slot) {
+    if (st->s[slot].done | st->err) return;
+    const double pAp = vreduce(ppAp, np);
+    const T alpha = (T)(st->s[slot].rz / pAp);
+    for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < n; i += (int64_t)gridDim.x * VT)
+        x[i] = x[i] + alpha * p[i];
+}
+
+// Initial state: r.r and r.z from the preconditioner's block sums, b.b
+// from the true residual's.
+__global__ __launch_bounds__(VT) void pcg_init_finalize_kernel(const double *__restrict__ prr,
+                                                               const double *__restrict__ prz, int np,
+                                                               const double *__restrict__ pbb, int npb,
+                                                               cg_state *st, double tol, int tol_mode,
+                                                               int maxit, int true_stop) {
+    const double rr = vreduce(prr, np);
+    const double rz = vreduce(prz, np);
+    const double bb = vreduce(pbb, npb);
+    if (threadIdx.x == 0) {
+        const double stop = tol_mode == SIPG_TOL_REL ? tol * sqrt(bb) : tol;
+        cg_slot s0;
+        s0.rr = rr;
+        s0.alpha = s0.beta = s0.pAp = 0.0;
+        s0.rz = rz;
+        s0.it = 0;
+        s0.done = (sqrt(rr) > stop && maxit > 0) ? 0 : 1;
+        s0.checked = 1;
+        s0.flags = 0;
+        st->s[0] = s0;
+        st->s[1] = s0;
+        st->rr0 = rr;
+        st->stop_true = stop;
+        st->rr_ref = rr;
+        st->bb = bb;
+        st->stop = stop;
+        st->maxit = maxit;
+        st->nrep = 0;
+        st->err = (!isfinite(rr) || !isfinite(bb) || !isfinite(rz)) ? 3 : 0;
+        st->true_stop = true_stop;
+    }
+}
+
+}  // namespace
+
+int pcg_num_blocks(int dtype, int p, double pen, int mode, int64_t noe) {
+    const bool cx = standard_pen(p, pen);
+    return dtype == SIPG_F64 ? PcgDegrees<double>::grid(p, mode, cx, noe)
+                             : PcgDegrees<float>::grid(p, mode, cx, noe);
+}
+
+int launch_pcg_rz(int dtype, int p, double pen, int mode, void *r, const void *Ap, void *z, void *pcopy,
+                  int64_t noe, const void *prec, const double *part_pAp, int np_pAp, double *part_rr,
+                  double *part_rz, cg_state *st, int slot, hipStream_t s) {
+    const bool cx = standard_pen(p, pen);
+    if (dtype == SIPG_F64)
+        return PcgDegrees<double>::launch(p, mode, cx, r, Ap, z, pcopy, noe, prec, part_pAp, np_pAp, part_rr,
+                                          part_rz, st, slot, s);
+    return PcgDegrees<float>::launch(p, mode, cx, r, Ap, z, pcopy, noe, prec, part_pAp, np_pAp, part_rr,
+                                     part_rz, st, slot, s);
+}
+
+int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, const double *part_pAp,
+                        int np_pAp, const double *part_rr, const double *part_rz, int np, int nblk,
+                        cg_state *st, int slot, hipStream_t s) {
+    if (dtype == SIPG_F64)
+        hipLaunchKernelGGL(pcg_update_p_kernel<double>, dim3(nblk), dim3(VT), 0, s, (double *)x, (double *)p,
+                           (const double *)z, n, part_pAp, np_pAp, part_rr, part_rz, np, st, slot);
+    else
+        hipLaunchKernelGGL(pcg_update_p_kernel<float>, dim3(nblk), dim3(VT), 0, s, (float *)x, (float *)p,
+                           (const float *)z, n, part_pAp, np_pAp, part_rr, part_rz, np, st, slot);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *part_pAp, int np_pAp, int nblk,
+                 const cg_state *st, int slot, hipStream_t s) {
+    if (dtype == SIPG_F64)
+        hipLaunchKernelGGL(pcg_x_kernel<double>, dim3(nblk), dim3(VT), 0, s, (double *)x, (const double *)p, n,
+                           part_pAp, np_pAp, st, slot);
+    else
+        hipLaunchKernelGGL(pcg_x_kernel<float>, dim3(nblk), dim3(VT), 0, s, (float *)x, (const float *)p, n,
+                           part_pAp, np_pAp, st, slot);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_pcg_init_finalize(const double *part_rr, const double *part_rz, int np, const double *part_bb,
+                             int npb, cg_state *st, double tol, int tol_mode, int maxit, int true_stop,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(pcg_init_finalize_kernel, dim3(1), dim3(VT), 0, s, part_rr, part_rz, np, part_bb, npb,
+                       st, tol, tol_mode, maxit, true_stop);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+}  // namespace sipg_internal

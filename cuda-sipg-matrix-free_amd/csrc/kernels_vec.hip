@@ -118,6 +118,7 @@ __global__ __launch_bounds__(NT) void cg_init_finalize_kernel(const double *__re
         cg_slot s0;
         s0.rr = rr;
         s0.alpha = s0.beta = s0.pAp = 0.0;
+        s0.rz = rr;
         s0.it = 0;
         s0.done = (sqrt(rr) > stop && maxit > 0) ? 0 : 1;   // while-condition of :69
         s0.checked = 1;                                     // r is the true residual here
@@ -196,6 +197,7 @@ __global__ __launch_bounds__(NT) void cg_update_p_kernel(const T *__restrict__ r
         nx.alpha = 0.0;
         nx.beta = beta_d;
         nx.pAp = 0.0;
+        nx.rz = rr_new;
         nx.it = cur.it + 1;
         nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
         nx.checked = replaced ? 1 : 0;

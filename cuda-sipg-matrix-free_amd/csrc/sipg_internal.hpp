@@ -53,6 +53,7 @@ struct cg_slot {
     double alpha;   // of the iteration that produced this slot
     double beta;
     double pAp;
+    double rz;      // preconditioned CG: r.z entering the iteration (CG: r.r)
     int it;         // iterations completed
     int done;       // 0 running, 1 finished (tolerance met or iteration cap)
     int checked;    // 1 when rr is a true residual b - A x
@@ -157,6 +158,35 @@ int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, h
 // rr block sums are its own), optionally with a restart p = r; a no-op
 // for a finished slot.
 int launch_mark_replaced(cg_state *st, int slot, int restart, hipStream_t s);
+
+// ---- launchers (kernels_pcg.hip) ------------------------------------------
+// z = M^-1 r (block-Jacobi: the interior element block's exact inverse), with
+// block sums of r.r and r.z.  mode 1: first r -= alpha Ap (alpha = slot.rz /
+// p.Ap from the apply's sums; no-op once the slot is finished); mode 2: also
+// p = z; mode 0: neither.  pcg_num_blocks = number of block sums written.
+// (pen selects the kernel: the standard 5 (p+1)^2 has its coefficients
+// compiled in, any other reads them from prec)
+int pcg_num_blocks(int dtype, int p, double pen, int mode, int64_t noe);
+int launch_pcg_rz(int dtype, int p, double pen, int mode, void *r, const void *Ap, void *z, void *pcopy,
+                  int64_t noe, const void *prec, const double *part_pAp, int np_pAp, double *part_rr,
+                  double *part_rz, cg_state *st, int slot, hipStream_t s);
+// p = z + beta p (and x += alpha p unless r was replaced this iteration);
+// writes the next slot
+int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, const double *part_pAp,
+                        int np_pAp, const double *part_rr, const double *part_rz, int np, int nblk,
+                        cg_state *st, int slot, hipStream_t s);
+// x += alpha p, before a true residual replaces r in the same iteration
+int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *part_pAp, int np_pAp, int nblk,
+                 const cg_state *st, int slot, hipStream_t s);
+int launch_pcg_init_finalize(const double *part_rr, const double *part_rz, int np, const double *part_bb,
+                             int npb, cg_state *st, double tol, int tol_mode, int maxit, int true_stop,
+                             hipStream_t s);
+// ---- block-Jacobi tables (precond.cpp) ------------------------------------
+// [S (row-major, S[i][a]), Linv[a][c] = 1/(lambda_a + lambda_c)] of the
+// interior element block K U W + W U K, K = K1 + own face terms (see
+// kernels_pcg.hip); SIPG_ERR_INVALID when that block is not positive
+// definite (penalty too small).
+int build_block_jacobi(int p, double pen, std::vector<double> &tab);
 int launch_convert(int from_dtype, int to_dtype, const void *src, void *dst, int64_t n,
                    hipStream_t s);
 // batched launchers (FP64); counts live in device memory so a captured
@@ -215,6 +245,8 @@ struct sipg_op {
     size_t ws_bytes = 0;
     // FP32 twin for the mixed-precision solver (owned)
     sipg_op *twin = nullptr;
+    // block-Jacobi tables in the op's dtype, built on first use
+    void *d_prec = nullptr;
 };
 
 namespace sipg_internal {
@@ -228,7 +260,7 @@ inline bool overlaps(const void *a, const void *b, size_t bytes) {
 int ensure_ws(sipg_op *op, size_t bytes);
 // Partial-buffer layout inside op->d_part
 struct parts {
-    double *pAp, *rr, *bb, *misc;
+    double *pAp, *rr, *bb, *misc, *rz;
     int napply;
 };
 parts part_layout(sipg_op *op);

@@ -41,6 +41,10 @@ struct Engine {
     double replace_factor = 0.0;
     bool true_stop = false;
     int maxit = 0;
+    // preconditioned CG (block-Jacobi): z = M^-1 r, and the device tables
+    bool pcg = false;
+    void *z = nullptr;
+    const void *prec = nullptr;
 
     hipStream_t s = nullptr;
     cg_state *st = nullptr;
@@ -50,6 +54,8 @@ struct Engine {
     bool fold = false;
     int64_t launched = 0;
 
+    int64_t noe() const { return (int64_t)opI->dimx * opI->dimy; }
+
     int init(double tol, int tol_mode) {
         int rc;
         // r = b - A x ; p = r ; r.r ; b.b    (conjugate_gradient_gpu.hpp:40-60)
@@ -57,6 +63,17 @@ struct Engine {
                                opO->pen, nullptr, nullptr, 0, s)))
             return rc;
         if (mixed) SIPG_HIP_TRY(hipMemsetAsync(xI, 0, (size_t)opI->n * dtype_size(opI->dtype), s));
+        if (pcg) {
+            // r = b - A x ; z = M^-1 r ; p = z ; r.r, r.z, b.b
+            // (conjugate_gradient_gpu.hpp:136-160)
+            if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, nullptr, opI->n, P.rr, P.bb, nb, s)))
+                return rc;
+            if ((rc = launch_pcg_rz(opI->dtype, opI->p, opI->pen, 2, r, nullptr, z, p, noe(), prec, nullptr, 0, P.rr, P.rz,
+                                    nullptr, 0, s)))
+                return rc;
+            return launch_pcg_init_finalize(P.rr, P.rz, npc(2), P.bb, nb, st, tol, tol_mode, maxit,
+                                            true_stop ? 1 : 0, s);
+        }
         if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, p, opI->n, P.rr, P.bb, nb, s)))
             return rc;
         return launch_cg_init_finalize(P.rr, P.bb, nb, st, tol, tol_mode, maxit, true_stop ? 1 : 0, s);
@@ -67,7 +84,53 @@ struct Engine {
     // increments into x), so the p update forms beta and p from the true
     // residual: the reliable update of van der Vorst & Ye.  `restart`
     // makes that p update restart from p = r.
+    int npc(int mode) const { return pcg_num_blocks(opI->dtype, opI->p, opI->pen, mode, noe()); }
+
+    // One PCG iteration (conjugate_gradient_gpu.hpp:161-205): the apply with
+    // p.Ap, then r -= alpha Ap fused with z = M^-1 r and r.r, r.z, then
+    // x += alpha p and p = z + beta p.  With `replace`, x is updated first
+    // and r is recomputed as b - A x (then z and its sums) before the p
+    // update, as in the CG iteration.
+    int iteration_pcg(bool replace, bool restart_p) {
+        const int slot = (int)(launched & 1);
+        int rc;
+        if ((rc = launch_apply(opI->dtype, opI->p, p, Ap, opI->d_tab, opI->dimx, opI->dimy, opI->pen,
+                               P.pAp, st, slot, s)))
+            return rc;
+        const double *src = P.pAp;
+        int np = P.napply;
+        if (fold) {
+            if ((rc = launch_reduce_partials(P.pAp, P.napply, P.misc, s))) return rc;
+            src = P.misc;
+            np = 1;
+        }
+        if ((rc = launch_pcg_rz(opI->dtype, opI->p, opI->pen, 1, r, Ap, z, nullptr, noe(), prec, src, np, P.rr, P.rz, st,
+                                slot, s)))
+            return rc;
+        int nrz = npc(1);
+        if (replace) {
+            if ((rc = launch_pcg_x(opI->dtype, xI, p, opI->n, src, np, nb, st, slot, s))) return rc;
+            if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s))) return rc;
+            if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
+                                   opO->pen, nullptr, nullptr, 0, s)))
+                return rc;
+            if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, nullptr, opI->n, P.rr,
+                                           nullptr, nb, s)))
+                return rc;
+            if ((rc = launch_pcg_rz(opI->dtype, opI->p, opI->pen, 0, r, nullptr, z, nullptr, noe(), prec, nullptr, 0, P.rr,
+                                    P.rz, nullptr, 0, s)))
+                return rc;
+            nrz = npc(0);
+            if ((rc = launch_mark_replaced(st, slot, restart_p ? 1 : 0, s))) return rc;
+        }
+        if ((rc = launch_pcg_update_p(opI->dtype, xI, p, z, opI->n, src, np, P.rr, P.rz, nrz, nb, st, slot, s)))
+            return rc;
+        ++launched;
+        return SIPG_OK;
+    }
+
     int iteration(bool replace, bool restart_p) {
+        if (pcg) return iteration_pcg(replace, restart_p);
         const int slot = (int)(launched & 1);
         int rc;
         if ((rc = launch_apply(opI->dtype, opI->p, p, Ap, opI->d_tab, opI->dimx, opI->dimy, opI->pen,
@@ -206,17 +269,45 @@ int check_opts(const sipg_cg_opts *o) {
     return SIPG_OK;
 }
 
-int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg_cg_info *info) {
+// Block-Jacobi tables of the op (its dtype), built and uploaded once.
+int ensure_prec(sipg_op *op) {
+    if (op->d_prec) return SIPG_OK;
+    std::vector<double> tab;
+    int rc = build_block_jacobi(op->p, op->pen, tab);
+    if (rc) return rc;
+    const size_t es = dtype_size(op->dtype);
+    std::vector<float> tf;
+    const void *src = tab.data();
+    if (op->dtype == SIPG_F32) {
+        tf.assign(tab.begin(), tab.end());
+        src = tf.data();
+    }
+    void *d = nullptr;
+    SIPG_HIP_TRY(hipMalloc(&d, tab.size() * es));
+    const hipError_t e = hipMemcpy(d, src, tab.size() * es, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        set_last_hip_error(e);
+        return SIPG_ERR_HIP;
+    }
+    op->d_prec = d;
+    return SIPG_OK;
+}
+
+int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg_cg_info *info,
+                bool pcg = false) {
     int rc = check_opts(o);
     if (rc) return rc;
     const size_t vb = (size_t)op->n * dtype_size(op->dtype);
     if (overlaps(x, b, vb)) return SIPG_ERR_ALIAS;
+    if (pcg && (rc = ensure_prec(op))) return rc;
     // With true residuals the iterate is group-updated (van der Vorst & Ye
     // 2000): the CG increments go to a separate vector z that is folded
     // into x only when r is recomputed, so the rounding of x += alpha p
     // does not accumulate at the scale of |x| between recomputations.
     const bool group = o->replace_factor > 0.0 || o->true_residual_stop != 0;
-    if ((rc = ensure_ws(op, (group ? 4 : 3) * vb))) return rc;
+    const int nvec = 3 + (group ? 1 : 0) + (pcg ? 1 : 0);
+    if ((rc = ensure_ws(op, nvec * vb))) return rc;
     char *ws = (char *)op->ws;
     Engine E;
     E.opI = E.opO = op;
@@ -226,6 +317,9 @@ int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg
     E.r = ws;
     E.p = ws + vb;
     E.Ap = ws + 2 * vb;
+    E.pcg = pcg;
+    E.z = pcg ? ws + (size_t)(nvec - 1) * vb : nullptr;
+    E.prec = op->d_prec;
     E.AxO = E.Ap;        // the initial and the replacement A x use the Ap slot
     E.b = b;
     E.replace_factor = o->replace_factor;
@@ -265,6 +359,22 @@ int sipg_cg(sipg_op *op, void *x, const void *b, double tol, int tol_mode, int m
 int sipg_cg_ex(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {
     if (!op || !x || !b) return SIPG_ERR_INVALID;
     return solve_plain(op, x, b, opts, info);
+}
+
+int sipg_pcg(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {
+    if (!op || !x || !b) return SIPG_ERR_INVALID;
+    return solve_plain(op, x, b, opts, info, true);
+}
+
+int sipg_prec_apply(sipg_op *op, const void *r, void *z) {
+    if (!op || !r || !z) return SIPG_ERR_INVALID;
+    const size_t vb = (size_t)op->n * dtype_size(op->dtype);
+    if (overlaps(r, z, vb)) return SIPG_ERR_ALIAS;
+    int rc = ensure_prec(op);
+    if (rc) return rc;
+    parts P = part_layout(op);
+    return launch_pcg_rz(op->dtype, op->p, op->pen, 0, const_cast<void *>(r), nullptr, z, nullptr,
+                         (int64_t)op->dimx * op->dimy, op->d_prec, nullptr, 0, P.rr, P.rz, nullptr, 0, op->stream);
 }
 
 int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {

@@ -93,6 +93,8 @@ def _declare(L):
         "sipg_true_relres": (i, [_vp, _vp, _vp, _dp]),
         "sipg_cg_ex": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
         "sipg_cg_mixed": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
+        "sipg_pcg": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
+        "sipg_prec_apply": (i, [_vp, _vp, _vp]),
         "sipg_vec_dot": (i, [i, i64, _vp, _vp, _dp, _vp]),
         "sipg_vec_axpy": (i, [i, i64, d, _vp, _vp, _vp]),
         "sipg_vec_copy": (i, [i, i64, _vp, _vp, _vp]),
@@ -291,6 +293,17 @@ class Operator:
         info = CgInfo()
         st = lib().sipg_cg_mixed(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
         return st, info
+
+    def pcg(self, x, b, tol, tol_mode=TOL_REL, maxit=0, replace_factor=0.0, true_stop=True, restart=0):
+        """Block-Jacobi preconditioned CG (sipg_pcg); options as cg_ex."""
+        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1 if true_stop else 0, int(restart))
+        info = CgInfo()
+        st = lib().sipg_pcg(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
+        return st, info
+
+    def prec_apply(self, r, z):
+        """z = M^-1 r with the block-Jacobi preconditioner of pcg."""
+        check(lib().sipg_prec_apply(self.h, r.ptr, z.ptr), "prec_apply")
 
     def true_relres(self, x, b):
         r = ctypes.c_double()

@@ -185,6 +185,23 @@ int sipg_cg_ex(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, si
  * are FP64 device vectors. */
 int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
 
+/* Block-Jacobi preconditioned CG: replaces preconditioned_conjugate_gradient
+ * (conjugate_gradient_gpu.hpp:116-213) with block_diagonal_preconditioner_2d
+ * (laplacian_operator.hpp:95-186).  M is the exact diagonal block of A of an
+ * interior element, (K U W + W U K) with K the 1-D stiffness plus the
+ * element's own face terms, applied to every element and inverted by fast
+ * diagonalisation; the reference instead inverts a dense block built with
+ * pen truncated to an integer and keeps part of it (mode_matrix.hpp:495-516).
+ * opts and info as for sipg_cg_ex: res_norm and the stopping test are on
+ * r.r (not r.z), with the same reliable updates and true-residual stop.
+ * x is in/out in the op's dtype.  SIPG_ERR_INVALID when the interior block
+ * is not positive definite (penalty too small). */
+int sipg_pcg(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
+
+/* z = M^-1 r with the preconditioner of sipg_pcg (device vectors of the
+ * op's dtype, r and z distinct); replaces _prec_mvm (abs_mvm.hpp:15). */
+int sipg_prec_apply(sipg_op *op, const void *r, void *z);
+
 /* ||b - A x||_2 / ||b||_2 in FP64 (op must be SIPG_F64). */
 int sipg_true_relres(sipg_op *op, const void *x, const void *b, double *relres);
 

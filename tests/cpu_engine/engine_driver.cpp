@@ -29,7 +29,7 @@ static double relres(int d, int p, double pen, const std::vector<double> &x, con
 }
 
 static void run(const char *name, int d, int p, double pen, int mode, double tol, int maxit, double rf,
-                int restart = 1) {
+                int restart = 1, int true_stop = 1) {
     sipg_op *op = nullptr;
     int rc = sipg_op_create(&op, d, d, p, pen, SIPG_F64, nullptr);
     if (rc) { printf("{\"case\":\"%s\",\"create_rc\":%d}\n", name, rc); return; }
@@ -41,7 +41,7 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
     sipg_malloc(&b, n * 8);
     sipg_upload(op, bh.data(), b);  // reference order -> element-major device order
     sipg_upload(op, xh.data(), x);
-    sipg_cg_opts o{tol, SIPG_TOL_REL, maxit, rf, 1, restart};
+    sipg_cg_opts o{tol, SIPG_TOL_REL, maxit, rf, true_stop, restart};
     sipg_cg_info info{};
     if (mode == 0) {
         int it = 0;
@@ -51,8 +51,10 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
         info.res_norm = res;
     } else if (mode == 1) {
         rc = sipg_cg_ex(op, x, b, &o, &info);
-    } else {
+    } else if (mode == 2) {
         rc = sipg_cg_mixed(op, x, b, &o, &info);
+    } else {
+        rc = sipg_pcg(op, x, b, &o, &info);
     }
     sipg_download(op, x, xh.data());
     std::vector<double> xo(n, 0.0);
@@ -76,5 +78,10 @@ int main() {
     run("mixed", 8, 4, 125.0, 2, 1e-10, 0, 0.1);
     run("mixed_cap", 8, 4, 125.0, 2, 1e-10, 30, 0.1);
     run("mixed_keep_p", 8, 4, 125.0, 2, 1e-10, 0, 0.1, 0);
+    run("pcg_replace", 8, 4, 125.0, 3, 1e-10, 0, 0.1, 0);
+    run("pcg_true_stop", 8, 4, 125.0, 3, 1e-10, 0, 0.0, 0);
+    run("pcg_plain", 8, 4, 125.0, 3, 1e-10, 0, 0.0, 0, 0);
+    run("pcg_other_pen", 8, 4, 100.0, 3, 1e-10, 0, 0.1, 0);
+    run("pcg_cap", 8, 4, 125.0, 3, 1e-12, 5, 0.0, 0, 0);
     return 0;
 }

@@ -121,6 +121,7 @@ int launch_cg_init_finalize(const double *prr, const double *pbb, int np, cg_sta
     const double stop = tol_mode == SIPG_TOL_REL ? tol * std::sqrt(bb) : tol;
     cg_slot s0{};
     s0.rr = rr;
+    s0.rz = rr;
     s0.done = (std::sqrt(rr) > stop && maxit > 0) ? 0 : 1;
     s0.checked = 1;
     st->s[0] = s0;
@@ -176,6 +177,7 @@ static void update_p_t(const T *r, T *p, int64_t n, const double *prr, int np, c
     const bool need_true = st->true_stop && !replaced;
     cg_slot nx{};
     nx.rr = rr_new;
+    nx.rz = rr_new;
     nx.beta = beta_d;
     nx.it = cur.it + 1;
     nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
@@ -283,6 +285,153 @@ int launch_rhs_fourier(int dtype, int p, int dim, const void *, int nmodes, cons
     orc_rhs(dim, p, 5.0 * (p + 1) * (p + 1), &pb, br.data());   // pen does not enter with g = 0
     for (int64_t e = 0; e < noe; ++e)
         for (int64_t q = 0; q < nq; ++q) tp<double>(b)[e * nq + q] = br[q * noe + e];
+    return SIPG_OK;
+}
+
+// ---- PCG (kernels_pcg.hip) ----------------------------------------------
+// z = M^-1 r per element with the op's tables [S, Linv]:
+// Z = S [(S^T R S) .* Linv] S^T on each (p+1) x (p+1) tile.
+int pcg_num_blocks(int, int p, double, int, int64_t) { return p <= kMaxDegree ? kMockApplyBlocks : -1; }
+
+template <typename T>
+static void pcg_rz_t(int p, int mode, T *r, const T *Ap, T *z, T *pcopy, int64_t noe, const T *prec,
+                     const double *ppAp, int np, double *prr, double *prz, cg_state *st, int slot) {
+    if (mode == 1) {
+        if (st->s[slot].done | st->err) return;
+        const double pAp = sum_partials(ppAp, np);
+        const double a = st->s[slot].rz / pAp;
+        if (!(pAp > 0.0) || !std::isfinite(a)) { st->err = std::isfinite(pAp) ? 2 : 3; return; }
+        const int64_t n = noe * (p + 1) * (p + 1);
+        for (int64_t i = 0; i < n; ++i) r[i] = r[i] - (T)a * Ap[i];
+    }
+    const int n1 = p + 1, nq = n1 * n1;
+    const T *S = prec, *L = prec + nq;
+    std::vector<double> t1(nq), t2(nq);
+    double srr = 0, srz = 0;
+    for (int64_t e = 0; e < noe; ++e) {
+        const T *R = r + e * nq;
+        for (int a = 0; a < n1; ++a)                 // t1 = S^T R
+            for (int j = 0; j < n1; ++j) {
+                double acc = 0;
+                for (int i = 0; i < n1; ++i) acc += (double)S[i * n1 + a] * R[i * n1 + j];
+                t1[a * n1 + j] = acc;
+            }
+        for (int a = 0; a < n1; ++a)                 // t2 = (t1 S) .* L
+            for (int c = 0; c < n1; ++c) {
+                double acc = 0;
+                for (int b = 0; b < n1; ++b) acc += t1[a * n1 + b] * S[b * n1 + c];
+                t2[a * n1 + c] = acc * L[a * n1 + c];
+            }
+        for (int a = 0; a < n1; ++a)                 // t1 = t2 S^T
+            for (int d = 0; d < n1; ++d) {
+                double acc = 0;
+                for (int c = 0; c < n1; ++c) acc += t2[a * n1 + c] * S[d * n1 + c];
+                t1[a * n1 + d] = acc;
+            }
+        for (int a = 0; a < n1; ++a)                 // Z = S t1
+            for (int j = 0; j < n1; ++j) {
+                double acc = 0;
+                for (int i = 0; i < n1; ++i) acc += (double)S[a * n1 + i] * t1[i * n1 + j];
+                const T zv = (T)acc;
+                z[e * nq + a * n1 + j] = zv;
+                if (mode == 2) pcopy[e * nq + a * n1 + j] = zv;
+            }
+        for (int q = 0; q < nq; ++q) {
+            srr += (double)R[q] * (double)R[q];
+            srz += (double)R[q] * (double)z[e * nq + q];
+        }
+    }
+    put_partials(prr, kMockApplyBlocks, srr);
+    put_partials(prz, kMockApplyBlocks, srz);
+}
+
+int launch_pcg_rz(int dtype, int p, double, int mode, void *r, const void *Ap, void *z, void *pcopy,
+                  int64_t noe, const void *prec, const double *part_pAp, int np_pAp, double *part_rr,
+                  double *part_rz, cg_state *st, int slot, hipStream_t) {
+    if (p < 1 || p > kMaxDegree || !prec) return SIPG_ERR_INVALID;
+    if (dtype == SIPG_F64)
+        pcg_rz_t<double>(p, mode, tp<double>(r), tp<double>(Ap), tp<double>(z), tp<double>(pcopy), noe,
+                         tp<double>(prec), part_pAp, np_pAp, part_rr, part_rz, st, slot);
+    else
+        pcg_rz_t<float>(p, mode, tp<float>(r), tp<float>(Ap), tp<float>(z), tp<float>(pcopy), noe,
+                        tp<float>(prec), part_pAp, np_pAp, part_rr, part_rz, st, slot);
+    return SIPG_OK;
+}
+
+template <typename T>
+static void pcg_update_p_t(T *x, T *p, const T *z, int64_t n, const double *ppAp, int np_pAp,
+                           const double *prr, const double *prz, int np, cg_state *st, int slot) {
+    const cg_slot cur = st->s[slot];
+    if (cur.done | st->err) { st->s[slot ^ 1] = cur; return; }
+    const double rr_new = sum_partials(prr, np), rz_new = sum_partials(prz, np);
+    const double pAp = sum_partials(ppAp, np_pAp);
+    const bool replaced = (cur.flags & kSlotReplaced) != 0;
+    const double beta_d = (cur.flags & kSlotRestart) ? 0.0 : rz_new / cur.rz;
+    const bool met = !(std::sqrt(rr_new) > st->stop);
+    const bool need_true = st->true_stop && !replaced;
+    cg_slot nx{};
+    nx.rr = rr_new;
+    nx.rz = rz_new;
+    nx.beta = beta_d;
+    nx.it = cur.it + 1;
+    nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
+    nx.checked = replaced ? 1 : 0;
+    nx.flags = (met && need_true) ? kSlotWantCheck : 0;
+    st->s[slot ^ 1] = nx;
+    if (replaced) st->rr_ref = rr_new;
+    if (!std::isfinite(rr_new) || !std::isfinite(rz_new)) st->err = 3;
+    else if (!(rz_new > 0.0) && rr_new > 0.0) st->err = 2;
+    const T a = (T)(cur.rz / pAp), b = (T)beta_d;
+    for (int64_t i = 0; i < n; ++i) {
+        const T pi = p[i];
+        if (!replaced) x[i] = x[i] + a * pi;
+        p[i] = z[i] + b * pi;
+    }
+}
+
+int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, const double *part_pAp,
+                        int np_pAp, const double *part_rr, const double *part_rz, int np, int, cg_state *st,
+                        int slot, hipStream_t) {
+    if (dtype == SIPG_F64)
+        pcg_update_p_t<double>(tp<double>(x), tp<double>(p), tp<double>(z), n, part_pAp, np_pAp, part_rr, part_rz,
+                               np, st, slot);
+    else
+        pcg_update_p_t<float>(tp<float>(x), tp<float>(p), tp<float>(z), n, part_pAp, np_pAp, part_rr, part_rz, np,
+                              st, slot);
+    return SIPG_OK;
+}
+
+int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *part_pAp, int np_pAp, int,
+                 const cg_state *st, int slot, hipStream_t) {
+    if (st->s[slot].done | st->err) return SIPG_OK;
+    const double a = st->s[slot].rz / sum_partials(part_pAp, np_pAp);
+    for (int64_t i = 0; i < n; ++i) {
+        if (dtype == SIPG_F64) tp<double>(x)[i] += (double)a * tp<double>(p)[i];
+        else tp<float>(x)[i] += (float)a * tp<float>(p)[i];
+    }
+    return SIPG_OK;
+}
+
+int launch_pcg_init_finalize(const double *prr, const double *prz, int np, const double *pbb, int npb,
+                             cg_state *st, double tol, int tol_mode, int maxit, int true_stop, hipStream_t) {
+    const double rr = sum_partials(prr, np), rz = sum_partials(prz, np), bb = sum_partials(pbb, npb);
+    const double stop = tol_mode == SIPG_TOL_REL ? tol * std::sqrt(bb) : tol;
+    cg_slot s0{};
+    s0.rr = rr;
+    s0.rz = rz;
+    s0.done = (std::sqrt(rr) > stop && maxit > 0) ? 0 : 1;
+    s0.checked = 1;
+    st->s[0] = s0;
+    st->s[1] = s0;
+    st->rr0 = rr;
+    st->stop_true = stop;
+    st->rr_ref = rr;
+    st->bb = bb;
+    st->stop = stop;
+    st->maxit = maxit;
+    st->nrep = 0;
+    st->err = (!std::isfinite(rr) || !std::isfinite(bb) || !std::isfinite(rz)) ? 3 : 0;
+    st->true_stop = true_stop;
     return SIPG_OK;
 }
 

@@ -75,6 +75,26 @@ def test_mixed_cap_reports_not_converged(results):
     assert r["rc"] == 6 and r["iters"] == 30
 
 
+@pytest.mark.parametrize("case", ["pcg_replace", "pcg_true_stop", "pcg_other_pen"])
+def test_pcg_reaches_true_residual_in_fewer_iterations(results, case):
+    # block-Jacobi PCG (the host protocol of sipg_pcg with the kernels'
+    # CPU stand-ins): same solution as the oracle's CG, fewer iterations
+    r = results[case]
+    assert r["rc"] == 0 and r["relres"] <= 1e-10 and r["diff"] <= 1e-8, r
+    assert r["iters"] < 0.8 * r["oracle_iters"], r
+
+
+def test_pcg_plain_stops_on_the_recursion(results):
+    r = results["pcg_plain"]
+    assert r["rc"] == 0 and r["reps"] == 0 and r["relres"] <= 2e-10, r
+    assert r["iters"] < 0.8 * r["oracle_iters"], r
+
+
+def test_pcg_iteration_cap(results):
+    r = results["pcg_cap"]
+    assert r["rc"] == 6 and r["iters"] == 5, r
+
+
 @pytest.fixture(scope="module")
 def batch_results(results):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="halt_on_error=1")

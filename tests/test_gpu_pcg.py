@@ -1,0 +1,134 @@
+"""Block-Jacobi preconditioner and PCG on the GPU (sipg_prec_apply, sipg_pcg).
+
+The reference's preconditioned CG (conjugate_gradient_gpu.hpp:116-213) uses
+a block-diagonal preconditioner (laplacian_operator.hpp:95-186).  The
+library's M is the exact diagonal block of A of an interior element, so for
+a vector x supported on one interior element, M^-1 applied to (A x)
+restricted to that element gives x back: the oracle's operator pins the
+tables (eigenvectors, eigenvalues, contraction order) independently.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _one_element(dim, p, e, rng):
+    n1 = p + 1
+    noe = dim * dim
+    x = np.zeros(n1 * n1 * noe)
+    x.reshape(n1 * n1, noe)[:, e] = rng.uniform(-1, 1, n1 * n1)
+    return x
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 7, 8, 9, 12, 15, 16])
+@pytest.mark.parametrize("standard", [True, False])
+def test_prec_inverts_the_interior_block(sipg, oracle, p, standard):
+    dim = 3
+    pen = sipg.pen_sigma(p) if standard else 3.0 * (p + 1) ** 2 + 7.5
+    noe, n1 = dim * dim, p + 1
+    x = _one_element(dim, p, 4, np.random.default_rng(p))          # element (1, 1): interior
+    r = oracle.apply(dim, dim, p, pen, x).reshape(n1 * n1, noe)
+    mask = np.zeros_like(r)
+    mask[:, 4] = 1.0
+    r = (r * mask).reshape(-1)
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    rd, zd = op.vector(), op.vector()
+    op.upload(r, rd)
+    op.prec_apply(rd, zd)
+    z = op.download(zd).reshape(n1 * n1, noe)
+    want = x.reshape(n1 * n1, noe)[:, 4]
+    assert np.max(np.abs(z[:, 4] - want)) <= 1e-10 * np.max(np.abs(want)) * n1
+
+
+def test_prec_fp32_matches_fp64(sipg):
+    dim, p = 16, 8
+    pen = sipg.pen_sigma(p)
+    op64 = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    op32 = sipg.Operator(dim, dim, p, pen, sipg.F32)
+    r = np.random.default_rng(3).uniform(-1, 1, op64.n)
+    a, b = op64.vector(), op64.vector()
+    op64.upload(r, a)
+    op64.prec_apply(a, b)
+    z64 = op64.download(b)
+    c, d = op32.vector(), op32.vector()
+    op32.upload(r.astype(np.float32), c)
+    op32.prec_apply(c, d)
+    z32 = op32.download(d)
+    assert np.max(np.abs(z32 - z64)) <= 1e-5 * np.max(np.abs(z64))
+
+
+def test_prec_is_symmetric_positive(sipg):
+    # M^-1 is SPD: a.(M^-1 b) = b.(M^-1 a) and a.(M^-1 a) > 0
+    dim, p = 12, 5
+    op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F64)
+    rng = np.random.default_rng(11)
+    a, b = rng.uniform(-1, 1, op.n), rng.uniform(-1, 1, op.n)
+    da, db, za, zb = op.vector(), op.vector(), op.vector(), op.vector()
+    op.upload(a, da)
+    op.upload(b, db)
+    op.prec_apply(da, za)
+    op.prec_apply(db, zb)
+    Ma, Mb = op.download(za), op.download(zb)
+    assert abs(a @ Mb - b @ Ma) <= 1e-12 * abs(a @ Mb)
+    assert a @ Ma > 0
+
+
+@pytest.mark.parametrize("dim,p", [(8, 3), (6, 16), (10, 9)])
+def test_pcg_matches_the_oracle_solution(sipg, oracle, dim, p):
+    pen = sipg.pen_sigma(p)
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    b_ref = np.random.default_rng(dim + p).uniform(-1, 1, op.n)
+    b, x = op.vector(), op.vector()
+    op.upload(b_ref, b)
+    x.zero()
+    st, info = op.pcg(x, b, 1e-10, sipg.TOL_REL, replace_factor=3e-3, true_stop=True)
+    assert st == sipg.OK, (st, info.iters)
+    assert op.true_relres(x, b) <= 1e-10
+    xo, _, _ = oracle.cg(dim, dim, p, pen, b_ref, tol=1e-13)
+    _, it_cg, _ = oracle.cg(dim, dim, p, pen, b_ref, tol=1e-10)
+    u = op.download(x)
+    assert np.linalg.norm(u - xo) <= 1e-8 * np.linalg.norm(xo)
+    assert info.iters < 0.8 * it_cg, (info.iters, it_cg)
+
+
+def test_pcg_other_penalty_and_fp32(sipg):
+    # a penalty other than 5 (p+1)^2 takes the kernel that reads the tables
+    dim, p = 16, 5
+    op = sipg.Operator(dim, dim, p, 100.0, sipg.F64)
+    b_ref = np.random.default_rng(5).uniform(-1, 1, op.n)
+    b, x = op.vector(), op.vector()
+    op.upload(b_ref, b)
+    x.zero()
+    st, info = op.pcg(x, b, 1e-10, sipg.TOL_REL, replace_factor=3e-3, true_stop=True)
+    assert st == sipg.OK and op.true_relres(x, b) <= 1e-10
+    op32 = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F32)
+    b32, x32 = op32.vector(), op32.vector()
+    op32.upload(b_ref.astype(np.float32), b32)
+    x32.zero()
+    st, info = op32.pcg(x32, b32, 1e-5, sipg.TOL_REL, true_stop=False)
+    assert st == sipg.OK and info.res_norm <= 1e-5 * info.b_norm
+
+
+def test_pcg_c2_reaches_1e10_in_fewer_iterations(sipg):
+    # 256 x 256, p = 8: the benchmark problem
+    dim, p = 256, 8
+    op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F64)
+    b, x = op.vector(), op.vector()
+    op.rhs_fourier(sipg.forcing_draw(0, 4, 4, 4), b)
+    x.zero()
+    st, info = op.pcg(x, b, 1e-10, sipg.TOL_REL, maxit=20000, replace_factor=3e-3, true_stop=True)
+    assert st == sipg.OK, (st, info.iters)
+    assert op.true_relres(x, b) <= 1e-10
+    assert info.iters < 0.5 * 4589, info.iters
+
+
+def test_pcg_rejects_aliasing_and_bad_penalty(sipg):
+    op = sipg.Operator(4, 4, 3, sipg.pen_sigma(3), sipg.F64)
+    v = op.vector()
+    with pytest.raises(Exception):
+        op.prec_apply(v, v)
+    bad = sipg.Operator(4, 4, 3, 1e-3, sipg.F64)      # far below coercivity: block not SPD
+    r, z = bad.vector(), bad.vector()
+    with pytest.raises(Exception):
+        bad.prec_apply(r, z)
