@@ -336,6 +336,51 @@ int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg
     return E.run(info);
 }
 
+// FP32 inner (P)CG cycles on the op's FP32 twin inside FP64 refinement.
+int solve_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info, bool pcg) {
+    if (!op || !x || !b || op->dtype != SIPG_F64) return SIPG_ERR_INVALID;
+    int rc = check_opts(opts);
+    if (rc) return rc;
+    if (overlaps(x, b, (size_t)op->n * 8)) return SIPG_ERR_ALIAS;
+    if (!op->twin) {
+        if ((rc = sipg_op_create(&op->twin, op->dimx, op->dimy, op->p, op->pen, SIPG_F32, op->stream)))
+            return rc;
+    }
+    sipg_op *tw = op->twin;
+    tw->stream = op->stream;
+    if (pcg && (rc = ensure_prec(tw))) return rc;
+    const size_t v32 = (size_t)op->n * 4, v64 = (size_t)op->n * 8;
+    if ((rc = ensure_ws(tw, (pcg ? 5 : 4) * v32))) return rc;    // r^, p^, A p^, x^ (, z^)
+    if ((rc = ensure_ws(op, v64))) return rc;           // A x in FP64
+    char *w = (char *)tw->ws;
+    Engine E;
+    E.pcg = pcg;
+    E.z = pcg ? w + 4 * v32 : nullptr;
+    E.prec = tw->d_prec;
+    E.opI = tw;
+    E.opO = op;
+    E.r = w;
+    E.p = w + v32;
+    E.Ap = w + 2 * v32;
+    E.xI = w + 3 * v32;
+    E.xO = x;
+    E.b = b;
+    E.AxO = op->ws;
+    E.mixed = true;
+    E.restart = opts->restart;
+    E.replace_factor = opts->replace_factor > 0.0 ? opts->replace_factor : 0.1;
+    E.true_stop = true;
+    E.maxit = (int)std::min<int64_t>(opts->maxit > 0 ? (int64_t)opts->maxit : op->n, INT_MAX);
+    E.s = op->stream;
+    E.st = tw->d_state;
+    E.hst = tw->h_state;
+    E.P = part_layout(tw);
+    E.nb = vec_num_blocks(op->n);
+    E.fold = E.P.napply > kMaxPartials;
+    if ((rc = E.init(opts->tol, opts->tol_mode))) return rc;
+    return E.run(info);
+}
+
 }  // namespace
 
 extern "C" {
@@ -378,43 +423,11 @@ int sipg_prec_apply(sipg_op *op, const void *r, void *z) {
 }
 
 int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {
-    if (!op || !x || !b || op->dtype != SIPG_F64) return SIPG_ERR_INVALID;
-    int rc = check_opts(opts);
-    if (rc) return rc;
-    if (overlaps(x, b, (size_t)op->n * 8)) return SIPG_ERR_ALIAS;
-    if (!op->twin) {
-        if ((rc = sipg_op_create(&op->twin, op->dimx, op->dimy, op->p, op->pen, SIPG_F32, op->stream)))
-            return rc;
-    }
-    sipg_op *tw = op->twin;
-    tw->stream = op->stream;
-    const size_t v32 = (size_t)op->n * 4, v64 = (size_t)op->n * 8;
-    if ((rc = ensure_ws(tw, 4 * v32))) return rc;       // r^, p^, A p^, x^
-    if ((rc = ensure_ws(op, v64))) return rc;           // A x in FP64
-    char *w = (char *)tw->ws;
-    Engine E;
-    E.opI = tw;
-    E.opO = op;
-    E.r = w;
-    E.p = w + v32;
-    E.Ap = w + 2 * v32;
-    E.xI = w + 3 * v32;
-    E.xO = x;
-    E.b = b;
-    E.AxO = op->ws;
-    E.mixed = true;
-    E.restart = opts->restart;
-    E.replace_factor = opts->replace_factor > 0.0 ? opts->replace_factor : 0.1;
-    E.true_stop = true;
-    E.maxit = (int)std::min<int64_t>(opts->maxit > 0 ? (int64_t)opts->maxit : op->n, INT_MAX);
-    E.s = op->stream;
-    E.st = tw->d_state;
-    E.hst = tw->h_state;
-    E.P = part_layout(tw);
-    E.nb = vec_num_blocks(op->n);
-    E.fold = E.P.napply > kMaxPartials;
-    if ((rc = E.init(opts->tol, opts->tol_mode))) return rc;
-    return E.run(info);
+    return solve_mixed(op, x, b, opts, info, false);
+}
+
+int sipg_pcg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {
+    return solve_mixed(op, x, b, opts, info, true);
 }
 
 int sipg_true_relres(sipg_op *op, const void *x, const void *b, double *relres) {

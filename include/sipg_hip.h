@@ -198,6 +198,11 @@ int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts,
  * is not positive definite (penalty too small). */
 int sipg_pcg(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
 
+/* Mixed-precision block-Jacobi PCG (no counterpart in the reference): as
+ * sipg_cg_mixed, with the FP32 inner cycles preconditioned as in sipg_pcg
+ * (FP32 tables of the op's FP32 twin).  op must be SIPG_F64. */
+int sipg_pcg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
+
 /* z = M^-1 r with the preconditioner of sipg_pcg (device vectors of the
  * op's dtype, r and z distinct); replaces _prec_mvm (abs_mvm.hpp:15). */
 int sipg_prec_apply(sipg_op *op, const void *r, void *z);

@@ -53,8 +53,10 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
         rc = sipg_cg_ex(op, x, b, &o, &info);
     } else if (mode == 2) {
         rc = sipg_cg_mixed(op, x, b, &o, &info);
-    } else {
+    } else if (mode == 3) {
         rc = sipg_pcg(op, x, b, &o, &info);
+    } else {
+        rc = sipg_pcg_mixed(op, x, b, &o, &info);
     }
     sipg_download(op, x, xh.data());
     std::vector<double> xo(n, 0.0);
@@ -83,5 +85,6 @@ int main() {
     run("pcg_plain", 8, 4, 125.0, 3, 1e-10, 0, 0.0, 0, 0);
     run("pcg_other_pen", 8, 4, 100.0, 3, 1e-10, 0, 0.1, 0);
     run("pcg_cap", 8, 4, 125.0, 3, 1e-12, 5, 0.0, 0, 0);
+    run("pcg_mixed", 8, 4, 125.0, 4, 1e-10, 0, 0.1);
     return 0;
 }

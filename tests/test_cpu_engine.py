@@ -64,7 +64,7 @@ def test_true_residual_stop_without_periodic_replacement(results):
     assert r["oracle_iters"] <= r["iters"] <= r["oracle_iters"] + LAG
 
 
-@pytest.mark.parametrize("case", ["mixed", "mixed_keep_p"])
+@pytest.mark.parametrize("case", ["mixed", "mixed_keep_p", "pcg_mixed"])
 def test_mixed_precision_converges_in_fp64(results, case):
     r = results[case]
     assert r["rc"] == 0 and r["relres"] <= 1e-10 and r["diff"] <= 1e-8

@@ -123,6 +123,22 @@ def test_pcg_c2_reaches_1e10_in_fewer_iterations(sipg):
     assert info.iters < 0.5 * 4589, info.iters
 
 
+@pytest.mark.parametrize("restart", [1, 0])
+def test_pcg_mixed_reaches_fp64_tolerance(sipg, restart):
+    dim, p = 32, 8
+    op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F64)
+    b, x, x64 = op.vector(), op.vector(), op.vector()
+    op.rhs_fourier(sipg.forcing_draw(1, 4, 4, 4), b)
+    x.zero()
+    st, info = op.pcg_mixed(x, b, 1e-10, sipg.TOL_REL, replace_factor=1e-2, restart=restart)
+    assert st == sipg.OK, (st, info.iters)
+    assert op.true_relres(x, b) <= 1e-10
+    x64.zero()
+    st, _ = op.pcg(x64, b, 1e-12, sipg.TOL_REL, replace_factor=3e-3, true_stop=True)
+    u, u64 = op.download(x), op.download(x64)
+    assert np.linalg.norm(u - u64) <= 1e-8 * np.linalg.norm(u64)
+
+
 def test_pcg_rejects_aliasing_and_bad_penalty(sipg):
     op = sipg.Operator(4, 4, 3, sipg.pen_sigma(3), sipg.F64)
     v = op.vector()
