@@ -75,6 +75,8 @@ struct sipg_batch {
     batch_ctl *d_ctl = nullptr;
     double *ws = nullptr;                // R, P, AP, Z (increments since the last true residual)
     double *d_item_part = nullptr, *d_chunk_part = nullptr, *d_part2 = nullptr, *d_part3 = nullptr;
+    double *d_item_rr = nullptr, *d_item_rz = nullptr;   // PCG: per-item r.r and r.z
+    double *d_zq = nullptr;              // PCG: z = M^-1 r (allocated on first use)
     int item_total = 0, chunk_cap = 0;
     int *d_counts = nullptr;             // NSETS x (2 + ngroups)
     int *d_list[NSETS] = {};
@@ -178,10 +180,24 @@ int apply_set(sipg_batch &b, int set, const double *x, double *y) {
     return SIPG_OK;
 }
 
+// z = M^-1 r over the items of a list set (PCG; pv non-null: also p = z)
+int prec_set(sipg_batch &b, int set, double *pv, const cg_state *states, int slot) {
+    for (size_t g = 0; g < b.groups.size(); ++g) {
+        auto &G = b.groups[g];
+        int rc = launch_batch_prec(G.p, G.shape, b.R(), b.d_zq, pv, b.d_inst, G.d_items[set],
+                                   b.count(set, 2 + (int)g), G.cap, b.d_item_rr + G.base, b.d_item_rz + G.base,
+                                   states, slot, b.stream);
+        if (rc) return rc;
+    }
+    return SIPG_OK;
+}
+
 // One CG iteration of the active instances.  With `repl`, the instances of
 // the replacement set get r = b - A x between the x/r update and the p
-// update (reliable update; the set is empty in most replays).
-int enqueue_iteration(sipg_batch &b, double *X, const double *Bv, int slot, bool repl) {
+// update (reliable update; the set is empty in most replays).  With `pcg`,
+// z = M^-1 r follows (after any replacement), beta comes from r.z, and
+// p = z + beta p.
+int enqueue_iteration(sipg_batch &b, double *X, const double *Bv, int slot, bool repl, bool pcg) {
     const int A = SET_ACTIVE, Q = SET_REPL;
     int rc;
     for (size_t g = 0; g < b.groups.size(); ++g) {
@@ -205,6 +221,14 @@ int enqueue_iteration(sipg_batch &b, double *X, const double *Bv, int slot, bool
                                         b.d_part2, b.d_part3, b.stream)) ||
             (rc = launch_batch_mark(b.d_list[Q], b.count(Q, 0), b.ninst, b.d_state, slot, b.stream)))
             return rc;
+    }
+    if (pcg) {
+        if ((rc = prec_set(b, A, nullptr, b.d_state, slot)) ||
+            (rc = launch_batch_beta_pcg(b.d_list[A], b.count(A, 0), b.ninst, b.d_item_range[A], b.d_item_rr,
+                                        b.d_item_rz, b.d_state, slot, b.stream)))
+            return rc;
+        return launch_batch_update_p(b.d_zq, b.Pv(), b.d_chunks[A], b.count(A, 1), b.chunk_cap, b.d_state, slot,
+                                     b.stream);
     }
     if ((rc = launch_batch_beta(b.d_list[A], b.count(A, 0), b.ninst, b.d_chunk_range[A], b.d_chunk_part,
                                 b.d_chunk_range[Q], b.d_part2, b.d_state, slot, b.stream)))
@@ -240,9 +264,11 @@ int alloc_lists(sipg_batch &b) {
             if ((rc = dmalloc(&G.d_items[s], G.cap))) return rc;
         }
     }
-    if (b.d_item_part) (void)hipFree(b.d_item_part);
-    b.d_item_part = nullptr;
-    if ((rc = dmalloc(&b.d_item_part, b.item_total))) return rc;
+    for (double **q : {&b.d_item_part, &b.d_item_rr, &b.d_item_rz}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+        if ((rc = dmalloc(q, b.item_total))) return rc;
+    }
     // pinned staging: everything one set can hold
     size_t bytes = 0;
     for (auto &G : b.groups) bytes += ((size_t)G.cap * sizeof(batch_item) + 15) & ~size_t(15);
@@ -330,6 +356,230 @@ int tune(sipg_batch &b) {
 
 int ensure_tuned(sipg_batch &b) { return b.tuned ? SIPG_OK : tune(b); }
 
+int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opts, sipg_cg_info *info, bool pcg) {
+    if (!b || !x || !rhs || !opts) return SIPG_ERR_INVALID;
+    if (!(opts->tol >= 0.0) || (opts->tol_mode != SIPG_TOL_ABS && opts->tol_mode != SIPG_TOL_REL))
+        return SIPG_ERR_INVALID;
+    if (!(opts->replace_factor >= 0.0 && opts->replace_factor < 1.0)) return SIPG_ERR_INVALID;
+    if (overlaps(x, rhs, (size_t)b->total() * 8)) return SIPG_ERR_ALIAS;
+    int rc = ensure_tuned(*b);
+    if (rc) return rc;
+    sipg_batch &B = *b;
+    if (pcg) {
+        for (const auto &G : B.groups)
+            for (int i : G.inst)
+                if (!batch_prec_supported(G.p, B.pen[i], G.shape)) return SIPG_ERR_UNSUPPORTED;
+        if (!B.d_zq && (rc = dmalloc(&B.d_zq, (size_t)B.total()))) return rc;
+    }
+    const int ni = B.ninst;
+    double *X = (double *)x;
+    const double *Bv = (const double *)rhs;
+    const bool true_stop = opts->true_residual_stop != 0;
+    std::vector<int> maxit(ni);
+    std::vector<batch_ctl> ctl(ni);
+    for (int i = 0; i < ni; ++i) {
+        const int64_t n = B.off[i + 1] - B.off[i];
+        maxit[i] = (int)std::min<int64_t>(opts->maxit > 0 ? (int64_t)opts->maxit : n, INT_MAX);
+        ctl[i] = batch_ctl{opts->tol, opts->tol_mode, maxit[i]};
+    }
+    SIPG_HIP_TRY(hipMemcpy(B.d_ctl, ctl.data(), ni * sizeof(batch_ctl), hipMemcpyHostToDevice));
+
+    // r = b - A x, p = r, z = 0, initial states
+    SIPG_HIP_TRY(hipMemsetAsync(B.Z(), 0, (size_t)B.total() * sizeof(double), B.stream));
+    if ((rc = apply_set(B, SET_ALL, X, B.AP()))) return rc;
+    if ((rc = launch_batch_residual(Bv, B.AP(), B.R(), B.Pv(), B.d_chunks[SET_ALL], B.count(SET_ALL, 1),
+                                    B.chunk_cap, B.d_part2, B.d_part3, B.stream)))
+        return rc;
+    if (pcg && (rc = prec_set(B, SET_ALL, B.Pv(), nullptr, 0))) return rc;      // z = M^-1 r, p = z
+    if ((rc = launch_batch_init(B.d_list[SET_ALL], B.count(SET_ALL, 0), ni, B.d_chunk_range[SET_ALL], B.d_part2,
+                                B.d_part3, B.d_state, B.d_ctl, pcg ? B.d_item_range[SET_ALL] : nullptr,
+                                pcg ? B.d_item_rz : nullptr, B.stream)))
+        return rc;
+    cg_state *hs0 = B.h_state, *hs1 = B.h_state + ni;
+    SIPG_HIP_TRY(hipMemcpyAsync(hs0, B.d_state, ni * sizeof(cg_state), hipMemcpyDeviceToHost, B.stream));
+    SIPG_HIP_TRY(hipStreamSynchronize(B.stream));
+
+    enum { ACTIVE, FINISHED };
+    std::vector<char> phase(ni, ACTIVE);
+    int n_open = 0;
+    for (int i = 0; i < ni; ++i) {
+        const cg_slot &c = hs0[i].s[0];
+        if (hs0[i].err || c.done) phase[i] = FINISHED;      // nothing to do: r = b - A x was exact
+        else ++n_open;
+    }
+    std::vector<char> in_set(ni);
+    Lists L;
+    auto upload_active = [&]() {
+        for (int i = 0; i < ni; ++i) in_set[i] = phase[i] != FINISHED;
+        build_lists(B, in_set, L);
+        return upload_lists(B, SET_ACTIVE, L);
+    };
+    std::vector<char> none(ni, 0);
+    Lists Q;
+    build_lists(B, none, Q);
+    if ((rc = upload_active()) || (rc = upload_lists(B, SET_REPL, Q))) return rc;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(B.stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    if (n_open > 0) {
+        SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        SIPG_HIP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeRelaxed));
+        for (int it = 0; it < kGraphIters && !rc; ++it) rc = enqueue_iteration(B, X, Bv, it & 1, it == 0, pcg);
+        const hipError_t ec = hipStreamEndCapture(B.stream, &graph);
+        if (rc || ec != hipSuccess) {
+            if (ec != hipSuccess) set_last_hip_error(ec);
+            cleanup();
+            return rc ? rc : SIPG_ERR_HIP;
+        }
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            set_last_hip_error(ei);
+            cleanup();
+            return SIPG_ERR_HIP;
+        }
+    }
+    // One replay in flight ahead; state copy j is processed while replay
+    // j+1 runs, and whatever it decides (replacement set, shorter lists) is
+    // enqueued before replay j+2 and copy j+1.
+    int64_t replays = 0;
+    const int64_t max_replays = (*std::max_element(maxit.begin(), maxit.end()) + kGraphIters) / kGraphIters + 64;
+    int finished_since = 0;
+    int64_t open_work = 0;
+    for (int i = 0; i < ni; ++i)
+        if (phase[i] != FINISHED) open_work += B.off[i + 1] - B.off[i];
+    int64_t done_work = 0;
+    std::vector<int> nrep_req(ni, 0);
+    // final checks: the smallest true r.r of the failed ones, how many failed
+    // checks gained less than 10% on it, and whether one is in flight;
+    // kStallChecks such checks mean the FP64 floor -- the instance ends
+    // there, not converged
+    std::vector<double> check_ref(ni, -1.0);
+    std::vector<int> check_stale(ni, 0);
+    std::vector<char> check_req(ni, 0);
+    bool stopped = false;
+    const double rf2 = opts->replace_factor * opts->replace_factor;
+    auto launch_replay = [&]() -> int {
+        SIPG_HIP_TRY(hipGraphLaunch(exec, B.stream));
+        ++replays;
+        return SIPG_OK;
+    };
+    auto copy_states = [&](int k) -> int {
+        SIPG_HIP_TRY(hipMemcpyAsync(k ? hs1 : hs0, B.d_state, ni * sizeof(cg_state), hipMemcpyDeviceToHost,
+                                    B.stream));
+        SIPG_HIP_TRY(hipEventRecord(ev[k], B.stream));
+        return SIPG_OK;
+    };
+    if (n_open > 0) {
+        int cur = 0;
+        if ((rc = launch_replay()) || (rc = copy_states(0)) || (rc = launch_replay())) {
+            cleanup();
+            return rc;
+        }
+        for (;;) {
+            SIPG_HIP_TRY(hipEventSynchronize(ev[cur]));
+            const cg_state *hs = cur ? hs1 : hs0;
+            std::vector<char> repl(ni, 0);
+            bool any_repl = false;
+            for (int i = 0; i < ni; ++i) {
+                if (phase[i] == FINISHED) continue;
+                const cg_slot &c = hs[i].s[0];
+                if (hs[i].err || c.done) {
+                    phase[i] = FINISHED;
+                    --n_open;
+                    ++finished_since;
+                    done_work += B.off[i + 1] - B.off[i];
+                    continue;
+                }
+                // a replacement is requested once the previous one shows in
+                // the polled state; the final one once the recursion has
+                // fallen 4x below the last true residual (as in sipg_cg_ex)
+                if (hs[i].nrep != nrep_req[i]) continue;
+                if (check_req[i]) {
+                    check_req[i] = 0;
+                    const bool gain = check_ref[i] < 0.0 || hs[i].rr_ref <= 0.81 * check_ref[i];
+                    if (!gain) ++check_stale[i];
+                    if (check_stale[i] >= kStallChecks) {
+                        phase[i] = FINISHED;
+                        stopped = true;
+                        --n_open;
+                        ++finished_since;
+                        done_work += B.off[i + 1] - B.off[i];
+                        continue;
+                    }
+                    if (gain) check_ref[i] = hs[i].rr_ref;
+                }
+                const bool check = true_stop && (c.flags & kSlotWantCheck) && c.rr <= 0.0625 * hs[i].rr_ref;
+                const bool periodic = rf2 > 0.0 && c.rr <= rf2 * hs[i].rr_ref;
+                if (check || periodic) {
+                    repl[i] = 1;
+                    any_repl = true;
+                    ++nrep_req[i];
+                    check_req[i] = check ? 1 : 0;
+                }
+            }
+            // the replacement set of replay j+2 (empty most of the time)
+            if (any_repl || Q.list.size()) {
+                build_lists(B, repl, Q);
+                if ((rc = upload_lists(B, SET_REPL, Q))) break;
+            }
+            // shorter lists once a tenth of the open work has finished (at
+            // once for stopped instances, which are not finished on the device)
+            if (finished_since > 0 && n_open > 0 && (stopped || 10 * done_work >= open_work)) {
+                stopped = false;
+                if ((rc = upload_active())) break;
+                finished_since = 0;
+                open_work -= done_work;
+                done_work = 0;
+            }
+            if ((rc = copy_states(cur ^ 1))) break;
+            if (n_open == 0) break;
+            if (replays >= max_replays) {
+                rc = SIPG_NOT_CONVERGED;
+                break;
+            }
+            if ((rc = launch_replay())) break;
+            cur ^= 1;
+        }
+    }
+    cleanup();
+    if (rc && rc != SIPG_NOT_CONVERGED) return rc;
+    // increments after each instance's last true residual (none unless it
+    // stopped on its iteration cap)
+    int frc = launch_batch_fold(X, B.Z(), B.d_chunks[SET_ALL], B.count(SET_ALL, 1), B.chunk_cap, B.stream);
+    if (frc) return frc;
+    SIPG_HIP_TRY(hipStreamSynchronize(B.stream));
+    SIPG_HIP_TRY(hipMemcpy(hs0, B.d_state, ni * sizeof(cg_state), hipMemcpyDeviceToHost));
+    int status = rc;
+    for (int i = 0; i < ni; ++i) {
+        const cg_state &f = hs0[i];
+        const cg_slot &c = f.s[0];
+        int st = SIPG_OK;
+        if (f.err == 2) st = SIPG_ERR_BREAKDOWN;
+        else if (f.err == 3) st = SIPG_ERR_NONFINITE;
+        else if (!(std::sqrt(c.rr) <= f.stop) || (true_stop && !c.checked)) st = SIPG_NOT_CONVERGED;
+        if (info) {
+            info[i].status = st;
+            info[i].iters = c.it;
+            info[i].replacements = f.nrep;
+            info[i].pad = 0;
+            info[i].res_norm = std::sqrt(c.rr);
+            info[i].true_res_norm = (c.checked || f.nrep > 0) ? std::sqrt(f.rr_ref) : -1.0;
+            info[i].b_norm = std::sqrt(f.bb);
+        }
+        if (st != SIPG_OK && status == SIPG_OK) status = st;
+    }
+    return status;
+}
+
 }  // namespace
 
 extern "C" {
@@ -341,7 +591,7 @@ int sipg_batch_destroy(sipg_batch *b) {
         for (int s = 0; s < NSETS; ++s)
             if (G.d_items[s]) (void)hipFree(G.d_items[s]);
     void *dev[] = {b->d_inst, b->d_state, b->d_ctl, b->ws, b->d_item_part, b->d_chunk_part,
-                   b->d_part2, b->d_part3, b->d_counts, b->d_modes};
+                   b->d_part2, b->d_part3, b->d_counts, b->d_modes, b->d_item_rr, b->d_item_rz, b->d_zq};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     for (int s = 0; s < NSETS; ++s) {
@@ -547,219 +797,11 @@ int sipg_batch_download(sipg_batch *b, int inst, const void *dev, double *host) 
 }
 
 int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opts, sipg_cg_info *info) {
-    if (!b || !x || !rhs || !opts) return SIPG_ERR_INVALID;
-    if (!(opts->tol >= 0.0) || (opts->tol_mode != SIPG_TOL_ABS && opts->tol_mode != SIPG_TOL_REL))
-        return SIPG_ERR_INVALID;
-    if (!(opts->replace_factor >= 0.0 && opts->replace_factor < 1.0)) return SIPG_ERR_INVALID;
-    if (overlaps(x, rhs, (size_t)b->total() * 8)) return SIPG_ERR_ALIAS;
-    int rc = ensure_tuned(*b);
-    if (rc) return rc;
-    sipg_batch &B = *b;
-    const int ni = B.ninst;
-    double *X = (double *)x;
-    const double *Bv = (const double *)rhs;
-    const bool true_stop = opts->true_residual_stop != 0;
-    std::vector<int> maxit(ni);
-    std::vector<batch_ctl> ctl(ni);
-    for (int i = 0; i < ni; ++i) {
-        const int64_t n = B.off[i + 1] - B.off[i];
-        maxit[i] = (int)std::min<int64_t>(opts->maxit > 0 ? (int64_t)opts->maxit : n, INT_MAX);
-        ctl[i] = batch_ctl{opts->tol, opts->tol_mode, maxit[i]};
-    }
-    SIPG_HIP_TRY(hipMemcpy(B.d_ctl, ctl.data(), ni * sizeof(batch_ctl), hipMemcpyHostToDevice));
+    return batch_solve(b, x, rhs, opts, info, false);
+}
 
-    // r = b - A x, p = r, z = 0, initial states
-    SIPG_HIP_TRY(hipMemsetAsync(B.Z(), 0, (size_t)B.total() * sizeof(double), B.stream));
-    if ((rc = apply_set(B, SET_ALL, X, B.AP()))) return rc;
-    if ((rc = launch_batch_residual(Bv, B.AP(), B.R(), B.Pv(), B.d_chunks[SET_ALL], B.count(SET_ALL, 1),
-                                    B.chunk_cap, B.d_part2, B.d_part3, B.stream)))
-        return rc;
-    if ((rc = launch_batch_init(B.d_list[SET_ALL], B.count(SET_ALL, 0), ni, B.d_chunk_range[SET_ALL], B.d_part2,
-                                B.d_part3, B.d_state, B.d_ctl, B.stream)))
-        return rc;
-    cg_state *hs0 = B.h_state, *hs1 = B.h_state + ni;
-    SIPG_HIP_TRY(hipMemcpyAsync(hs0, B.d_state, ni * sizeof(cg_state), hipMemcpyDeviceToHost, B.stream));
-    SIPG_HIP_TRY(hipStreamSynchronize(B.stream));
-
-    enum { ACTIVE, FINISHED };
-    std::vector<char> phase(ni, ACTIVE);
-    int n_open = 0;
-    for (int i = 0; i < ni; ++i) {
-        const cg_slot &c = hs0[i].s[0];
-        if (hs0[i].err || c.done) phase[i] = FINISHED;      // nothing to do: r = b - A x was exact
-        else ++n_open;
-    }
-    std::vector<char> in_set(ni);
-    Lists L;
-    auto upload_active = [&]() {
-        for (int i = 0; i < ni; ++i) in_set[i] = phase[i] != FINISHED;
-        build_lists(B, in_set, L);
-        return upload_lists(B, SET_ACTIVE, L);
-    };
-    std::vector<char> none(ni, 0);
-    Lists Q;
-    build_lists(B, none, Q);
-    if ((rc = upload_active()) || (rc = upload_lists(B, SET_REPL, Q))) return rc;
-
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    auto cleanup = [&]() {
-        (void)hipStreamSynchronize(B.stream);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        for (auto &e : ev)
-            if (e) (void)hipEventDestroy(e);
-    };
-    if (n_open > 0) {
-        SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-        SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-        SIPG_HIP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeRelaxed));
-        for (int it = 0; it < kGraphIters && !rc; ++it) rc = enqueue_iteration(B, X, Bv, it & 1, it == 0);
-        const hipError_t ec = hipStreamEndCapture(B.stream, &graph);
-        if (rc || ec != hipSuccess) {
-            if (ec != hipSuccess) set_last_hip_error(ec);
-            cleanup();
-            return rc ? rc : SIPG_ERR_HIP;
-        }
-        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ei != hipSuccess) {
-            set_last_hip_error(ei);
-            cleanup();
-            return SIPG_ERR_HIP;
-        }
-    }
-    // One replay in flight ahead; state copy j is processed while replay
-    // j+1 runs, and whatever it decides (replacement set, shorter lists) is
-    // enqueued before replay j+2 and copy j+1.
-    int64_t replays = 0;
-    const int64_t max_replays = (*std::max_element(maxit.begin(), maxit.end()) + kGraphIters) / kGraphIters + 64;
-    int finished_since = 0;
-    int64_t open_work = 0;
-    for (int i = 0; i < ni; ++i)
-        if (phase[i] != FINISHED) open_work += B.off[i + 1] - B.off[i];
-    int64_t done_work = 0;
-    std::vector<int> nrep_req(ni, 0);
-    // final checks: the smallest true r.r of the failed ones, how many failed
-    // checks gained less than 10% on it, and whether one is in flight;
-    // kStallChecks such checks mean the FP64 floor -- the instance ends
-    // there, not converged
-    std::vector<double> check_ref(ni, -1.0);
-    std::vector<int> check_stale(ni, 0);
-    std::vector<char> check_req(ni, 0);
-    bool stopped = false;
-    const double rf2 = opts->replace_factor * opts->replace_factor;
-    auto launch_replay = [&]() -> int {
-        SIPG_HIP_TRY(hipGraphLaunch(exec, B.stream));
-        ++replays;
-        return SIPG_OK;
-    };
-    auto copy_states = [&](int k) -> int {
-        SIPG_HIP_TRY(hipMemcpyAsync(k ? hs1 : hs0, B.d_state, ni * sizeof(cg_state), hipMemcpyDeviceToHost,
-                                    B.stream));
-        SIPG_HIP_TRY(hipEventRecord(ev[k], B.stream));
-        return SIPG_OK;
-    };
-    if (n_open > 0) {
-        int cur = 0;
-        if ((rc = launch_replay()) || (rc = copy_states(0)) || (rc = launch_replay())) {
-            cleanup();
-            return rc;
-        }
-        for (;;) {
-            SIPG_HIP_TRY(hipEventSynchronize(ev[cur]));
-            const cg_state *hs = cur ? hs1 : hs0;
-            std::vector<char> repl(ni, 0);
-            bool any_repl = false;
-            for (int i = 0; i < ni; ++i) {
-                if (phase[i] == FINISHED) continue;
-                const cg_slot &c = hs[i].s[0];
-                if (hs[i].err || c.done) {
-                    phase[i] = FINISHED;
-                    --n_open;
-                    ++finished_since;
-                    done_work += B.off[i + 1] - B.off[i];
-                    continue;
-                }
-                // a replacement is requested once the previous one shows in
-                // the polled state; the final one once the recursion has
-                // fallen 4x below the last true residual (as in sipg_cg_ex)
-                if (hs[i].nrep != nrep_req[i]) continue;
-                if (check_req[i]) {
-                    check_req[i] = 0;
-                    const bool gain = check_ref[i] < 0.0 || hs[i].rr_ref <= 0.81 * check_ref[i];
-                    if (!gain) ++check_stale[i];
-                    if (check_stale[i] >= kStallChecks) {
-                        phase[i] = FINISHED;
-                        stopped = true;
-                        --n_open;
-                        ++finished_since;
-                        done_work += B.off[i + 1] - B.off[i];
-                        continue;
-                    }
-                    if (gain) check_ref[i] = hs[i].rr_ref;
-                }
-                const bool check = true_stop && (c.flags & kSlotWantCheck) && c.rr <= 0.0625 * hs[i].rr_ref;
-                const bool periodic = rf2 > 0.0 && c.rr <= rf2 * hs[i].rr_ref;
-                if (check || periodic) {
-                    repl[i] = 1;
-                    any_repl = true;
-                    ++nrep_req[i];
-                    check_req[i] = check ? 1 : 0;
-                }
-            }
-            // the replacement set of replay j+2 (empty most of the time)
-            if (any_repl || Q.list.size()) {
-                build_lists(B, repl, Q);
-                if ((rc = upload_lists(B, SET_REPL, Q))) break;
-            }
-            // shorter lists once a tenth of the open work has finished (at
-            // once for stopped instances, which are not finished on the device)
-            if (finished_since > 0 && n_open > 0 && (stopped || 10 * done_work >= open_work)) {
-                stopped = false;
-                if ((rc = upload_active())) break;
-                finished_since = 0;
-                open_work -= done_work;
-                done_work = 0;
-            }
-            if ((rc = copy_states(cur ^ 1))) break;
-            if (n_open == 0) break;
-            if (replays >= max_replays) {
-                rc = SIPG_NOT_CONVERGED;
-                break;
-            }
-            if ((rc = launch_replay())) break;
-            cur ^= 1;
-        }
-    }
-    cleanup();
-    if (rc && rc != SIPG_NOT_CONVERGED) return rc;
-    // increments after each instance's last true residual (none unless it
-    // stopped on its iteration cap)
-    int frc = launch_batch_fold(X, B.Z(), B.d_chunks[SET_ALL], B.count(SET_ALL, 1), B.chunk_cap, B.stream);
-    if (frc) return frc;
-    SIPG_HIP_TRY(hipStreamSynchronize(B.stream));
-    SIPG_HIP_TRY(hipMemcpy(hs0, B.d_state, ni * sizeof(cg_state), hipMemcpyDeviceToHost));
-    int status = rc;
-    for (int i = 0; i < ni; ++i) {
-        const cg_state &f = hs0[i];
-        const cg_slot &c = f.s[0];
-        int st = SIPG_OK;
-        if (f.err == 2) st = SIPG_ERR_BREAKDOWN;
-        else if (f.err == 3) st = SIPG_ERR_NONFINITE;
-        else if (!(std::sqrt(c.rr) <= f.stop) || (true_stop && !c.checked)) st = SIPG_NOT_CONVERGED;
-        if (info) {
-            info[i].status = st;
-            info[i].iters = c.it;
-            info[i].replacements = f.nrep;
-            info[i].pad = 0;
-            info[i].res_norm = std::sqrt(c.rr);
-            info[i].true_res_norm = (c.checked || f.nrep > 0) ? std::sqrt(f.rr_ref) : -1.0;
-            info[i].b_norm = std::sqrt(f.bb);
-        }
-        if (st != SIPG_OK && status == SIPG_OK) status = st;
-    }
-    return status;
+int sipg_batch_pcg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opts, sipg_cg_info *info) {
+    return batch_solve(b, x, rhs, opts, info, true);
 }
 
 }  // extern "C"

@@ -110,7 +110,7 @@ __device__ __forceinline__ double range_sum(const double *__restrict__ part, int
     return block_reduce(s, red);
 }
 
-// alpha = r.r / p.Ap per active instance (the host-side alpha of
+// alpha = r.z / p.Ap per active instance (r.z = r.r for plain CG; the host-side alpha of
 // conjugate_gradient_gpu.hpp:75-83); breakdown and non-finite values end
 // that instance only.
 __global__ __launch_bounds__(NT) void batch_alpha_kernel(const int *__restrict__ list, const int *__restrict__ count,
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(NT) void batch_alpha_kernel(const int *__restrict__
         const int2 r = item_range[i];
         const double pAp = range_sum(item_part, r.x, r.y, red);
         if (threadIdx.x == 0) {
-            const double alpha = st->s[slot].rr / pAp;
+            const double alpha = st->s[slot].rz / pAp;
             if (!(pAp > 0.0) || !isfinite(alpha)) {
                 st->err = isfinite(pAp) ? 2 : 3;
             } else {
@@ -195,6 +195,7 @@ __global__ __launch_bounds__(NT) void batch_beta_kernel(const int *__restrict__ 
             const bool need_true = st->true_stop && !replaced;
             cg_slot nx{};
             nx.rr = rr_new;
+            nx.rz = rr_new;
             nx.alpha = 0.0;
             nx.beta = rr_new / cur.rr;
             nx.pAp = 0.0;
@@ -288,11 +289,14 @@ __global__ __launch_bounds__(NT) void batch_residual_kernel(const T *__restrict_
 }
 
 // initial state of each listed instance from its r.r and b.b chunk sums
+// (and, for PCG, its r.z item sums; otherwise r.z = r.r)
 __global__ __launch_bounds__(NT) void batch_init_kernel(const int *__restrict__ list, const int *__restrict__ count,
                                                         const int2 *__restrict__ chunk_range,
                                                         const double *__restrict__ part_rr,
                                                         const double *__restrict__ part_bb, cg_state *states,
-                                                        const batch_ctl *__restrict__ ctl) {
+                                                        const batch_ctl *__restrict__ ctl,
+                                                        const int2 *__restrict__ item_range,
+                                                        const double *__restrict__ item_rz) {
     __shared__ double red[NT / 64];
     const int n = *count;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
@@ -300,12 +304,18 @@ __global__ __launch_bounds__(NT) void batch_init_kernel(const int *__restrict__ 
         const int2 r = chunk_range[i];
         const double rr = range_sum(part_rr, r.x, r.y, red);
         const double bb = range_sum(part_bb, r.x, r.y, red);
+        double rz = rr;
+        if (item_rz) {
+            const int2 q = item_range[i];
+            rz = range_sum(item_rz, q.x, q.y, red);
+        }
         if (threadIdx.x == 0) {
             cg_state *st = states + i;
             const batch_ctl c = ctl[i];
             const double stop = c.tol_mode == SIPG_TOL_REL ? c.tol * sqrt(bb) : c.tol;
             cg_slot s0{};
             s0.rr = rr;
+            s0.rz = rz;
             s0.alpha = s0.beta = s0.pAp = 0.0;
             s0.it = 0;
             s0.done = (sqrt(rr) > stop && c.maxit > 0) ? 0 : 1;
@@ -320,7 +330,7 @@ __global__ __launch_bounds__(NT) void batch_init_kernel(const int *__restrict__ 
             st->rr_ref = rr;
             st->maxit = c.maxit;
             st->nrep = 0;
-            st->err = (!isfinite(rr) || !isfinite(bb)) ? 3 : 0;
+            st->err = (!isfinite(rr) || !isfinite(bb) || !isfinite(rz)) ? 3 : 0;
             st->true_stop = 1;
         }
     }
@@ -449,9 +459,9 @@ int launch_batch_residual(const double *B, const double *AX, double *Rv, double 
 
 int launch_batch_init(const int *list, const int *count, int max_inst, const int2 *chunk_range,
                       const double *part_rr, const double *part_bb, cg_state *states, const batch_ctl *ctl,
-                      hipStream_t s) {
+                      const int2 *item_range, const double *item_rz, hipStream_t s) {
     hipLaunchKernelGGL(batch_init_kernel, dim3(inst_blocks(max_inst)), dim3(NT), 0, s, list, count, chunk_range,
-                       part_rr, part_bb, states, ctl);
+                       part_rr, part_bb, states, ctl, item_range, item_rz);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

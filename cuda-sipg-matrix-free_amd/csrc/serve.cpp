@@ -607,7 +607,7 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
                                         S->count(SET_INIT, 1), S->chunk_cap, S->d_part2, S->d_part3, S->stream)) ||
             (rc = launch_batch_init(S->d_list[SET_INIT], S->count(SET_INIT, 0), kMaxSlots,
                                     S->d_chunk_range[SET_INIT], S->d_part2, S->d_part3, S->d_state, S->d_ctl,
-                                    S->stream)))
+                                    nullptr, nullptr, S->stream)))
             return rc;
         for (int64_t id : admitted) S->active.push_back(id);
         lists_changed = true;

@@ -209,9 +209,22 @@ int launch_batch_update_p(const double *Rv, double *Pv, const batch_chunk *chunk
 int launch_batch_residual(const double *B, const double *AX, double *Rv, double *Pv,
                           const batch_chunk *chunks, const int *count, int max_chunks, double *part_rr,
                           double *part_bb, hipStream_t s);
+// (item_range / item_rz: the preconditioner's r.z item sums for PCG, or
+// nullptr for CG, where r.z = r.r)
 int launch_batch_init(const int *list, const int *count, int max_inst, const int2 *chunk_range,
                       const double *part_rr, const double *part_bb, cg_state *states, const batch_ctl *ctl,
-                      hipStream_t s);
+                      const int2 *item_range, const double *item_rz, hipStream_t s);
+// ---- batched PCG (kernels_batch_pcg.hip) ----------------------------------
+// z = M^-1 r over the listed blocks of degree p (shape as the operator's;
+// Pv non-null: also p = z), with per-item r.r and r.z; finished instances
+// skipped when states is non-null.  Requires pen = 5 (p+1)^2.
+bool batch_prec_supported(int p, double pen, int shape);
+int launch_batch_prec(int p, int shape, const double *R, double *Zq, double *Pv, const batch_inst *inst,
+                      const batch_item *items, const int *count, int max_items, double *item_rr, double *item_rz,
+                      const cg_state *states, int slot, hipStream_t s);
+// beta = r.z(new) / r.z(old) per instance from the item sums; next slot
+int launch_batch_beta_pcg(const int *list, const int *count, int max_inst, const int2 *item_range,
+                          const double *item_rr, const double *item_rz, cg_state *states, int slot, hipStream_t s);
 // x += z; z = 0 over the listed chunks (group update)
 int launch_batch_fold(double *X, double *Z, const batch_chunk *chunks, const int *count, int max_chunks,
                       hipStream_t s);

@@ -120,6 +120,7 @@ def _declare(L):
         "sipg_batch_upload": (i, [_vp, i, _dp, _vp]),
         "sipg_batch_download": (i, [_vp, i, _vp, _dp]),
         "sipg_batch_cg": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
+        "sipg_batch_pcg": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
         "sipg_server_create": (i, [ctypes.POINTER(_vp), d, _vp]),
         "sipg_server_destroy": (i, [_vp]),
         "sipg_server_set_shape": (i, [_vp, i, i, i]),
@@ -449,6 +450,12 @@ class Batch:
         o = CgOpts(tol, tol_mode, maxit, replace_factor, 1 if true_stop else 0, 0)
         infos = (CgInfo * self.ninst)()
         st = lib().sipg_batch_cg(self.h, x.ptr, b.ptr, ctypes.byref(o), infos)
+        return st, list(infos)
+
+    def pcg(self, x, b, tol, tol_mode=TOL_REL, maxit=0, true_stop=True, replace_factor=0.0):
+        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1 if true_stop else 0, 0)
+        infos = (CgInfo * self.ninst)()
+        st = lib().sipg_batch_pcg(self.h, x.ptr, b.ptr, ctypes.byref(o), infos)
         return st, list(infos)
 
 

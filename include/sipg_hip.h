@@ -256,6 +256,13 @@ int sipg_batch_download(sipg_batch *b, int inst, const void *dev, double *host);
  * instance converged, else the first failing instance's status. */
 int sipg_batch_cg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opts, sipg_cg_info *info);
 
+/* sipg_batch_cg with the block-Jacobi preconditioner of sipg_pcg for every
+ * instance (preconditioned_conjugate_gradient, conjugate_gradient_gpu.hpp:
+ * 116-213, batched).  Every instance must use the standard penalty
+ * pen = 5 (p+1)^2, whose preconditioner tables are compiled in; otherwise
+ * SIPG_ERR_UNSUPPORTED.  Options and results as sipg_batch_cg. */
+int sipg_batch_pcg(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opts, sipg_cg_info *info);
+
 /* ---- serving: continuous batching of a request stream ------------------- */
 /* A server solves a stream of independent FP64 problems (square meshes).
  * No counterpart in the reference.  Submitted requests wait in arrival

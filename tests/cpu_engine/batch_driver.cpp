@@ -29,7 +29,7 @@ static double relres(int d, int p, double pen, const std::vector<double> &x, con
 }
 
 static void run(const char *name, const std::vector<int> &dim, const std::vector<int> &p, std::vector<double> pen,
-                int maxit) {
+                int maxit, bool pcg = false) {
     const int ni = (int)dim.size();
     if (pen.empty())
         for (int k : p) pen.push_back(5.0 * (k + 1) * (k + 1));
@@ -60,7 +60,7 @@ static void run(const char *name, const std::vector<int> &dim, const std::vector
     }
     sipg_cg_opts o{1e-10, SIPG_TOL_REL, maxit, 0.0, 1, 0};
     std::vector<sipg_cg_info> info(ni);
-    rc = sipg_batch_cg(B, x, b, &o, info.data());
+    rc = pcg ? sipg_batch_pcg(B, x, b, &o, info.data()) : sipg_batch_cg(B, x, b, &o, info.data());
     for (int i = 0; i < ni; ++i) {
         std::vector<double> xh(bh[i].size()), xo(bh[i].size(), 0.0);
         sipg_batch_download(B, i, x, xh.data());
@@ -84,5 +84,7 @@ int main() {
     run("batch", {6, 4, 5, 3, 7, 4, 2}, {3, 2, 4, 5, 2, 13, 16}, {}, 0);
     run("batch_cap", {6, 4, 5}, {3, 2, 4}, {}, 5);
     run("batch_breakdown", {6, 6, 5}, {3, 3, 4}, {80.0, -500.0, 125.0}, 0);
+    run("batch_pcg", {6, 4, 5, 3, 7, 4, 2}, {3, 2, 4, 5, 2, 13, 16}, {}, 0, true);
+    run("batch_pcg_other_pen", {6, 4}, {3, 2}, {81.0, 45.0}, 0, true);
     return 0;
 }

@@ -83,7 +83,7 @@ int launch_batch_alpha(const int *list, const int *count, int, const int2 *item_
             const int2 r = item_range[list[k]];
             double pAp = 0.0;
             for (int i = r.x; i < r.y; ++i) pAp += item_part[i];
-            const double alpha = st->s[slot].rr / pAp;
+            const double alpha = st->s[slot].rz / pAp;
             if (!(pAp > 0.0) || !std::isfinite(alpha)) {
                 st->err = std::isfinite(pAp) ? 2 : 3;
             } else {
@@ -136,6 +136,7 @@ int launch_batch_beta(const int *list, const int *count, int, const int2 *chunk_
             const bool need_true = st->true_stop && !replaced;
             cg_slot nx{};
             nx.rr = rr;
+            nx.rz = rr;
             nx.beta = rr / cur.rr;
             nx.it = cur.it + 1;
             nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
@@ -212,7 +213,8 @@ int launch_batch_residual(const double *B, const double *AX, double *Rv, double 
 }
 
 int launch_batch_init(const int *list, const int *count, int, const int2 *chunk_range, const double *part_rr,
-                      const double *part_bb, cg_state *states, const batch_ctl *ctl, hipStream_t) {
+                      const double *part_bb, cg_state *states, const batch_ctl *ctl, const int2 *item_range,
+                      const double *item_rz, hipStream_t) {
     mock_run([=]() {
         for (int k = 0; k < *count; ++k) {
             const int i = list[k];
@@ -228,6 +230,12 @@ int launch_batch_init(const int *list, const int *count, int, const int2 *chunk_
             const double stop = ctl[i].tol_mode == SIPG_TOL_REL ? tol * std::sqrt(bb) : tol;
             cg_slot s0{};
             s0.rr = rr;
+            s0.rz = rr;
+            if (item_rz) {
+                double rz = 0.0;
+                for (int c = item_range[i].x; c < item_range[i].y; ++c) rz += item_rz[c];
+                s0.rz = rz;
+            }
             s0.done = (std::sqrt(rr) > stop && maxit_i > 0) ? 0 : 1;
             s0.checked = 1;
             st->s[0] = s0;
@@ -241,6 +249,109 @@ int launch_batch_init(const int *list, const int *count, int, const int2 *chunk_
             st->nrep = 0;
             st->err = (!std::isfinite(rr) || !std::isfinite(bb)) ? 3 : 0;
             st->true_stop = 1;
+        }
+    });
+    return SIPG_OK;
+}
+
+// ---- batched PCG (kernels_batch_pcg.hip) ---------------------------------
+bool batch_prec_supported(int p, double pen, int shape) {
+    return batch_shape_supported(p, shape) && pen == 5.0 * (p + 1) * (p + 1);
+}
+
+// z = M^-1 r on each element of every listed block, with the op tables of
+// build_block_jacobi; per-item r.r and r.z over the block's elements
+int launch_batch_prec(int p, int shape, const double *R, double *Zq, double *Pv, const batch_inst *inst,
+                      const batch_item *items, const int *count, int max_items, double *item_rr, double *item_rz,
+                      const cg_state *states, int slot, hipStream_t) {
+    if (!batch_shape_supported(p, shape)) return SIPG_ERR_UNSUPPORTED;
+    std::vector<double> tab;
+    const double pen = 5.0 * (p + 1) * (p + 1);
+    if (build_block_jacobi(p, pen, tab)) return SIPG_ERR_INVALID;
+    mock_run([=]() {
+        const int n = *count;
+        if (n > max_items) std::abort();
+        const int BX = kBatchShapeBX[shape], BY = kBatchShapeBY[shape];
+        const int n1 = p + 1, nq = n1 * n1;
+        const double *S = tab.data(), *L = tab.data() + nq;
+        std::vector<double> t1(nq), t2(nq);
+        for (int k = 0; k < n; ++k) {
+            const batch_item it = items[k];
+            if (states && finished(states + it.inst, slot)) continue;
+            const batch_inst in = inst[it.inst];
+            double srr = 0, srz = 0;
+            for (int ey = it.y0; ey < it.y0 + BY && ey < in.dimy; ++ey)
+                for (int ex = it.x0; ex < it.x0 + BX && ex < in.dimx; ++ex) {
+                    const int64_t b = in.off + ((int64_t)ey * in.dimx + ex) * nq;
+                    const double *Re = R + b;
+                    for (int a = 0; a < n1; ++a)
+                        for (int j = 0; j < n1; ++j) {
+                            double acc = 0;
+                            for (int i = 0; i < n1; ++i) acc += S[i * n1 + a] * Re[i * n1 + j];
+                            t1[a * n1 + j] = acc;
+                        }
+                    for (int a = 0; a < n1; ++a)
+                        for (int c = 0; c < n1; ++c) {
+                            double acc = 0;
+                            for (int bb = 0; bb < n1; ++bb) acc += t1[a * n1 + bb] * S[bb * n1 + c];
+                            t2[a * n1 + c] = acc * L[a * n1 + c];
+                        }
+                    for (int a = 0; a < n1; ++a)
+                        for (int d = 0; d < n1; ++d) {
+                            double acc = 0;
+                            for (int c = 0; c < n1; ++c) acc += t2[a * n1 + c] * S[d * n1 + c];
+                            t1[a * n1 + d] = acc;
+                        }
+                    for (int a = 0; a < n1; ++a)
+                        for (int j = 0; j < n1; ++j) {
+                            double acc = 0;
+                            for (int i = 0; i < n1; ++i) acc += S[a * n1 + i] * t1[i * n1 + j];
+                            Zq[b + a * n1 + j] = acc;
+                            if (Pv) Pv[b + a * n1 + j] = acc;
+                        }
+                    for (int q = 0; q < nq; ++q) {
+                        srr += Re[q] * Re[q];
+                        srz += Re[q] * Zq[b + q];
+                    }
+                }
+            item_rr[k] = srr;
+            item_rz[k] = srz;
+        }
+    });
+    return SIPG_OK;
+}
+
+int launch_batch_beta_pcg(const int *list, const int *count, int, const int2 *item_range, const double *item_rr,
+                          const double *item_rz, cg_state *states, int slot, hipStream_t) {
+    mock_run([=]() {
+        for (int k = 0; k < *count; ++k) {
+            cg_state *st = states + list[k];
+            const cg_slot cur = st->s[slot];
+            if (cur.done | st->err) {
+                st->s[slot ^ 1] = cur;
+                continue;
+            }
+            const int2 r = item_range[list[k]];
+            double rr = 0.0, rz = 0.0;
+            for (int i = r.x; i < r.y; ++i) {
+                rr += item_rr[i];
+                rz += item_rz[i];
+            }
+            const bool replaced = (cur.flags & kSlotReplaced) != 0;
+            const bool met = !(std::sqrt(rr) > st->stop);
+            const bool need_true = st->true_stop && !replaced;
+            cg_slot nx{};
+            nx.rr = rr;
+            nx.rz = rz;
+            nx.beta = rz / cur.rz;
+            nx.it = cur.it + 1;
+            nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
+            nx.checked = replaced ? 1 : 0;
+            nx.flags = (met && need_true) ? kSlotWantCheck : 0;
+            st->s[slot ^ 1] = nx;
+            if (replaced) st->rr_ref = rr;
+            if (!std::isfinite(rr) || !std::isfinite(rz)) st->err = 3;
+            else if (!(rz > 0.0) && rr > 0.0) st->err = 2;
         }
     });
     return SIPG_OK;

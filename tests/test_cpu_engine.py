@@ -131,6 +131,21 @@ def test_batch_engine_breakdown_is_per_instance(batch_results):
     assert rows[0]["relres"] <= 1e-10 and rows[2]["relres"] <= 1e-10
 
 
+def test_batch_pcg_solves_every_instance_in_fewer_iterations(batch_results):
+    # batched block-Jacobi PCG: same solutions as the oracle's CG, fewer
+    # iterations for every instance but the smallest
+    for r in batch_results["batch_pcg"]:
+        assert r["rc"] == 0 and r["status"] == 0 and r["relres"] <= 1e-10, r
+        assert r["diff"] <= 1e-8 and r["iters"] <= r["oracle_iters"] + 16, r
+    its = sum(r["iters"] for r in batch_results["batch_pcg"])
+    ito = sum(r["oracle_iters"] for r in batch_results["batch_pcg"])
+    assert its < 0.8 * ito
+
+
+def test_batch_pcg_needs_the_standard_penalty(batch_results):
+    assert all(r["rc"] == 3 for r in batch_results["batch_pcg_other_pen"])
+
+
 @pytest.fixture(scope="module")
 def serve_results(results):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="halt_on_error=1")

@@ -162,3 +162,40 @@ def test_batch_breakdown_stays_in_its_instance(sipg):
     assert infos[1].status == sipg.ERR_BREAKDOWN
     assert infos[0].status == sipg.OK and infos[2].status == sipg.OK
     del meshes
+
+
+@pytest.mark.parametrize("shape", [None, (16, 4), (8, 4), (8, 2)])
+def test_batch_pcg_matches_single_pcg(sipg, shape):
+    # batched block-Jacobi PCG (sipg_batch_pcg) against sipg_pcg of each
+    # instance alone: same tolerance on the true residual, a similar
+    # iteration count, the same solution
+    meshes = [(d, d, p) for d, p in SQUARE]
+    B, b = _fourier_batch(sipg, meshes)
+    if shape is not None:
+        for p in sorted({m[2] for m in meshes}):
+            assert B.set_shape(p, *shape) == sipg.OK
+    x = B.vector()
+    x.zero()
+    st, infos = B.pcg(x, b, 1e-10, sipg.TOL_REL, maxit=5000, replace_factor=3e-3)
+    assert st == sipg.OK, [(k.status, k.iters) for k in infos]
+    for i, (d, _, p) in enumerate(meshes):
+        op = sipg.Operator(d, d, p, sipg.pen_sigma(p), sipg.F64)
+        xi = op.to_device(B.download(i, x))
+        bi = op.to_device(B.download(i, b))
+        assert op.true_relres(xi, bi) <= 1e-10, i
+        xs = op.vector()
+        xs.zero()
+        st1, info1 = op.pcg(xs, bi, 1e-10, sipg.TOL_REL, replace_factor=3e-3, true_stop=True)
+        assert st1 == sipg.OK
+        assert info1.iters - 16 <= infos[i].iters <= 1.25 * info1.iters + 64, (i, infos[i].iters, info1.iters)
+        ref, got = op.download(xs), B.download(i, x)
+        assert np.linalg.norm(got - ref) <= 1e-7 * np.linalg.norm(ref), i
+
+
+def test_batch_pcg_needs_the_standard_penalty(sipg):
+    B = sipg.Batch([6, 6], [6, 6], [3, 3], [sipg.pen_sigma(3), 81.0])
+    b, x = B.vector(), B.vector()
+    b.zero()
+    x.zero()
+    st, _ = B.pcg(x, b, 1e-10, sipg.TOL_REL)
+    assert st == 3          # SIPG_ERR_UNSUPPORTED
