@@ -124,8 +124,9 @@ def batched_leg(sipg, log_every=True):
     """2048 independent FP64 solves in one batch: mesh side ~ U{4..64},
     degree ~ U{2..10} (numpy default_rng(4)), instance i forced by
     forcing_draw(i, 1..4 modes, m,n <= 4), pen = 5 (p+1)^2, 1e-10 true
-    relative residual each.  Compared with solving a sample of the same
-    instances one after another with the single-problem solver."""
+    relative residual each, by batched CG and by batched block-Jacobi PCG.
+    Each is compared with solving a sample of the same instances (every
+    64th) one after another with the single-problem solver of that method."""
     N = 2048
     rng = np.random.default_rng(4)
     dims = rng.integers(4, 65, N)
@@ -138,42 +139,54 @@ def batched_leg(sipg, log_every=True):
     B.tune()
     shapes = {int(p): B.tuning(int(p))[:2] for p in sorted(set(ps.tolist()))}
     x = B.vector()
-    times = []
-    for _ in range(2):                      # the first solve is the warm-up
-        x.zero()
-        sipg.synchronize()
-        t0 = time.perf_counter()
-        st, infos = B.cg(x, b, TOL, sipg.TOL_REL, maxit=20000, replace_factor=3e-3)
-        times.append(time.perf_counter() - t0)
-    t = times[-1]
-    its = np.array([k.iters for k in infos])
-    rel = max(k.true_res_norm / k.b_norm for k in infos)
-    failed = [(int(dims[i]), int(ps[i]), int(infos[i].iters), sipg.strerror(infos[i].status))
-              for i in range(N) if infos[i].status != sipg.OK]
-    # looping the single-problem solver over every 64th instance
-    tl, dof = 0.0, 0
-    for i in range(0, N, 64):
-        d, p = int(dims[i]), int(ps[i])
-        op = sipg.Operator(d, d, p, pens[i], sipg.F64)
-        bi = op.vector()
-        op.rhs_fourier(sipg.forcing_draw(i, 1, 4, 4), bi)
-        xi = op.vector()
-        xi.zero()
-        sipg.synchronize()
-        t0 = time.perf_counter()
-        sti, _ = op.cg_ex(xi, bi, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=3e-3, true_stop=True)
-        tl += time.perf_counter() - t0
-        dof += op.n
-        if sti != sipg.OK:
-            raise RuntimeError(f"single solve {i} failed: {sipg.strerror(sti)}")
-    out = {"instances": N, "dof": int(B.n), "ms_per_batch_solve": 1e3 * t, "dof_per_s": B.n / t,
-           "status": sipg.strerror(st), "failed_instances": failed[:20], "n_failed": len(failed),
-           "iterations": {"min": int(its.min()), "median": int(np.median(its)), "max": int(its.max())},
-           "max_true_relres": rel, "block_shape_per_degree": shapes,
-           "loop_single_solver": {"instances": len(range(0, N, 64)), "dof_per_s": dof / tl},
-           "speedup_vs_loop": (B.n / t) / (dof / tl)}
-    log(f"batched: {N} solves, {B.n} DoF in {1e3 * t:.0f} ms = {B.n / t:.3e} DoF/s "
-        f"(loop of single solves {dof / tl:.3e} DoF/s), max true relres {rel:.2e}")
+
+    def one(method):
+        solve = B.cg if method == "cg" else B.pcg
+        times = []
+        for _ in range(2):                      # the first solve is the warm-up
+            x.zero()
+            sipg.synchronize()
+            t0 = time.perf_counter()
+            st, infos = solve(x, b, TOL, sipg.TOL_REL, maxit=20000, replace_factor=3e-3)
+            times.append(time.perf_counter() - t0)
+        t = times[-1]
+        its = np.array([k.iters for k in infos])
+        rel = max(k.true_res_norm / k.b_norm for k in infos)
+        failed = [(int(dims[i]), int(ps[i]), int(infos[i].iters), sipg.strerror(infos[i].status))
+                  for i in range(N) if infos[i].status != sipg.OK]
+        tl, dof = 0.0, 0
+        for i in range(0, N, 64):
+            d, p = int(dims[i]), int(ps[i])
+            op = sipg.Operator(d, d, p, pens[i], sipg.F64)
+            bi = op.vector()
+            op.rhs_fourier(sipg.forcing_draw(i, 1, 4, 4), bi)
+            xi = op.vector()
+            xi.zero()
+            sipg.synchronize()
+            t0 = time.perf_counter()
+            if method == "cg":
+                sti, _ = op.cg_ex(xi, bi, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=3e-3,
+                                  true_stop=True)
+            else:
+                sti, _ = op.pcg(xi, bi, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=3e-3,
+                                true_stop=True)
+            tl += time.perf_counter() - t0
+            dof += op.n
+            if sti != sipg.OK:
+                raise RuntimeError(f"single {method} solve {i} failed: {sipg.strerror(sti)}")
+        log(f"batched {method}: {N} solves, {B.n} DoF in {1e3 * t:.0f} ms = {B.n / t:.3e} DoF/s "
+            f"(loop of single solves {dof / tl:.3e} DoF/s), max true relres {rel:.2e}")
+        return {"ms_per_batch_solve": 1e3 * t, "dof_per_s": B.n / t,
+                "status": sipg.strerror(st), "failed_instances": failed[:20], "n_failed": len(failed),
+                "iterations": {"min": int(its.min()), "median": int(np.median(its)), "max": int(its.max())},
+                "max_true_relres": rel,
+                "loop_single_solver": {"instances": len(range(0, N, 64)), "dof_per_s": dof / tl},
+                "speedup_vs_loop": (B.n / t) / (dof / tl)}
+
+    out = {"instances": N, "dof": int(B.n), "block_shape_per_degree": shapes}
+    out.update(one("cg"))
+    out["pcg"] = one("pcg")
+    out["pcg_speedup_vs_cg"] = out["pcg"]["dof_per_s"] / out["dof_per_s"]
     return out
 
 

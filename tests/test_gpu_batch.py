@@ -185,9 +185,11 @@ def test_batch_pcg_matches_single_pcg(sipg, shape):
         assert op.true_relres(xi, bi) <= 1e-10, i
         xs = op.vector()
         xs.zero()
-        st1, info1 = op.pcg(xs, bi, 1e-10, sipg.TOL_REL, replace_factor=3e-3, true_stop=True)
+        # the recursion's own stop: the device ends it at the first
+        # iteration below the tolerance, so the count is exact
+        st1, info1 = op.pcg(xs, bi, 1e-10, sipg.TOL_REL, true_stop=False)
         assert st1 == sipg.OK
-        assert info1.iters - 16 <= infos[i].iters <= 1.25 * info1.iters + 64, (i, infos[i].iters, info1.iters)
+        assert info1.iters - 3 <= infos[i].iters <= 1.25 * info1.iters + 64, (i, infos[i].iters, info1.iters)
         ref, got = op.download(xs), B.download(i, x)
         assert np.linalg.norm(got - ref) <= 1e-7 * np.linalg.norm(ref), i
 
