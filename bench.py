@@ -190,12 +190,13 @@ def batched_leg(sipg, log_every=True):
     return out
 
 
-def serving_leg(sipg, seconds, rate):
+def serving_leg(sipg, seconds, rate, method="pcg"):
     """Poisson arrivals (rate per second, numpy default_rng(0x5EED)) for
     `seconds`: mesh side log-uniform in 8..256, degree ~ U{2..12}, 70 %
     marked mixed precision, tolerance 1e-8 or 1e-10, request k forced by
     forcing_draw(k, 1..4 modes).  Served by one sipg_server with an 80 GB
-    HBM budget; latency = completion - scheduled arrival."""
+    HBM budget, by CG or block-Jacobi PCG; latency = completion - scheduled
+    arrival."""
     rng = np.random.default_rng(0x5EED)
     gaps = rng.exponential(1.0 / rate, int(rate * seconds * 2) + 16)
     arrive = np.cumsum(gaps)
@@ -210,6 +211,8 @@ def serving_leg(sipg, seconds, rate):
     Bt = sipg.Batch(dims[sample], dims[sample], ps[sample], [sipg.pen_sigma(int(p)) for p in ps[sample]])
     Bt.tune()
     S = sipg.Server(80e9)
+    if S.set_method(sipg.METHOD_PCG if method == "pcg" else sipg.METHOD_CG) != sipg.OK:
+        raise RuntimeError(f"server method {method} refused")
     for p in sorted(set(ps[sample].tolist())):
         bx, by, _ = Bt.tuning(int(p))
         S.set_shape(int(p), bx, by)
@@ -245,7 +248,7 @@ def serving_leg(sipg, seconds, rate):
             lat[k] = time.perf_counter() - t0 - arrive[k]
         if time.perf_counter() - last_log > 5.0:
             last_log = time.perf_counter()
-            log(f"serving: {len(done)}/{K} done, {st['running']} running, {st['queued']} queued")
+            log(f"serving {method}: {len(done)}/{K} done, {st['running']} running, {st['queued']} queued")
     wall = time.perf_counter() - t0
     statuses = [done[k].status for k in range(K)]
     rel_ok = all(0.0 <= done[k].true_res_norm <= tols[k] * done[k].b_norm * 1.0000001 for k in range(K))
@@ -254,7 +257,8 @@ def serving_leg(sipg, seconds, rate):
                "true_relres": done[k].true_res_norm / done[k].b_norm if done[k].true_res_norm >= 0 else None}
               for k in range(K) if done[k].status != sipg.OK]
     dof = int(sum(int(dims[k]) ** 2 * (int(ps[k]) + 1) ** 2 for k in range(K)))
-    out = {"arrival_rate_per_s": rate, "arrival_window_s": seconds, "requests": K,
+    out = {"method": "FP64 block-Jacobi PCG" if method == "pcg" else "FP64 CG",
+           "arrival_rate_per_s": rate, "arrival_window_s": seconds, "requests": K,
            "solves_per_s": K / wall, "dof_per_s": dof / wall, "wall_s": wall,
            "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                          "max": float(lat.max())},
@@ -264,7 +268,7 @@ def serving_leg(sipg, seconds, rate):
            "mixed_marked": int(mixed.sum()),
            "note": ("requests marked mixed precision are served by the FP64 path: FP32 inner CG "
                     "needed 5-6x the FP64 time on these operators (eps32 * kappa ~ 1)")}
-    log(f"serving: {K} requests in {wall:.1f} s, {K / wall:.1f} solves/s, p50 {out['latency_s']['p50']:.3f} s, "
+    log(f"serving {method}: {K} requests in {wall:.1f} s, {K / wall:.1f} solves/s, p50 {out['latency_s']['p50']:.3f} s, "
         f"p99 {out['latency_s']['p99']:.3f} s")
     return out
 
@@ -433,7 +437,10 @@ def main():
     log(f"timed ({solver}): {ms_step:.1f} ms/solve, {value:.3e} DoF/s")
 
     batched = None if args.no_batch else batched_leg(sipg)
-    serving = serving_leg(sipg, args.serve_seconds, args.serve_rate) if args.serve_seconds > 0 else None
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serving_leg(sipg, args.serve_seconds, args.serve_rate, "pcg")
+        serving["cg"] = serving_leg(sipg, args.serve_seconds, args.serve_rate, "cg")
 
     traffic, traffic_note = (None, "not measured (--no-traffic)")
     if not args.no_traffic:

@@ -460,7 +460,7 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
     std::vector<int> nrep_req(ni, 0);
     // final checks: the smallest true r.r of the failed ones, how many failed
     // checks gained less than 10% on it, and whether one is in flight;
-    // kStallChecks such checks mean the FP64 floor -- the instance ends
+    // kStallChecks (PCG: kStallChecksPcg) such checks mean the FP64 floor -- the instance ends
     // there, not converged
     std::vector<double> check_ref(ni, -1.0);
     std::vector<int> check_stale(ni, 0);
@@ -507,7 +507,7 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
                     check_req[i] = 0;
                     const bool gain = check_ref[i] < 0.0 || hs[i].rr_ref <= 0.81 * check_ref[i];
                     if (!gain) ++check_stale[i];
-                    if (check_stale[i] >= kStallChecks) {
+                    if (check_stale[i] >= (pcg ? kStallChecksPcg : kStallChecks)) {
                         phase[i] = FINISHED;
                         stopped = true;
                         --n_open;

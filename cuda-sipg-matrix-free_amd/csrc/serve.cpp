@@ -13,14 +13,16 @@
 //   5. replays kServeIters CG iterations of every running request from a
 //      captured hipGraph (the batched kernels of kernels_batch.hip over
 //      device-resident work lists), and waits for it.
+// Under SIPG_METHOD_PCG the iterations are block-Jacobi PCG ones (the
+// batched preconditioner of kernels_batch_pcg.hip after each x/r update).
 // A request's arithmetic does not depend on the other requests in flight:
 // its reductions run over its own blocks and chunks in a fixed order, so a
 // result is the same alone and in any mix.
 //
 // Memory: six pools (x, b, r, p, Ap, and z, the increments since the last
-// true residual) share one offset space; a request of n unknowns reserves
-// the same range of n elements in each, 48 n bytes in all.  The budget
-// bounds the six pools together.
+// true residual), and under PCG a seventh (M^-1 r), share one offset space;
+// a request of n unknowns reserves the same range of n elements in each,
+// 48 n (56 n) bytes in all.  The budget bounds the pools together.
 #include "sipg_internal.hpp"
 
 #include <algorithm>
@@ -38,7 +40,7 @@ namespace {
 constexpr int kServeIters = 8;       // iterations per replay (even)
 constexpr int64_t kChunk = 8192;
 constexpr int kMaxSlots = 4096;      // requests on the device at once
-constexpr int kPools = 6;            // x, b, r, p, Ap, z
+constexpr int kPools = 7;            // x, b, r, p, Ap, z (increments), zq (M^-1 r, PCG)
 enum { SET_ACTIVE = 0, SET_INIT = 1, SET_REPL = 2, NSETS = 3 };
 enum { QUEUED, RUNNING, DONE };
 constexpr double kServeReplace = 3e-3;  // true residual at each 300x drop
@@ -72,6 +74,9 @@ struct SlotRec {                     // per-slot data uploaded at admission
 struct sipg_server {
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    double budget = 0;                       // bytes, all pools
+    int method = SIPG_METHOD_CG;
+    int npools = 6;                          // 7 under PCG
     int64_t cap = 0;                         // elements per pool
     double *pool[kPools] = {};
     std::map<int64_t, int64_t> freel;        // offset -> length of free ranges
@@ -88,14 +93,17 @@ struct sipg_server {
     int shape[kMaxDegree + 1];
     int item_cap[NSETS][kMaxDegree + 1] = {};
     batch_item *d_items[NSETS][kMaxDegree + 1] = {};
-    int item_base[kMaxDegree + 1] = {};      // item_part offsets (active set)
-    int item_total = 0;
+    // per-item partial sums (active set: p.Ap, and r.r and r.z under PCG;
+    // admission set: r.r and r.z under PCG), offsets per degree
+    int item_base[NSETS][kMaxDegree + 1] = {};
+    int item_total[NSETS] = {};
     int chunk_cap = 0;
     batch_chunk *d_chunks[NSETS] = {};
     int *d_list[NSETS] = {};
     int2 *d_item_range[NSETS] = {}, *d_chunk_range[NSETS] = {};
     int *d_counts = nullptr;                 // NSETS x (2 + kMaxDegree + 1)
     double *d_item_part = nullptr, *d_chunk_part = nullptr, *d_part2 = nullptr, *d_part3 = nullptr;
+    double *d_item_rr[NSETS] = {}, *d_item_rz[NSETS] = {};
     char *h_stage[NSETS] = {};
     size_t stage_bytes[NSETS] = {};
     hipEvent_t stage_ev[NSETS] = {};
@@ -172,19 +180,23 @@ int ensure_items(sipg_server &S, int set, int p, int n) {
     int rc = dmalloc(&S.d_items[set][p], c);
     if (rc) return rc;
     S.item_cap[set][p] = c;
-    if (set == SET_REPL) S.graph_stale = true;     // the graph holds the buffer
-    if (set == SET_ACTIVE) {
-        // item_part offsets follow the active capacities
-        S.item_total = 0;
-        for (int q = 1; q <= kMaxDegree; ++q) {
-            S.item_base[q] = S.item_total;
-            S.item_total += S.item_cap[SET_ACTIVE][q];
-        }
-        if (S.d_item_part) (void)hipFree(S.d_item_part);
-        S.d_item_part = nullptr;
-        if ((rc = dmalloc(&S.d_item_part, S.item_total))) return rc;
-        S.graph_stale = true;
+    if (set == SET_REPL) {
+        S.graph_stale = true;                       // the graph holds the buffer
+        return SIPG_OK;
     }
+    // partial-sum offsets follow the set's capacities
+    S.item_total[set] = 0;
+    for (int q = 1; q <= kMaxDegree; ++q) {
+        S.item_base[set][q] = S.item_total[set];
+        S.item_total[set] += S.item_cap[set][q];
+    }
+    double **bufs[] = {&S.d_item_rr[set], &S.d_item_rz[set], &S.d_item_part};
+    for (int k = 0; k < (set == SET_ACTIVE ? 3 : 2); ++k) {
+        if (*bufs[k]) (void)hipFree(*bufs[k]);
+        *bufs[k] = nullptr;
+        if ((rc = dmalloc(bufs[k], S.item_total[set]))) return rc;
+    }
+    if (set == SET_ACTIVE) S.graph_stale = true;
     return SIPG_OK;
 }
 
@@ -231,11 +243,11 @@ int upload_set(sipg_server &S, int set, const std::vector<int64_t> &ids,
     int rc;
     for (int p = 1; p <= kMaxDegree; ++p)
         if ((rc = ensure_items(S, set, p, (int)items[p].size()))) return rc;
-    if (set == SET_ACTIVE)
+    if (set != SET_REPL)
         for (int64_t id : ids) {
             const Request &R = S.reqs.at(id);
-            item_range[R.slot].x += S.item_base[R.p];
-            item_range[R.slot].y += S.item_base[R.p];
+            item_range[R.slot].x += S.item_base[set][R.p];
+            item_range[R.slot].y += S.item_base[set][R.p];
         }
     if ((int)chunks.size() > S.chunk_cap) return SIPG_ERR_BUDGET;
     size_t bytes = 64 * (kMaxDegree + 8);
@@ -298,6 +310,19 @@ int apply_set(sipg_server &S, int set, const std::vector<int> &degrees, const do
     return SIPG_OK;
 }
 
+// z = M^-1 r over the items of a set (PCG; pv non-null: also p = z), with
+// per-item r.r and r.z; states non-null: finished requests are skipped
+int prec_set(sipg_server &S, int set, const std::vector<int> &degrees, double *pv, const cg_state *states,
+             int slot) {
+    for (int p : degrees) {
+        int rc = launch_batch_prec(p, S.shape[p], S.pool[2], S.pool[6], pv, S.d_inst, S.d_items[set][p],
+                                   S.count(set, 2 + p), S.item_cap[set][p], S.d_item_rr[set] + S.item_base[set][p],
+                                   S.d_item_rz[set] + S.item_base[set][p], states, slot, S.stream);
+        if (rc) return rc;
+    }
+    return SIPG_OK;
+}
+
 std::vector<int> degrees_of(const sipg_server &S, const std::vector<int64_t> &ids) {
     std::vector<char> has(kMaxDegree + 1, 0);
     for (int64_t id : ids) has[S.reqs.at(id).p] = 1;
@@ -320,7 +345,7 @@ int capture_graph(sipg_server &S, const std::vector<int> &degrees) {
         const int slot = it & 1;
         for (int p : degrees) {
             if ((rc = launch_batch_apply(p, S.shape[p], Pv, AP, S.d_inst, S.d_items[A][p], S.count(A, 2 + p),
-                                         S.item_cap[A][p], S.d_item_part + S.item_base[p], S.d_state, slot,
+                                         S.item_cap[A][p], S.d_item_part + S.item_base[A][p], S.d_state, slot,
                                          S.stream)))
                 break;
         }
@@ -339,6 +364,16 @@ int capture_graph(sipg_server &S, const std::vector<int> &degrees) {
                                             S.d_part2, S.d_part3, S.stream)) ||
                 (rc = launch_batch_mark(S.d_list[Q], S.count(Q, 0), kMaxSlots, S.d_state, slot, S.stream)))
                 break;
+        }
+        if (S.method == SIPG_METHOD_PCG) {
+            // z = M^-1 r (after any replacement), beta from r.z, p = z + beta p
+            if ((rc = prec_set(S, A, degrees, nullptr, S.d_state, slot)) ||
+                (rc = launch_batch_beta_pcg(S.d_list[A], S.count(A, 0), kMaxSlots, S.d_item_range[A],
+                                            S.d_item_rr[A], S.d_item_rz[A], S.d_state, slot, S.stream)) ||
+                (rc = launch_batch_update_p(S.pool[6], Pv, S.d_chunks[A], S.count(A, 1), S.chunk_cap, S.d_state,
+                                            slot, S.stream)))
+                break;
+            continue;
         }
         if ((rc = launch_batch_beta(S.d_list[A], S.count(A, 0), kMaxSlots, S.d_chunk_range[A], S.d_chunk_part,
                                     S.d_chunk_range[SET_REPL], S.d_part2, S.d_state, slot, S.stream)) ||
@@ -398,6 +433,8 @@ int sipg_server_destroy(sipg_server *S) {
     for (void *q : dev)
         if (q) (void)hipFree(q);
     for (int s = 0; s < NSETS; ++s) {
+        if (S->d_item_rr[s]) (void)hipFree(S->d_item_rr[s]);
+        if (S->d_item_rz[s]) (void)hipFree(S->d_item_rz[s]);
         for (int p = 0; p <= kMaxDegree; ++p)
             if (S->d_items[s][p]) (void)hipFree(S->d_items[s][p]);
         void *q[] = {S->d_chunks[s], S->d_list[s], S->d_item_range[s], S->d_chunk_range[s]};
@@ -429,12 +466,13 @@ int sipg_server_create(sipg_server **out, double hbm_budget_bytes, void *stream)
         }
         S->own_stream = true;
     }
-    S->cap = (int64_t)(hbm_budget_bytes / (kPools * sizeof(double)));
+    S->budget = hbm_budget_bytes;
+    S->cap = (int64_t)(hbm_budget_bytes / (S->npools * sizeof(double)));
     S->freel[0] = S->cap;
     for (int p = 0; p <= kMaxDegree; ++p) S->shape[p] = 1;   // 8x4 until set otherwise
     int rc;
-    for (auto &pl : S->pool)
-        if ((rc = dmalloc(&pl, S->cap))) return fail(rc);
+    for (int k = 0; k < S->npools; ++k)
+        if ((rc = dmalloc(&S->pool[k], S->cap))) return fail(rc);
     S->chunk_cap = (int)std::min<int64_t>(S->cap / kChunk + kMaxSlots, INT_MAX / 2);
     if ((rc = dmalloc(&S->d_inst, kMaxSlots)) || (rc = dmalloc(&S->d_state, kMaxSlots)) ||
         (rc = dmalloc(&S->d_ctl, kMaxSlots)) ||
@@ -474,6 +512,8 @@ int sipg_server_set_shape(sipg_server *S, int p, int bx, int by) {
     for (int k = 0; k < kBatchShapes; ++k) {
         if (kBatchShapeBX[k] != bx || kBatchShapeBY[k] != by) continue;
         if (!batch_shape_supported(p, k)) return SIPG_ERR_UNSUPPORTED;
+        if (S->method == SIPG_METHOD_PCG && !batch_prec_supported(p, 5.0 * (p + 1) * (p + 1), k))
+            return SIPG_ERR_UNSUPPORTED;
         for (auto &kv : S->reqs)
             if (kv.second.p == p && kv.second.phase == RUNNING)
                 return SIPG_ERR_INVALID;       // only while no request of degree p runs
@@ -484,6 +524,43 @@ int sipg_server_set_shape(sipg_server *S, int p, int bx, int by) {
     return SIPG_ERR_UNSUPPORTED;
 }
 
+int sipg_server_set_method(sipg_server *S, int method) {
+    if (!S || (method != SIPG_METHOD_CG && method != SIPG_METHOD_PCG) || !S->reqs.empty()) return SIPG_ERR_INVALID;
+    const int np = method == SIPG_METHOD_PCG ? 7 : 6;
+    if (method == SIPG_METHOD_PCG)
+        for (int p = 1; p <= kMaxDegree; ++p)
+            if (!batch_prec_supported(p, 5.0 * (p + 1) * (p + 1), S->shape[p])) return SIPG_ERR_UNSUPPORTED;
+    if (np != S->npools) {
+        // the same budget over a different number of pools
+        SIPG_HIP_TRY(hipStreamSynchronize(S->stream));
+        for (double *&pl : S->pool) {
+            if (pl) (void)hipFree(pl);
+            pl = nullptr;
+        }
+        S->npools = np;
+        S->cap = (int64_t)(S->budget / (np * sizeof(double)));
+        S->freel.clear();
+        S->freel[0] = S->cap;
+        S->used = 0;
+        for (int k = 0; k < np; ++k) {
+            const int rc = dmalloc(&S->pool[k], S->cap);
+            if (rc) {
+                // no pools: every later submit reports SIPG_ERR_BUDGET
+                for (double *&pl : S->pool) {
+                    if (pl) (void)hipFree(pl);
+                    pl = nullptr;
+                }
+                S->cap = 0;
+                S->freel.clear();
+                return rc;
+            }
+        }
+    }
+    S->method = method;
+    S->graph_stale = true;
+    return SIPG_OK;
+}
+
 int sipg_server_submit(sipg_server *S, int dim, int p, double pen, int nmodes, const double *a, const int *m,
                        const int *n, double tol, int tol_mode, int maxit, int64_t *id) {
     if (!S || dim < 1 || p < 1 || !std::isfinite(pen) || nmodes < 0 || nmodes > 64 ||
@@ -491,6 +568,7 @@ int sipg_server_submit(sipg_server *S, int dim, int p, double pen, int nmodes, c
         return SIPG_ERR_INVALID;
     if (p > kMaxDegree) return SIPG_ERR_UNSUPPORTED;
     if ((int64_t)6 * dim * (p + 1) * (p + 1) > (int64_t)INT_MAX) return SIPG_ERR_UNSUPPORTED;
+    if (S->method == SIPG_METHOD_PCG && !batch_prec_supported(p, pen, S->shape[p])) return SIPG_ERR_UNSUPPORTED;
     const int64_t len = (int64_t)dim * dim * (p + 1) * (p + 1);
     if (len > S->cap) return SIPG_ERR_BUDGET;             // would never fit
     Request R;
@@ -539,13 +617,13 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
                 continue;
             }
             if (R.check_req) {
-                // kStallChecks failed final checks that gain less than 10%
-                // on the best one mean the FP64 floor: the request ends
-                // there
+                // kStallChecks (PCG: kStallChecksPcg) failed final checks
+                // that gain less than 10% on the best one mean the FP64
+                // floor: the request ends there
                 R.check_req = false;
                 const bool gain = R.check_ref < 0.0 || st.rr_ref <= 0.81 * R.check_ref;
                 if (!gain) ++R.check_stale;
-                if (R.check_stale >= kStallChecks) {
+                if (R.check_stale >= (S->method == SIPG_METHOD_PCG ? kStallChecksPcg : kStallChecks)) {
                     finish(*S, R, st);
                     lists_changed = true;
                     continue;
@@ -602,12 +680,15 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
                                          (const int *)(md + 64) + 64, S->pool[1] + R.off, S->stream)))
                 return rc;
         }
-        // x = 0: r = p = b; initial states
-        if ((rc = launch_batch_residual(S->pool[1], S->pool[4], S->pool[2], S->pool[3], S->d_chunks[SET_INIT],
-                                        S->count(SET_INIT, 1), S->chunk_cap, S->d_part2, S->d_part3, S->stream)) ||
-            (rc = launch_batch_init(S->d_list[SET_INIT], S->count(SET_INIT, 0), kMaxSlots,
-                                    S->d_chunk_range[SET_INIT], S->d_part2, S->d_part3, S->d_state, S->d_ctl,
-                                    nullptr, nullptr, S->stream)))
+        // x = 0: r = p = b (PCG: z = M^-1 r, p = z); initial states
+        const int I = SET_INIT;
+        const bool pcg = S->method == SIPG_METHOD_PCG;
+        if ((rc = launch_batch_residual(S->pool[1], S->pool[4], S->pool[2], S->pool[3], S->d_chunks[I],
+                                        S->count(I, 1), S->chunk_cap, S->d_part2, S->d_part3, S->stream)) ||
+            (pcg && (rc = prec_set(*S, I, degrees_of(*S, admitted), S->pool[3], nullptr, 0))) ||
+            (rc = launch_batch_init(S->d_list[I], S->count(I, 0), kMaxSlots, S->d_chunk_range[I], S->d_part2,
+                                    S->d_part3, S->d_state, S->d_ctl, pcg ? S->d_item_range[I] : nullptr,
+                                    pcg ? S->d_item_rz[I] : nullptr, S->stream)))
             return rc;
         for (int64_t id : admitted) S->active.push_back(id);
         lists_changed = true;
@@ -661,7 +742,7 @@ int sipg_server_stats(const sipg_server *S, int *queued, int *running, int *done
     if (queued) *queued = (int)S->queue.size();
     if (running) *running = (int)S->active.size();
     if (done_held) *done_held = (int)S->done_q.size();
-    if (bytes_used) *bytes_used = (double)S->used * kPools * sizeof(double);
+    if (bytes_used) *bytes_used = (double)S->used * S->npools * sizeof(double);
     return SIPG_OK;
 }
 

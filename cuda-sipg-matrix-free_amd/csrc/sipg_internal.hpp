@@ -74,6 +74,12 @@ constexpr int kSlotWantCheck = 4;
 // scatters around a fixed level, so a lucky low one does not reset the
 // count.
 constexpr int kStallChecks = 2;
+// The same for PCG.  Its residual 2-norm swings several-fold from one
+// iteration to the next while its envelope falls, and a final check lands
+// a few iterations after the recursion met the tolerance, at a random point
+// of that swing: two checks in a row can miss a 10% gain well above the
+// FP64 floor.
+constexpr int kStallChecksPcg = 4;
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b

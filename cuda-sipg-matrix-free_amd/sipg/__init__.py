@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libsipg_hip.so")
 
 F64, F32 = 0, 1
 TOL_ABS, TOL_REL = 0, 1
+METHOD_CG, METHOD_PCG = 0, 1
 
 OK = 0
 ERR_INVALID = 1
@@ -124,6 +125,7 @@ def _declare(L):
         "sipg_server_create": (i, [ctypes.POINTER(_vp), d, _vp]),
         "sipg_server_destroy": (i, [_vp]),
         "sipg_server_set_shape": (i, [_vp, i, i, i]),
+        "sipg_server_set_method": (i, [_vp, i]),
         "sipg_server_submit": (i, [_vp, i, i, d, i, _dp, _ip, _ip, d, i, i, ctypes.POINTER(i64)]),
         "sipg_server_progress": (i, [_vp, _ip]),
         "sipg_server_take": (i, [_vp, ctypes.POINTER(i64), ctypes.POINTER(CgInfo), _dp, i64]),
@@ -474,6 +476,10 @@ class Server:
 
     def set_shape(self, p, bx, by):
         return lib().sipg_server_set_shape(self.h, p, bx, by)
+
+    def set_method(self, method):
+        """METHOD_CG or METHOD_PCG, while the server holds no request."""
+        return lib().sipg_server_set_method(self.h, method)
 
     def submit(self, dim, p, pen, modes, tol, tol_mode=TOL_REL, maxit=0):
         a, m, n = modes

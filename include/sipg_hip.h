@@ -281,6 +281,15 @@ int sipg_server_destroy(sipg_server *s);
 /* Block shape of the operator kernel for degree p (a sipg_batch_tune
  * result; 8x4 by default), while no request of degree p is running. */
 int sipg_server_set_shape(sipg_server *s, int p, int bx, int by);
+/* Solver of every request: SIPG_METHOD_CG (the default) or SIPG_METHOD_PCG,
+ * the block-Jacobi preconditioned CG of sipg_batch_pcg
+ * (preconditioned_conjugate_gradient, conjugate_gradient_gpu.hpp:116-213).
+ * Only while the server holds no request (else SIPG_ERR_INVALID).  Under
+ * PCG a request of n unknowns needs 56 n bytes (one more vector), submit
+ * refuses a penalty other than 5 (p+1)^2 with SIPG_ERR_UNSUPPORTED, and so
+ * does set_shape a block shape the preconditioner kernel lacks. */
+enum { SIPG_METHOD_CG = 0, SIPG_METHOD_PCG = 1 };
+int sipg_server_set_method(sipg_server *s, int method);
 /* Queue a request: dim x dim mesh of degree p, penalty pen, Fourier forcing
  * (as sipg_rhs_fourier), tolerance (tol, tol_mode), maxit <= 0: n.
  * SIPG_ERR_BUDGET if it can never fit the budget. */

@@ -34,11 +34,13 @@ static std::vector<Req> make_requests(int count) {
 
 // solves `reqs` on one server; returns solutions by request index
 static std::map<int, std::vector<double>> serve(const std::vector<Req> &reqs, double budget, int *max_queued,
-                                                int *rc_out, std::vector<sipg_cg_info> *infos) {
+                                                int *rc_out, std::vector<sipg_cg_info> *infos,
+                                                int method = SIPG_METHOD_CG) {
     std::map<int, std::vector<double>> sol;
     sipg_server *S = nullptr;
     int rc = sipg_server_create(&S, budget, nullptr);
     if (rc) { *rc_out = rc; return sol; }
+    if ((rc = sipg_server_set_method(S, method))) { *rc_out = rc; sipg_server_destroy(S); return sol; }
     std::map<int64_t, int> index;
     for (size_t i = 0; i < reqs.size(); ++i) {
         const Req &r = reqs[i];
@@ -100,6 +102,58 @@ int main() {
                "\"true_rel\":%.3e,\"alone_maxdiff\":%.3e}\n",
                k, rc1, infos.size() > k ? infos[k].status : -1, infos.size() > k ? infos[k].iters : -1, r.tol,
                infos.size() > k && infos[k].b_norm > 0 ? infos[k].true_res_norm / infos[k].b_norm : -1.0, same);
+    }
+    // the same stream under PCG (56 bytes per unknown: the same share of
+    // the demand fits)
+    std::vector<sipg_cg_info> pinfos;
+    int pq = 0, prc = 0;
+    auto pmix = serve(reqs, 56.0 * demand / 3.0, &pq, &prc, &pinfos, SIPG_METHOD_PCG);
+    printf("{\"case\":\"serve_pcg_mix\",\"rc\":%d,\"done\":%zu,\"total\":%zu,\"max_queued\":%d}\n", prc,
+           pmix.size(), reqs.size(), pq);
+    for (size_t k = 0; k < reqs.size(); ++k) {
+        double same = -1.0, dcg = -1.0, ncg = 0.0;
+        if (k % 4 == 0) {           // alone vs in the mix, for a sample
+            std::vector<Req> one{reqs[k]};
+            int mq = 0, rc1 = 0;
+            auto alone = serve(one, 56.0 * demand / 3.0, &mq, &rc1, nullptr, SIPG_METHOD_PCG);
+            if (pmix.count(k) && alone.count(0)) {
+                same = 0.0;
+                for (size_t i = 0; i < pmix[k].size(); ++i)
+                    same = std::fmax(same, std::fabs(pmix[k][i] - alone[0][i]));
+            }
+        }
+        if (pmix.count(k) && mix.count(k)) {
+            dcg = 0.0;
+            for (size_t i = 0; i < pmix[k].size(); ++i) {
+                dcg += (pmix[k][i] - mix[k][i]) * (pmix[k][i] - mix[k][i]);
+                ncg += mix[k][i] * mix[k][i];
+            }
+            dcg = std::sqrt(dcg / ncg);
+        }
+        printf("{\"case\":\"serve_pcg_one\",\"k\":%zu,\"status\":%d,\"iters\":%d,\"cg_iters\":%d,"
+               "\"tol\":%.0e,\"true_rel\":%.3e,\"rel_vs_cg\":%.3e,\"alone_maxdiff\":%.3e}\n",
+               k, pinfos.size() > k ? pinfos[k].status : -1, pinfos.size() > k ? pinfos[k].iters : -1,
+               infos.size() > k ? infos[k].iters : -1, reqs[k].tol,
+               pinfos.size() > k && pinfos[k].b_norm > 0 ? pinfos[k].true_res_norm / pinfos[k].b_norm : -1.0, dcg,
+               same);
+    }
+    // refusals: a non-standard penalty under PCG; a method change while a
+    // request is held
+    {
+        sipg_server *S = nullptr;
+        int rc_pen = -1, rc_busy = -1, rc_create = sipg_server_create(&S, 1e7, nullptr);
+        if (!rc_create) {
+            sipg_server_set_method(S, SIPG_METHOD_PCG);
+            int64_t id = 0;
+            const Req &r = reqs[0];
+            rc_pen = sipg_server_submit(S, r.dim, r.p, 81.0, r.nm, r.a, r.m, r.n, 1e-8, SIPG_TOL_REL, 0, &id);
+            sipg_server_submit(S, r.dim, r.p, 5.0 * (r.p + 1) * (r.p + 1), r.nm, r.a, r.m, r.n, 1e-8,
+                               SIPG_TOL_REL, 0, &id);
+            rc_busy = sipg_server_set_method(S, SIPG_METHOD_CG);
+            sipg_server_destroy(S);
+        }
+        printf("{\"case\":\"serve_pcg_refusals\",\"create\":%d,\"pen\":%d,\"busy\":%d}\n", rc_create, rc_pen,
+               rc_busy);
     }
     return 0;
 }

@@ -163,7 +163,29 @@ def test_server_admits_within_budget_and_finishes_all(serve_results):
 
 
 def test_server_result_same_alone_and_in_the_mix(serve_results):
-    for r in serve_results[1:]:
+    ones = [r for r in serve_results if r["case"] == "serve_one"]
+    assert len(ones) == 24
+    for r in ones:
         assert r["rc"] == 0 and r["status"] == 0, r
         assert 0 <= r["true_rel"] <= r["tol"], r
         assert r["alone_maxdiff"] == 0.0, r
+
+
+def test_server_pcg_finishes_all_like_cg(serve_results):
+    mix = next(r for r in serve_results if r["case"] == "serve_pcg_mix")
+    assert mix["rc"] == 0 and mix["done"] == mix["total"] == 24 and mix["max_queued"] > 0
+    ones = [r for r in serve_results if r["case"] == "serve_pcg_one"]
+    assert len(ones) == 24
+    for r in ones:
+        assert r["status"] == 0 and 0 <= r["true_rel"] <= r["tol"], r
+        # two solutions to the same tolerance: their distance is of the
+        # order of the tolerance times the condition number's growth
+        assert r["rel_vs_cg"] <= 1e4 * r["tol"], r
+        if r["alone_maxdiff"] >= 0:
+            assert r["alone_maxdiff"] == 0.0, r
+    assert sum(r["iters"] for r in ones) < 0.8 * sum(r["cg_iters"] for r in ones)
+
+
+def test_server_pcg_refusals(serve_results):
+    r = next(r for r in serve_results if r["case"] == "serve_pcg_refusals")
+    assert r["create"] == 0 and r["pen"] == 3 and r["busy"] == 1      # UNSUPPORTED, INVALID

@@ -19,8 +19,9 @@ def _requests(n):
     return out
 
 
-def _serve(sipg, reqs, budget):
+def _serve(sipg, reqs, budget, method=0):
     S = sipg.Server(budget)
+    assert S.set_method(method) == sipg.OK
     ids = {}
     for i, (dim, p, tol, seed) in enumerate(reqs):
         ids[S.submit(dim, p, sipg.pen_sigma(p), sipg.forcing_draw(seed, 1, 4, 4), tol)] = i
@@ -40,16 +41,19 @@ def _serve(sipg, reqs, budget):
     return out, max_queued
 
 
-def test_server_results_alone_and_in_the_mix(sipg):
+@pytest.mark.parametrize("method", ["cg", "pcg"])
+def test_server_results_alone_and_in_the_mix(sipg, method):
+    m = sipg.METHOD_PCG if method == "pcg" else sipg.METHOD_CG
+    per = 56 if method == "pcg" else 48                # bytes per unknown
     reqs = _requests(16)
-    demand = sum(48 * d * d * (p + 1) ** 2 for d, p, _, _ in reqs)
-    budget = max(demand / 3, max(48 * d * d * (p + 1) ** 2 for d, p, _, _ in reqs))
-    mix, max_queued = _serve(sipg, reqs, budget)
+    demand = sum(per * d * d * (p + 1) ** 2 for d, p, _, _ in reqs)
+    budget = max(demand / 3, max(per * d * d * (p + 1) ** 2 for d, p, _, _ in reqs))
+    mix, max_queued = _serve(sipg, reqs, budget, m)
     assert len(mix) == len(reqs) and max_queued > 0
     for i, (dim, p, tol, seed) in enumerate(reqs):
         info, x = mix[i]
         assert info.status == sipg.OK, (i, info.status)
-        alone, _ = _serve(sipg, [reqs[i]], budget)
+        alone, _ = _serve(sipg, [reqs[i]], budget, m)
         assert np.array_equal(alone[0][1], x), i
         op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F64)
         b = op.vector()
@@ -63,3 +67,47 @@ def test_server_rejects_a_request_over_budget(sipg):
     with pytest.raises(sipg.SipgError) as e:
         S.submit(64, 10, sipg.pen_sigma(10), sipg.forcing_draw(0, 1, 4, 4), 1e-8)
     assert e.value.status == sipg.ERR_BUDGET
+
+
+def test_server_pcg_needs_fewer_iterations(sipg):
+    reqs = _requests(8)
+    budget = sum(56 * d * d * (p + 1) ** 2 for d, p, _, _ in reqs)
+    cg, _ = _serve(sipg, reqs, budget, sipg.METHOD_CG)
+    pcg, _ = _serve(sipg, reqs, budget, sipg.METHOD_PCG)
+    assert sum(pcg[i][0].iters for i in pcg) < 0.8 * sum(cg[i][0].iters for i in cg)
+    for i in range(len(reqs)):
+        a, b = cg[i][1], pcg[i][1]
+        assert np.linalg.norm(a - b) <= 1e4 * reqs[i][2] * np.linalg.norm(a), i
+
+
+def test_server_pcg_refusals(sipg):
+    S = sipg.Server(1e7)
+    assert S.set_method(sipg.METHOD_PCG) == sipg.OK
+    with pytest.raises(sipg.SipgError) as e:
+        S.submit(6, 3, 81.0, sipg.forcing_draw(0, 1, 4, 4), 1e-8)
+    assert e.value.status == sipg.ERR_UNSUPPORTED
+    S.submit(6, 3, sipg.pen_sigma(3), sipg.forcing_draw(0, 1, 4, 4), 1e-8)
+    assert S.set_method(sipg.METHOD_CG) == sipg.ERR_INVALID
+
+
+@pytest.mark.parametrize("shape", [(8, 4), (16, 4), (8, 2)])
+def test_server_pcg_reaches_1e8_on_an_oscillating_residual(sipg, shape):
+    # 96 x 96, p = 8, tol 1e-8: PCG's residual norm swings several-fold per
+    # iteration near the tolerance, so several final checks can land above
+    # it before one lands below; the solve must not be declared stuck
+    S = sipg.Server(1e9)
+    assert S.set_method(sipg.METHOD_PCG) == sipg.OK
+    assert S.set_shape(8, *shape) == sipg.OK
+    fd = sipg.forcing_draw(358, 1, 4, 4)
+    S.submit(96, 8, sipg.pen_sigma(8), fd, 1e-8, maxit=30000)
+    r = None
+    for _ in range(20000):
+        S.progress()
+        r = S.take(96 * 96 * 81)
+        if r is not None:
+            break
+    assert r is not None and r[1].status == sipg.OK, (r[1].status, r[1].iters)
+    op = sipg.Operator(96, 96, 8, sipg.pen_sigma(8), sipg.F64)
+    b = op.vector()
+    op.rhs_fourier(fd, b)
+    assert op.true_relres(op.to_device(r[2]), b) <= 1e-8 * 1.0000001
