@@ -318,6 +318,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="arrival window of the serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="arrivals per second")
+    ap.add_argument("--serve-cg", action="store_true",
+                    help="also serve the same arrivals by CG (the served solver is PCG)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs that measure roofline.traffic")
     ap.add_argument("--profile-apply", type=int, default=0, metavar="N",
@@ -440,7 +442,8 @@ def main():
     serving = None
     if args.serve_seconds > 0:
         serving = serving_leg(sipg, args.serve_seconds, args.serve_rate, "pcg")
-        serving["cg"] = serving_leg(sipg, args.serve_seconds, args.serve_rate, "cg")
+        if args.serve_cg:
+            serving["cg"] = serving_leg(sipg, args.serve_seconds, args.serve_rate, "cg")
 
     traffic, traffic_note = (None, "not measured (--no-traffic)")
     if not args.no_traffic:
