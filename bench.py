@@ -437,6 +437,10 @@ def main():
     value = n * args.steps / t_total
     it = int(np.median(iters))
     log(f"timed ({solver}): {ms_step:.1f} ms/solve, {value:.3e} DoF/s")
+    # discretisation error of the solution against the Fourier solution
+    # (sipg_sem_2d::err_norms, on the device; outside the timed region)
+    err = op.err_norms_fourier(modes, xs)
+    log(f"error vs exact solution: max {err['max']:.3e}, L2 {err['L2']:.3e}, H1 {err['H1']:.3e}")
 
     batched = None if args.no_batch else batched_leg(sipg)
     serving = None
@@ -469,6 +473,7 @@ def main():
             "dim": DIM, "p": P, "dof": n, "pen": pen,
             "solver": SOLVERS[solver] + " (the fastest of CG, PCG and mixed here)",
             "iterations": it, "true_relres": relres,
+            "error_vs_exact": {k: float(v) for k, v in err.items()},
             "fp64_plain": {"dof_per_s": n / (ms64 * 1e-3), "ms_per_solve": ms64,
                            "iterations": it64[-1], "true_relres": rel64},
             "pcg": pcg,

@@ -311,6 +311,48 @@ int sipg_rhs_fourier(sipg_op *op, int nmodes, const double *a, const int *m, con
     return SIPG_OK;
 }
 
+int sipg_err_norms_fourier(sipg_op *op, const void *x, int nmodes, const double *a, const int *m, const int *n,
+                           double *out) {
+    if (!op || !x || !out || nmodes < 0 || nmodes > 64 || (nmodes > 0 && (!a || !m || !n)))
+        return SIPG_ERR_INVALID;
+    if (op->dimx != op->dimy) return SIPG_ERR_INVALID;
+    const int p = op->p, n1 = p + 1;
+    std::vector<double> z(n1), w(n1), D((size_t)n1 * n1);
+    gll(p, z.data(), w.data());
+    dphi(p, z.data(), D.data());
+    // modes in the scalar scratch (as sipg_rhs_fourier); D in the head of
+    // the p.Ap partials, the block partials in the four regions after it
+    const parts P = part_layout(op);
+    double *da = op->d_scalar + 64;
+    int *dm = (int *)(op->d_scalar + 128);
+    int *dn = dm + 64;
+    if (nmodes > 0) {
+        SIPG_HIP_TRY(hipMemcpyAsync(da, a, nmodes * sizeof(double), hipMemcpyHostToDevice, op->stream));
+        SIPG_HIP_TRY(hipMemcpyAsync(dm, m, nmodes * sizeof(int), hipMemcpyHostToDevice, op->stream));
+        SIPG_HIP_TRY(hipMemcpyAsync(dn, n, nmodes * sizeof(int), hipMemcpyHostToDevice, op->stream));
+    }
+    SIPG_HIP_TRY(hipMemcpyAsync(P.pAp, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice, op->stream));
+    const int64_t total = op->n;
+    const int nb = (int)std::min<int64_t>(kMaxPartials, (total + kVecThreads - 1) / kVecThreads);
+    int rc = launch_err_norms(op->dtype, p, op->dimx, (const double *)op->d_tab64, P.pAp, nmodes, da, dm, dn, x,
+                              P.rr, nb, op->stream);
+    if (rc) return rc;
+    std::vector<double> h((size_t)4 * nb);
+    SIPG_HIP_TRY(hipMemcpyAsync(h.data(), P.rr, h.size() * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+    SIPG_HIP_TRY(hipStreamSynchronize(op->stream));
+    double emax = -1.0, l2 = 0.0, sx = 0.0, sy = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        emax = std::max(emax, h[b]);
+        l2 += h[(size_t)nb + b];
+        sx += h[(size_t)2 * nb + b];
+        sy += h[(size_t)3 * nb + b];
+    }
+    out[0] = emax;
+    out[1] = std::sqrt(l2);
+    out[2] = std::sqrt(l2 + sx + sy);
+    return SIPG_OK;
+}
+
 int sipg_time_apply(sipg_op *op, const void *x, void *y, int reps, double *ms_per_apply) {
     if (!op || !x || !y || !ms_per_apply || reps < 1) return SIPG_ERR_INVALID;
     if (overlaps(x, y, (size_t)op->n * dtype_size(op->dtype))) return SIPG_ERR_ALIAS;

@@ -329,6 +329,79 @@ __global__ __launch_bounds__(NT) void rhs_fourier_kernel(int p, int dim, const d
     }
 }
 
+// Error norms of u against the Fourier solution u_ex = sum_k a_k sin(m_k pi x)
+// sin(n_k pi y) (sipg_sem_2d::err_norms, sipg_sem_2d.hpp:311-370), one thread
+// per GLL node: the nodal error, and the x and y derivatives of u at the node
+// from the element's line through it, u_x = 2 dim sum_i u(i,q) D_i(p) (the
+// basis is nodal, so phi_j(z_q) = [j = q]), weighted by the GLL quadrature
+// weights (w_p/2dim)(w_q/2dim).  Block partials: max |e|, sum w e^2,
+// sum w (u_x - u_ex,x)^2, sum w (u_y - u_ex,y)^2.  D[i*(p+1)+k] = phi_i'(z_k).
+template <typename T>
+__global__ __launch_bounds__(NT) void err_norms_kernel(int p, int dim, const double *__restrict__ tab,
+                                                       const double *__restrict__ D, int nmodes,
+                                                       const double *__restrict__ am, const int *__restrict__ mm,
+                                                       const int *__restrict__ nn, const T *__restrict__ u,
+                                                       double *__restrict__ part) {
+    const int n1 = p + 1, nq = n1 * n1;
+    const int64_t noe = (int64_t)dim * dim, n = noe * nq;
+    const double *w = tab;
+    const double *z = tab + 3 * n1 + n1 * n1;
+    const double h = 1.0 / dim, two_dim = 2.0 * dim;
+    double emax = -1.0, l2 = 0.0, sx = 0.0, sy = 0.0;
+    for (int64_t idx = (int64_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * NT) {
+        const int64_t e = idx / nq;
+        const int node = (int)(idx - e * nq);
+        const int i1 = node / n1, i2 = node - (node / n1) * n1;
+        const T *ue = u + e * nq;
+        double dxu = 0.0, dyu = 0.0;
+        for (int i = 0; i < n1; ++i) {
+            dxu += (double)ue[i * n1 + i2] * two_dim * D[i * n1 + i1];
+            dyu += (double)ue[i1 * n1 + i] * two_dim * D[i * n1 + i2];
+        }
+        const int col = (int)(e % dim), row = (int)(e / dim);
+        const double X = col * h + (z[i1] * .5 + .5) / dim;
+        const double Y = 1.0 - (row + 1) * h + (z[i2] * .5 + .5) / dim;
+        double ux = 0.0, dx = 0.0, dy = 0.0;
+        for (int k = 0; k < nmodes; ++k) {
+            const double m = mm[k] * M_PI, q = nn[k] * M_PI;
+            const double sxm = sin(m * X), cxm = cos(m * X), syq = sin(q * Y), cyq = cos(q * Y);
+            ux += am[k] * sxm * syq;
+            dx += am[k] * m * cxm * syq;
+            dy += am[k] * q * sxm * cyq;
+        }
+        const double wgt = (w[i1] * .5 / dim) * (w[i2] * .5 / dim);
+        const double err = fabs((double)ue[node] - ux);
+        emax = fmax(emax, err);
+        l2 += wgt * err * err;
+        sx += wgt * (dxu - dx) * (dxu - dx);
+        sy += wgt * (dyu - dy) * (dyu - dy);
+    }
+    __shared__ double red[4][NT / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) emax = fmax(emax, __shfl_xor(emax, off, 64));
+    l2 = wave_sum(l2);
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    if (lane == 0) {
+        red[0][wid] = emax;
+        red[1][wid] = l2;
+        red[2][wid] = sx;
+        red[3][wid] = sy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r[4] = {-1.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < NT / 64; ++i) {
+            r[0] = fmax(r[0], red[0][i]);
+            r[1] += red[1][i];
+            r[2] += red[2][i];
+            r[3] += red[3][i];
+        }
+        for (int k = 0; k < 4; ++k) part[(size_t)k * gridDim.x + blockIdx.x] = r[k];
+    }
+}
+
 }  // namespace
 
 int vec_num_blocks(int64_t n) {
@@ -492,6 +565,16 @@ int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t 
             hipLaunchKernelGGL(to_reference_layout_kernel<double>, dim3((int)nb), dim3(NT), 0, s, (const double *)src, (double *)dst, noe, nq),
             hipLaunchKernelGGL(to_reference_layout_kernel<float>, dim3((int)nb), dim3(NT), 0, s, (const float *)src, (float *)dst, noe, nq));
     }
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_err_norms(int dtype, int p, int dim, const double *tab_f64, const double *D_dev, int nmodes,
+                     const double *a_dev, const int *m_dev, const int *n_dev, const void *u, double *part, int nb,
+                     hipStream_t s) {
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(err_norms_kernel<double>, dim3(nb), dim3(NT), 0, s, p, dim, tab_f64, D_dev, nmodes, a_dev, m_dev, n_dev, (const double *)u, part),
+        hipLaunchKernelGGL(err_norms_kernel<float>, dim3(nb), dim3(NT), 0, s, p, dim, tab_f64, D_dev, nmodes, a_dev, m_dev, n_dev, (const float *)u, part));
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

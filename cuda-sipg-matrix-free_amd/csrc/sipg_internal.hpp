@@ -240,6 +240,10 @@ int launch_batch_mark(const int *list, const int *count, int max_inst, cg_state 
 // reference order <-> element-major device order (to_device = 1: ref -> dev)
 int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t noe, int nq,
                   hipStream_t s);
+// block partials of the error norms (4 x nb: max, L2, dx, dy); D = phi_i'(z_k)
+int launch_err_norms(int dtype, int p, int dim, const double *tab_f64, const double *D_dev, int nmodes,
+                     const double *a_dev, const int *m_dev, const int *n_dev, const void *u, double *part, int nb,
+                     hipStream_t s);
 int launch_rhs_fourier(int dtype, int p, int dim, const void *tab_f64, int nmodes,
                        const double *a_dev, const int *m_dev, const int *n_dev, void *b,
                        hipStream_t s);

@@ -104,6 +104,7 @@ def _declare(L):
         "sipg_vec_alpha_update": (i, [i, i64, d, _vp, _vp, _vp, _vp, _vp]),
         "sipg_forcing_draw": (i, [ctypes.c_uint64, i, i, i, _ip, _dp, _ip, _ip]),
         "sipg_rhs_fourier": (i, [_vp, i, _dp, _ip, _ip, _vp]),
+        "sipg_err_norms_fourier": (i, [_vp, _vp, i, _dp, _ip, _ip, _dp]),
         "sipg_time_apply": (i, [_vp, _vp, _vp, i, _dp]),
         "sipg_upload": (i, [_vp, _dp, _vp]),
         "sipg_download": (i, [_vp, _vp, _dp]),
@@ -328,6 +329,18 @@ class Operator:
         n = np.ascontiguousarray(n, dtype=np.int32)
         check(lib().sipg_rhs_fourier(self.h, len(a), a.ctypes.data_as(_dp), m.ctypes.data_as(_ip),
                                      n.ctypes.data_as(_ip), b.ptr), "rhs_fourier")
+
+    def err_norms_fourier(self, modes, x):
+        """Max nodal, L2 and H1 errors of x against the Fourier solution
+        (sipg_sem_2d::err_norms), on the device."""
+        a, m, n = modes
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        out = np.zeros(3)
+        check(lib().sipg_err_norms_fourier(self.h, x.ptr, len(a), a.ctypes.data_as(_dp), m.ctypes.data_as(_ip),
+                                           n.ctypes.data_as(_ip), out.ctypes.data_as(_dp)), "err_norms_fourier")
+        return {"max": out[0], "L2": out[1], "H1": out[2]}
 
     def time_apply(self, x, y, reps):
         ms = ctypes.c_double()

@@ -314,6 +314,14 @@ int sipg_forcing_draw(uint64_t seed, int kmin, int kmax, int mmax,
  * homogeneous Dirichlet data (sipg_sem_2d.hpp:257-262 with g = 0), in the
  * op's dtype, computed on the device. */
 int sipg_rhs_fourier(sipg_op *op, int nmodes, const double *a, const int *m, const int *n, void *b);
+/* Error norms of the device vector x (op's dtype and layout) against the
+ * Fourier solution u = sum_k a_k sin(m_k pi x) sin(n_k pi y), computed on
+ * the device: out[0] = max nodal error, out[1] = L2 error, out[2] = H1 error
+ * (L2 plus both derivative errors), by GLL quadrature on each element as
+ * sipg_sem_2d::err_norms (sipg_sem_2d.hpp:311-370; members max_err, L2_err,
+ * H1_err, sipg_sem_2d.hpp:73-81).  Square meshes. */
+int sipg_err_norms_fourier(sipg_op *op, const void *x, int nmodes, const double *a, const int *m, const int *n,
+                           double *out);
 
 /* ---- measurement -------------------------------------------------------- */
 /* Runs `reps` back-to-back applies y = A x on the op's stream between two

@@ -288,6 +288,25 @@ int launch_rhs_fourier(int dtype, int p, int dim, const void *, int nmodes, cons
     return SIPG_OK;
 }
 
+// error norms by the oracle, as one block partial (max, L2^2, 0, 0)
+int launch_err_norms(int dtype, int p, int dim, const double *, const double *, int nmodes, const double *a,
+                     const int *m, const int *n, const void *u, double *part, int nb, hipStream_t) {
+    if (dtype != SIPG_F64 || nb < 1) return SIPG_ERR_UNSUPPORTED;
+    orc_problem pb{0, nmodes, a, m, n};
+    const int64_t noe = (int64_t)dim * dim, nq = (int64_t)(p + 1) * (p + 1);
+    std::vector<double> ur(noe * nq);
+    for (int64_t e = 0; e < noe; ++e)
+        for (int64_t q = 0; q < nq; ++q) ur[q * noe + e] = tp<const double>(u)[e * nq + q];
+    double out[3];
+    orc_err_norms(dim, p, ur.data(), &pb, out);
+    for (int k = 0; k < 4; ++k)
+        for (int b = 0; b < nb; ++b) part[(size_t)k * nb + b] = k == 0 ? -1.0 : 0.0;
+    part[0] = out[0];
+    part[nb] = out[1] * out[1];
+    part[2 * nb] = out[2] * out[2] - out[1] * out[1];
+    return SIPG_OK;
+}
+
 // ---- PCG (kernels_pcg.hip) ----------------------------------------------
 // z = M^-1 r per element with the op's tables [S, Linv]:
 // Z = S [(S^T R S) .* Linv] S^T on each (p+1) x (p+1) tile.

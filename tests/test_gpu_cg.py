@@ -266,3 +266,64 @@ def test_operator_free_vector_kernels(sipg):
     sipg.synchronize()
     assert np.array_equal(C.download(np.float64), a)
     assert sipg.lib().sipg_vec_copy(sipg.F64, n, A.ptr, A.ptr, None) == sipg.ERR_ALIAS
+
+
+@pytest.mark.parametrize("dim,p", [(8, 3), (13, 6), (16, 2)])
+def test_err_norms_match_oracle(sipg, oracle, dim, p):
+    # sipg_err_norms_fourier against the oracle's err_norms (itself pinned by
+    # the thesis error tables): on a random vector (errors of order one, so
+    # the comparison is tight) and on a CG solution (the discretisation error)
+    modes = sipg.forcing_draw(3, 4, 4, 4)
+    prob = oracle.problem(0, modes)
+    op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F64)
+    xr = np.random.default_rng(dim * p).uniform(-1, 1, op.n)
+    x = op.to_device(xr)
+    got = op.err_norms_fourier(modes, x)
+    ref = oracle.err_norms(dim, p, xr, prob)
+    for k in ("max", "L2", "H1"):
+        assert abs(got[k] - ref[k]) <= 1e-12 * ref[k], (k, got[k], ref[k])
+    b = op.vector()
+    op.rhs_fourier(modes, b)
+    x.zero()
+    st, it, res = op.cg(x, b, 1e-13, sipg.TOL_REL)
+    got = op.err_norms_fourier(modes, x)
+    ref = oracle.err_norms(dim, p, op.download(x), prob)
+    # the oracle places nodes on the reference's mesh coordinates, summed
+    # edge by edge (O(1e-16) off the exact ones): 1e-5 of an error of 1e-9
+    for k in ("max", "L2", "H1"):
+        assert abs(got[k] - ref[k]) <= 1e-5 * ref[k], (k, got[k], ref[k])
+    # convergence: a finer mesh of the same degree has a smaller error
+    op2 = sipg.Operator(2 * dim, 2 * dim, p, sipg.pen_sigma(p), sipg.F64)
+    b2, x2 = op2.vector(), op2.vector()
+    op2.rhs_fourier(modes, b2)
+    x2.zero()
+    op2.cg(x2, b2, 1e-13, sipg.TOL_REL)
+    fine = op2.err_norms_fourier(modes, x2)
+    assert fine["L2"] < 0.25 * got["L2"] and fine["H1"] < 0.5 * got["H1"], (fine, got)
+
+
+def test_err_norms_of_the_interpolant(sipg):
+    # the nodal interpolant of u_ex has zero nodal error (max and L2 are
+    # nodal quadratures), and its H1 error is the derivative error of the
+    # degree-p interpolant, which falls with the mesh size.  The nodes are
+    # placed by the reference's convention (element e = dim*row + col,
+    # row 0 at the top: x = (col + (z_i+1)/2)/dim, y = 1 - (row + 1 - (z_j+1)/2)/dim)
+    modes = sipg.forcing_draw(5, 4, 4, 4)
+    a, m, n = modes
+    h1 = []
+    p = 5
+    z, _, _, _ = sipg.basis_tables(p)
+    for dim, dtype in ((6, sipg.F64), (12, sipg.F64), (6, sipg.F32)):
+        e = np.arange(dim * dim)
+        col, row = e % dim, e // dim
+        X = (col[None, None, :] + (z[:, None, None] + 1) / 2) / dim            # [i1, 1, e]
+        Y = 1 - (row[None, None, :] + 1 - (z[None, :, None] + 1) / 2) / dim    # [1, i2, e]
+        u = sum(a[k] * np.sin(m[k] * np.pi * X) * np.sin(n[k] * np.pi * Y) for k in range(len(a)))
+        op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), dtype)
+        x = op.to_device(u.reshape(-1))                  # reference order (i1, i2, e)
+        got = op.err_norms_fourier(modes, x)
+        tol = 1e-14 if dtype == sipg.F64 else 1e-6       # FP32 rounding of the uploaded u
+        assert got["max"] <= tol and got["L2"] <= tol, (dim, dtype, got)
+        if dtype == sipg.F64:
+            h1.append(got["H1"])
+    assert 0 < h1[1] < 0.5 ** 4 * h1[0], h1
