@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <random>
 
@@ -409,6 +410,42 @@ int sipg_download(sipg_op *op, const void *dev, double *host) {
             for (int64_t q = 0; q < nq; ++q) host[q * noe + e] = tmp[e * nq + q];
     }
     return SIPG_OK;
+}
+
+int sipg_write_result(sipg_op *op, const void *x, const char *path) {
+    if (!op || !x) return SIPG_ERR_INVALID;
+    if (op->dimx != op->dimy) return SIPG_ERR_INVALID;
+    std::vector<double> u(op->n);
+    int rc = sipg_download(op, x, u.data());
+    if (rc) return rc;
+    const int dim = op->dimx, p = op->p, n1 = p + 1;
+    const int64_t noe = (int64_t)dim * dim;
+    std::vector<double> z(n1), w(n1), nodes(n1);
+    gll(p, z.data(), w.data());
+    for (int i = 0; i < n1; ++i) nodes[i] = (z[i] * .5 + .5) / dim;
+    FILE *f = (path && path[0]) ? std::fopen(path, "w") : stderr;
+    if (!f) return SIPG_ERR_INVALID;
+    // element corners as the mesh builder places them: x summed along each
+    // row from 0, y stepped down from 1 once per row (build_square_mesh.hpp:
+    // 251-284), so the printed coordinates carry the same rounding
+    const double delta = 1. / (double)dim;
+    double yy = 1;
+    bool ok = true;
+    for (int row = 0; row < dim && ok; ++row) {
+        double xx = 0;
+        yy -= delta;
+        for (int col = 0; col < dim && ok; ++col) {
+            const int64_t e = (int64_t)row * dim + col;
+            for (int i = 0; i < n1; ++i)
+                for (int j = 0; j < n1; ++j)
+                    ok = ok && std::fprintf(f, "%10g\t%10g\t%10g\n", xx + nodes[i], yy + nodes[j],
+                                            u[((int64_t)i * n1 + j) * noe + e]) > 0;
+            ok = ok && std::fputc('\n', f) != EOF;
+            xx += delta;
+        }
+    }
+    if (f != stderr) ok = (std::fclose(f) == 0) && ok;
+    return ok ? SIPG_OK : SIPG_ERR_INVALID;
 }
 
 int sipg_to_device_layout(sipg_op *op, const void *ref, void *dev) {

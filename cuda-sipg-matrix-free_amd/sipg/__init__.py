@@ -105,6 +105,7 @@ def _declare(L):
         "sipg_forcing_draw": (i, [ctypes.c_uint64, i, i, i, _ip, _dp, _ip, _ip]),
         "sipg_rhs_fourier": (i, [_vp, i, _dp, _ip, _ip, _vp]),
         "sipg_err_norms_fourier": (i, [_vp, _vp, i, _dp, _ip, _ip, _dp]),
+        "sipg_write_result": (i, [_vp, _vp, ctypes.c_char_p]),
         "sipg_time_apply": (i, [_vp, _vp, _vp, i, _dp]),
         "sipg_upload": (i, [_vp, _dp, _vp]),
         "sipg_download": (i, [_vp, _vp, _dp]),
@@ -341,6 +342,11 @@ class Operator:
         check(lib().sipg_err_norms_fourier(self.h, x.ptr, len(a), a.ctypes.data_as(_dp), m.ctypes.data_as(_ip),
                                            n.ctypes.data_as(_ip), out.ctypes.data_as(_dp)), "err_norms_fourier")
         return {"max": out[0], "L2": out[1], "H1": out[2]}
+
+    def write_result(self, x, path=None):
+        """x y u per node as text (sipg_sem_2d::print_result); None: stderr."""
+        check(lib().sipg_write_result(self.h, x.ptr, None if path is None else os.fsencode(path)),
+              "write_result")
 
     def time_apply(self, x, y, reps):
         ms = ctypes.c_double()

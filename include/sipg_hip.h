@@ -322,6 +322,13 @@ int sipg_rhs_fourier(sipg_op *op, int nmodes, const double *a, const int *m, con
  * H1_err, sipg_sem_2d.hpp:73-81).  Square meshes. */
 int sipg_err_norms_fourier(sipg_op *op, const void *x, int nmodes, const double *a, const int *m, const int *n,
                            double *out);
+/* The solution as text, as sipg_sem_2d::print_result (sipg_sem_2d.hpp:
+ * 375-403): per element (row-major from the top-left), per node (x index
+ * outer, y inner) one line "x<TAB>y<TAB>u", each field printf "%10g" (the
+ * ostream's setw(10) at its default precision), and an empty line after
+ * each element.  path NULL or "": standard error (where the reference
+ * prints).  x: device vector of the op.  Square meshes. */
+int sipg_write_result(sipg_op *op, const void *x, const char *path);
 
 /* ---- measurement -------------------------------------------------------- */
 /* Runs `reps` back-to-back applies y = A x on the op's stream between two

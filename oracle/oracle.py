@@ -56,6 +56,8 @@ def lib():
                                 ctypes.c_double, ctypes.c_int, _dp, _dp]
         L.orc_solve.restype = ctypes.c_int
         L.orc_mesh_coords.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_print_result.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_char_p]
+        L.orc_print_result.restype = ctypes.c_int
         for fn in ("orc_u_ex", "orc_f", "orc_dx_u_ex", "orc_dy_u_ex"):
             getattr(L, fn).argtypes = [ctypes.POINTER(Problem), ctypes.c_double, ctypes.c_double]
             getattr(L, fn).restype = ctypes.c_double
@@ -154,6 +156,13 @@ def err_norms(dim, p, u, prob):
     out = np.zeros(3)
     lib().orc_err_norms(dim, p, _d(u), prob.ref(), _d(out))
     return {"max": out[0], "L2": out[1], "H1": out[2]}
+
+
+def print_result(dim, p, u, path):
+    """print_result's text for u (reference order) written to path."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    if lib().orc_print_result(dim, p, _d(u), os.fsencode(path)) != 0:
+        raise OSError(f"cannot write {path}")
 
 
 def solve(dim, p, pen, prob, tol=1e-14, tol_mode=0):

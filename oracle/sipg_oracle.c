@@ -6,6 +6,7 @@
 #include "sipg_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -434,6 +435,29 @@ void orc_err_norms(int dim, int p, const double *u, const orc_problem *pb, doubl
     out[1] = sqrt(L2);
     out[2] = sqrt(L2 + dxs + dys);
     free(D); free(xe); free(ye);
+}
+
+/* sipg_sem_2d.hpp:375-403 */
+int orc_print_result(int dim, int p, const double *u, const char *path)
+{
+    const int n1 = p + 1;
+    const long noe = (long)dim * dim;
+    double z[64], wq[64], nodes[64];
+    double *xe = (double *)malloc(sizeof(double) * noe);
+    double *ye = (double *)malloc(sizeof(double) * noe);
+    FILE *f = fopen(path, "w");
+    if (!f) { free(xe); free(ye); return -1; }
+    orc_gll(p, z, wq);
+    orc_mesh_coords(dim, xe, ye);
+    for (int i = 0; i <= p; ++i) nodes[i] = (z[i] * .5 + .5) / dim;
+    for (long e = 0; e < noe; ++e) {
+        for (int i = 0; i <= p; ++i)
+            for (int j = 0; j <= p; ++j)
+                fprintf(f, "%10g\t%10g\t%10g\n", xe[e] + nodes[i], ye[e] + nodes[j], u[((long)i * n1 + j) * noe + e]);
+        fprintf(f, "\n");
+    }
+    free(xe); free(ye);
+    return fclose(f) == 0 ? 0 : -1;
 }
 
 int orc_solve(int dim, int p, double pen, const orc_problem *pb, double tol,

@@ -97,6 +97,12 @@ int orc_cg(int dimx, int dimy, int p, double pen, const double *b, double *x,
  * out[0] = max error, out[1] = L2, out[2] = H1. */
 void orc_err_norms(int dim, int p, const double *u, const orc_problem *pb, double *out);
 
+/* print_result (sipg_sem_2d.hpp:375-403) into the file `path`: per element
+ * and node "x\ty\tu" with setw(10) at the stream's default precision (the
+ * printf %10g), an empty line after each element.  u in the reference
+ * order.  Returns 0, or -1 if the file cannot be written. */
+int orc_print_result(int dim, int p, const double *u, const char *path);
+
 /* Convenience: rhs + cg from zero + norms.  Returns iterations. */
 int orc_solve(int dim, int p, double pen, const orc_problem *pb, double tol,
               int tol_mode, double *u, double *norms_out);

@@ -327,3 +327,19 @@ def test_err_norms_of_the_interpolant(sipg):
         if dtype == sipg.F64:
             h1.append(got["H1"])
     assert 0 < h1[1] < 0.5 ** 4 * h1[0], h1
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_write_result_matches_oracle(sipg, oracle, tmp_path, dtype):
+    # sipg_write_result is print_result's text: byte for byte the oracle's
+    # file for the same values (the oracle's format is pinned on the CPU)
+    dim, p = 13, 4
+    op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F64 if dtype == "f64" else sipg.F32)
+    b, x = op.vector(), op.vector()
+    op.rhs_fourier(sipg.forcing_draw(2, 4, 4, 4), b)
+    x.zero()
+    op.cg(x, b, 1e-6 if dtype == "f32" else 1e-12, sipg.TOL_REL)
+    got, want = tmp_path / "gpu.txt", tmp_path / "oracle.txt"
+    op.write_result(x, str(got))
+    oracle.print_result(dim, p, op.download(x), str(want))
+    assert got.read_bytes() == want.read_bytes()
