@@ -42,12 +42,16 @@ SOLVERS = {
     "fp64": "FP64 CG with reliable updates (true residual every 300x drop, search direction kept) "
             "and a true-residual stop",
     "mixed": "mixed: FP32 block-Jacobi PCG cycles inside FP64 iterative refinement",
+    "pcg2": "FP64 PCG with a two-level preconditioner (the block-Jacobi blocks plus an exact coarse solve "
+            "on the continuous bilinear functions of the element vertices, by sine transforms) with "
+            "reliable updates (true residual every 10x drop, search direction kept) and a true-residual stop",
 }
 
 
 def gpu_solve(sipg, op, b, x, method):
     """One solve from x = 0 to a 1e-10 true relative residual; method is
-    "fp64" (CG), "pcg" (block-Jacobi PCG) or "mixed"."""
+    "fp64" (CG), "pcg" (block-Jacobi PCG), "pcg2" (two-level PCG) or
+    "mixed"."""
     x.zero()
     sipg.synchronize()
     t0 = time.perf_counter()
@@ -59,6 +63,11 @@ def gpu_solve(sipg, op, b, x, method):
     elif method == "pcg":
         st, info = op.pcg(x, b, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=3e-3,
                           true_stop=True, restart=0)
+    elif method == "pcg2":
+        # a true residual at every 10x drop: the coarse level's large early
+        # corrections leave the recursion ~1e-9 off b - A x otherwise
+        st, info = op.pcg2(x, b, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=0.1,
+                           true_stop=True, restart=0)
     else:
         # reliable updates: a true residual each time the recursive one has
         # fallen 300x, keeping the search direction; the finish is decided on
@@ -391,6 +400,27 @@ def main():
     log(f"pcg  : {msp:.1f} ms/solve, {itp[-1]} iterations, {infop.replacements} replacements, "
         f"{pcg['us_per_iteration']:.1f} us/iter, true relres {relp:.2e}, time vs CG {pcg['time_vs_cg']:.2f}")
 
+    # ---- two-level PCG (FP64) ---------------------------------------------
+    x2 = op.vector()
+    gpu_solve(sipg, op, b, x2, "pcg2")                            # warm-up (builds the coarse tables)
+    t2, it2 = [], []
+    for _ in range(args.steps):
+        t, info2 = gpu_solve(sipg, op, b, x2, "pcg2")
+        t2.append(t)
+        it2.append(info2.iters)
+    rel2 = op.true_relres(x2, b)
+    if not rel2 <= TOL:
+        raise RuntimeError(f"two-level PCG true relative residual {rel2:.3e} above {TOL}")
+    ms2 = 1e3 * float(np.median(t2))
+    pcg2 = {"dof_per_s": n / (ms2 * 1e-3), "ms_per_solve": ms2, "iterations": it2[-1],
+            "replacements": info2.replacements, "true_relres": rel2,
+            "us_per_iteration": 1e3 * ms2 / it2[-1],
+            "rel_l2_vs_cg_solution": float(np.linalg.norm(op.download(x2) - x64_h) / np.linalg.norm(x64_h)),
+            "time_vs_cg": ms2 / ms64, "time_vs_pcg": ms2 / msp}
+    log(f"pcg2 : {ms2:.1f} ms/solve, {it2[-1]} iterations, {info2.replacements} replacements, "
+        f"{pcg2['us_per_iteration']:.1f} us/iter, true relres {rel2:.2e}, time vs CG {pcg2['time_vs_cg']:.3f}, "
+        f"vs PCG {pcg2['time_vs_pcg']:.3f}")
+
     # ---- mixed-precision solver -------------------------------------------
     mixed = {"status": "not run"}
     xm = op.vector()
@@ -413,7 +443,7 @@ def main():
         mixed = {"status": f"failed: {exc}"}
         log(f"mixed: {exc}")
     # the timed solver: the fastest of the three
-    times = {"fp64": ms64, "pcg": msp}
+    times = {"fp64": ms64, "pcg": msp, "pcg2": ms2}
     if mixed.get("status") == "ok":
         times["mixed"] = mixed["ms_per_solve"]
     solver = min(times, key=times.get)
@@ -471,12 +501,13 @@ def main():
             "workload": "256x256 elements, p=8, 5,308,416 DoF, matrix-free SIPG CG to 1e-10 true "
                         "relative residual (FP64), pen=5(p+1)^2",
             "dim": DIM, "p": P, "dof": n, "pen": pen,
-            "solver": SOLVERS[solver] + " (the fastest of CG, PCG and mixed here)",
+            "solver": SOLVERS[solver] + " (the fastest of CG, PCG, two-level PCG and mixed here)",
             "iterations": it, "true_relres": relres,
             "error_vs_exact": {k: float(v) for k, v in err.items()},
             "fp64_plain": {"dof_per_s": n / (ms64 * 1e-3), "ms_per_solve": ms64,
                            "iterations": it64[-1], "true_relres": rel64},
             "pcg": pcg,
+            "pcg2": pcg2,
             "mixed": mixed,
             "apply_us_f64": ms_apply * 1e3,
             "batched": batched,

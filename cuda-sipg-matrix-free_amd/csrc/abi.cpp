@@ -183,6 +183,7 @@ int sipg_op_destroy(sipg_op *op) {
     if (op->h_state) (void)hipHostFree(op->h_state);
     if (op->ws) (void)hipFree(op->ws);
     if (op->d_prec) (void)hipFree(op->d_prec);
+    if (op->d_coarse) (void)hipFree(op->d_coarse);
     if (op->twin) sipg_op_destroy(op->twin);
     delete op;
     return SIPG_OK;

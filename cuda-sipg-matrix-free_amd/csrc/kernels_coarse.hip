@@ -1,0 +1,396 @@
+// kernels_coarse.hip -- the coarse level of the two-level preconditioner.
+//
+// M2^-1 r = M_BJ^-1 r + P0^T A0^-1 P0 r: the block-Jacobi preconditioner of
+// kernels_pcg.hip plus an exact solve on V0, the continuous bilinear (Q1)
+// functions on the element vertices that vanish on the boundary.  P0^T
+// interpolates vertex values bilinearly into every node of the adjacent
+// elements; P0 is its transpose.  A continuous function has no jumps, so on
+// V0 every face term of the SIPG form vanishes and A0 = P0 A P0^T is the Q1
+// stiffness matrix, with the 1-D factors K = tridiag(-1, 2, -1) and the GLL
+// quadrature mass M = tridiag(m1, 2 m0, m1) (exact for p >= 2: 1/6, 1/3;
+// lumped for p = 1): A0 = Kx (x) My + Mx (x) Ky on the (dimy-1) x (dimx-1)
+// interior vertices.  Both factors are diagonalised by the discrete sine
+// transform Q_jk = sqrt(2/dim) sin(j k pi / dim), so
+//
+//   A0^-1 S = Qy [ (Qy S Qx) ./ Lambda ] Qx,
+//   Lambda_ab = lamKy_a lamMx_b + lamMy_a lamKx_b
+//
+// four dense products of (dim-1)^2 matrices.  One coarse step per
+// preconditioner application:
+//   restrict  -- per element the four corner-weighted sums of r (E4)
+//   gather    -- S(a,b) = sum of the four elements meeting at vertex (a,b)
+//   4 x gemm  -- the DST solve; the last writes C into the zero-bordered
+//                vertex grid Cpad and the block partials of S.C
+// and the p update interpolates Cpad into every node (p = z + P0^T C + beta p)
+// with r.z = r.z_BJ + S.C.
+#include "sipg_internal.hpp"
+
+#include <algorithm>
+
+namespace sipg_internal {
+
+namespace {
+
+constexpr int VT = kVecThreads;
+// elements per restrict workgroup: their tiles fill at most 32 KiB of LDS
+template <int P>
+constexpr int rne() { return 4096 / ((P + 1) * (P + 1)) < 32 ? 4096 / ((P + 1) * (P + 1)) : 32; }
+constexpr int GT = 32;           // gemm output tile (GT x GT), k chunk GT
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// fixed-order sum over the workgroup of VT threads, returned to all
+__device__ __forceinline__ double vsum_all(double v) {
+    __shared__ double red[VT / 64];
+    __shared__ double total;
+    v = wsum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < VT / 64; ++i) s += red[i];
+        total = s;
+    }
+    __syncthreads();
+    return total;
+}
+
+__device__ __forceinline__ double vreduce(const double *__restrict__ part, int np) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += VT) s += part[i];
+    return vsum_all(s);
+}
+
+// E4[c * noe + e] = sum_nodes r(e, i1, i2) wx_c(i1) wy_c(i2); corner c = 0
+// bottom-left, 1 bottom-right, 2 top-left, 3 top-right; wx = 1 - xi or xi,
+// wy = 1 - xi or xi (xi = (z+1)/2: x from the element's left, y from its
+// bottom).
+template <typename T, int P>
+__global__ __launch_bounds__(VT) void restrict_kernel(const T *__restrict__ r, const double *__restrict__ xi,
+                                                      int64_t noe, double *__restrict__ E4) {
+    constexpr int N1 = P + 1, NQ = N1 * N1, RNE = rne<P>();
+    __shared__ double tile[RNE * NQ];
+    __shared__ double sx[N1];
+    if (threadIdx.x < N1) sx[threadIdx.x] = xi[threadIdx.x];
+    const int64_t e0 = (int64_t)blockIdx.x * RNE;
+    const int ne = (int)min<int64_t>(RNE, noe - e0);
+    const T *src = r + e0 * NQ;
+    for (int i = threadIdx.x; i < ne * NQ; i += VT) tile[i] = (double)src[i];
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 4 * ne) {
+        const int el = t >> 2, c = t & 3;
+        const double *u = tile + el * NQ;
+        double acc = 0.0;
+        for (int i1 = 0; i1 < N1; ++i1) {
+            double line = 0.0;
+#pragma unroll
+            for (int i2 = 0; i2 < N1; ++i2) line += u[i1 * N1 + i2] * ((c & 2) ? sx[i2] : 1.0 - sx[i2]);
+            acc += line * ((c & 1) ? sx[i1] : 1.0 - sx[i1]);
+        }
+        E4[(int64_t)c * noe + e0 + el] = acc;
+    }
+}
+
+// S[(a-1)*nvx + (b-1)] for interior vertex (a, b): vertex (a, b) sits at
+// x = b hx, y = 1 - a hy; element (row, col) has its top-left corner at
+// vertex (row, col).
+__global__ __launch_bounds__(VT) void gather_kernel(const double *__restrict__ E4, int dimx, int dimy,
+                                                    double *__restrict__ S) {
+    const int nvx = dimx - 1, nvy = dimy - 1;
+    const int64_t noe = (int64_t)dimx * dimy;
+    const int total = nvx * nvy;
+    for (int k = blockIdx.x * VT + threadIdx.x; k < total; k += gridDim.x * VT) {
+        const int a = k / nvx + 1, b = k % nvx + 1;
+        const double br = E4[1 * noe + (int64_t)(a - 1) * dimx + (b - 1)];   // element (a-1, b-1)
+        const double bl = E4[0 * noe + (int64_t)(a - 1) * dimx + b];         // element (a-1, b)
+        const double tr = E4[3 * noe + (int64_t)a * dimx + (b - 1)];         // element (a, b-1)
+        const double tl = E4[2 * noe + (int64_t)a * dimx + b];               // element (a, b)
+        S[k] = (br + bl) + (tr + tl);
+    }
+}
+
+// out = L (M x K) * R (K x N), row-major, tiled GT x GT with k chunks in LDS.
+//   MODE 0: out = L R
+//   MODE 1: out = (L R) .* W               (W: M x N, the 1/Lambda table)
+//   MODE 2: Cpad[(i+1)*(N+2) + j+1] = L R, and per-workgroup partials of
+//           sum (L R) .* W into part[blockIdx.y * gridDim.x + blockIdx.x]
+template <int MODE>
+__global__ __launch_bounds__(256) void coarse_gemm_kernel(const double *__restrict__ L, const double *__restrict__ R,
+                                                          int M, int K, int N, const double *__restrict__ W,
+                                                          double *__restrict__ out, double *__restrict__ part) {
+    __shared__ double As[GT][GT + 1];
+    __shared__ double Bs[GT][GT + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;       // 16 x 16 threads, 2 x 2 outputs each
+    const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int k0 = 0; k0 < K; k0 += GT) {
+        for (int i = threadIdx.x; i < GT * GT; i += 256) {
+            const int rr = i / GT, cc = i % GT;
+            const int gr = row0 + rr, gk = k0 + cc;
+            As[rr][cc] = (gr < M && gk < K) ? L[(int64_t)gr * K + gk] : 0.0;
+            const int kr = k0 + rr, gc = col0 + cc;
+            Bs[rr][cc] = (kr < K && gc < N) ? R[(int64_t)kr * N + gc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < GT; ++kk) {
+            const double a0 = As[ty][kk], a1 = As[ty + 16][kk];
+            const double b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
+            acc[0][0] += a0 * b0;
+            acc[0][1] += a0 * b1;
+            acc[1][0] += a1 * b0;
+            acc[1][1] += a1 * b1;
+        }
+        __syncthreads();
+    }
+    double dot = 0.0;
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            const int gr = row0 + ty + 16 * u, gc = col0 + tx + 16 * v;
+            if (gr >= M || gc >= N) continue;
+            const int64_t o = (int64_t)gr * N + gc;
+            if (MODE == 0) out[o] = acc[u][v];
+            if (MODE == 1) out[o] = acc[u][v] * W[o];
+            if (MODE == 2) {
+                out[(int64_t)(gr + 1) * (N + 2) + gc + 1] = acc[u][v];
+                dot += acc[u][v] * W[o];
+            }
+        }
+    if (MODE == 2) {
+        __shared__ double red[4];
+        dot = wsum(dot);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
+        __syncthreads();
+        if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    }
+}
+
+// bilinear interpolation of the zero-bordered vertex grid Cpad
+// ((dimy+1) x (dimx+1)) at node (i1, i2) of element e
+template <int P>
+__device__ __forceinline__ double prolong(const double *__restrict__ Cpad, const double *sx, int dimx, int e,
+                                          int i1, int i2) {
+    const int row = e / dimx, col = e - (e / dimx) * dimx;
+    const int ld = dimx + 1;
+    const double tl = Cpad[row * ld + col], tr = Cpad[row * ld + col + 1];
+    const double bl = Cpad[(row + 1) * ld + col], br = Cpad[(row + 1) * ld + col + 1];
+    const double x = sx[i1], y = sx[i2];
+    return (1.0 - y) * ((1.0 - x) * bl + x * br) + y * ((1.0 - x) * tl + x * tr);
+}
+
+// p += P0^T C (the start of a solve: p = z_BJ was written by the
+// preconditioner), or with z given: z += P0^T C (a plain application)
+template <typename T, int P>
+__global__ __launch_bounds__(VT) void add_coarse_kernel(T *__restrict__ v, const double *__restrict__ Cpad,
+                                                        const double *__restrict__ xi, int dimx, int64_t n) {
+    constexpr int N1 = P + 1, NQ = N1 * N1;
+    __shared__ double sx[N1];
+    if (threadIdx.x < N1) sx[threadIdx.x] = xi[threadIdx.x];
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < n; i += (int64_t)gridDim.x * VT) {
+        const int e = (int)(i / NQ), node = (int)(i - (int64_t)e * NQ);
+        const int i1 = node / N1, i2 = node - (node / N1) * N1;
+        v[i] = (T)((double)v[i] + prolong<P>(Cpad, sx, dimx, e, i1, i2));
+    }
+}
+
+// pcg_update_p_kernel with the coarse level: r.z = sum(prz) + sum(pc)
+// (the block-Jacobi part and S.C) and p = z + P0^T C + beta p.
+template <typename T, int P>
+__global__ __launch_bounds__(VT) void pcg2_update_p_kernel(T *__restrict__ x, T *__restrict__ p,
+                                                           const T *__restrict__ z, int64_t n,
+                                                           const double *__restrict__ ppAp, int np_pAp,
+                                                           const double *__restrict__ prr,
+                                                           const double *__restrict__ prz, int np,
+                                                           const double *__restrict__ pc, int npc,
+                                                           const double *__restrict__ Cpad,
+                                                           const double *__restrict__ xi, int dimx, cg_state *st,
+                                                           int slot) {
+    constexpr int N1 = P + 1, NQ = N1 * N1;
+    __shared__ double sx[N1];
+    const cg_slot cur = st->s[slot];
+    if (cur.done | st->err) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->s[slot ^ 1] = cur;   // carry forward
+        return;
+    }
+    if (threadIdx.x < N1) sx[threadIdx.x] = xi[threadIdx.x];
+    const double rr_new = vreduce(prr, np);
+    const double rz_new = vreduce(prz, np) + vreduce(pc, npc);
+    const double pAp = vreduce(ppAp, np_pAp);
+    const bool replaced = (cur.flags & kSlotReplaced) != 0;
+    const double beta_d = (cur.flags & kSlotRestart) ? 0.0 : rz_new / cur.rz;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const bool met = !(sqrt(rr_new) > st->stop);
+        const bool need_true = st->true_stop && !replaced;
+        cg_slot nx;
+        nx.rr = rr_new;
+        nx.alpha = 0.0;
+        nx.beta = beta_d;
+        nx.pAp = 0.0;
+        nx.rz = rz_new;
+        nx.it = cur.it + 1;
+        nx.done = ((met && !need_true) || nx.it >= st->maxit) ? 1 : 0;
+        nx.checked = replaced ? 1 : 0;
+        nx.flags = (met && need_true) ? kSlotWantCheck : 0;
+        st->s[slot ^ 1] = nx;
+        if (replaced) st->rr_ref = rr_new;
+        if (!isfinite(rr_new) || !isfinite(rz_new)) st->err = 3;
+        else if (!(rz_new > 0.0) && rr_new > 0.0) st->err = 2;   // M not positive on r
+    }
+    const T alpha = (T)(cur.rz / pAp);
+    const T beta = (T)beta_d;
+    for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < n; i += (int64_t)gridDim.x * VT) {
+        const int e = (int)(i / NQ), node = (int)(i - (int64_t)e * NQ);
+        const int i1 = node / N1, i2 = node - (node / N1) * N1;
+        const T pi = p[i];
+        if (!replaced) x[i] = x[i] + alpha * pi;
+        p[i] = (T)((double)z[i] + prolong<P>(Cpad, sx, dimx, e, i1, i2)) + beta * pi;
+    }
+}
+
+template <int P>
+struct CoarseFns {
+    template <typename T>
+    static void restrict_(const void *r, const double *xi, int64_t noe, double *E4, hipStream_t s) {
+        const int nb = (int)((noe + rne<P>() - 1) / rne<P>());
+        hipLaunchKernelGGL((restrict_kernel<T, P>), dim3(nb), dim3(VT), 0, s, (const T *)r, xi, noe, E4);
+    }
+    template <typename T>
+    static void add_(void *v, const double *Cpad, const double *xi, int dimx, int64_t n, int nblk, hipStream_t s) {
+        hipLaunchKernelGGL((add_coarse_kernel<T, P>), dim3(nblk), dim3(VT), 0, s, (T *)v, Cpad, xi, dimx, n);
+    }
+    template <typename T>
+    static void update_(void *x, void *p, const void *z, int64_t n, const double *ppAp, int np_pAp, const double *prr,
+                        const double *prz, int np, const double *pc, int npc, const double *Cpad, const double *xi,
+                        int dimx, int nblk, cg_state *st, int slot, hipStream_t s) {
+        hipLaunchKernelGGL((pcg2_update_p_kernel<T, P>), dim3(nblk), dim3(VT), 0, s, (T *)x, (T *)p, (const T *)z, n,
+                           ppAp, np_pAp, prr, prz, np, pc, npc, Cpad, xi, dimx, st, slot);
+    }
+};
+
+struct coarse_args {
+    const void *r = nullptr;
+    void *v = nullptr, *x = nullptr;
+    const void *z = nullptr;
+    const double *xi = nullptr, *Cpad = nullptr, *ppAp = nullptr, *prr = nullptr, *prz = nullptr, *pc = nullptr;
+    double *E4 = nullptr;
+    int64_t noe = 0, n = 0;
+    int dimx = 0, nblk = 0, np_pAp = 0, np = 0, npc = 0, slot = 0;
+    cg_state *st = nullptr;
+    hipStream_t s = nullptr;
+};
+
+template <int P>
+int dispatch_p(int p, int dtype, int what, const coarse_args &a) {
+    if (p != P) {
+        if constexpr (P < kMaxDegree) return dispatch_p<P + 1>(p, dtype, what, a);
+        else return SIPG_ERR_UNSUPPORTED;
+    }
+    using F = CoarseFns<P>;
+    const bool f64 = dtype == SIPG_F64;
+    if (what == 0) {
+        if (f64) F::template restrict_<double>(a.r, a.xi, a.noe, a.E4, a.s);
+        else F::template restrict_<float>(a.r, a.xi, a.noe, a.E4, a.s);
+    } else if (what == 1) {
+        if (f64) F::template add_<double>(a.v, a.Cpad, a.xi, a.dimx, a.n, a.nblk, a.s);
+        else F::template add_<float>(a.v, a.Cpad, a.xi, a.dimx, a.n, a.nblk, a.s);
+    } else {
+        if (f64)
+            F::template update_<double>(a.x, a.v, a.z, a.n, a.ppAp, a.np_pAp, a.prr, a.prz, a.np, a.pc, a.npc, a.Cpad,
+                                        a.xi, a.dimx, a.nblk, a.st, a.slot, a.s);
+        else
+            F::template update_<float>(a.x, a.v, a.z, a.n, a.ppAp, a.np_pAp, a.prr, a.prz, a.np, a.pc, a.npc, a.Cpad,
+                                       a.xi, a.dimx, a.nblk, a.st, a.slot, a.s);
+    }
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+dim3 gemm_grid(int M, int N) { return dim3((N + GT - 1) / GT, (M + GT - 1) / GT); }
+
+}  // namespace
+
+int coarse_num_partials(int dimx, int dimy) {
+    const int nvx = dimx - 1, nvy = dimy - 1;
+    if (nvx < 1 || nvy < 1) return 0;
+    const dim3 g = gemm_grid(nvy, nvx);
+    return (int)(g.x * g.y);
+}
+
+int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t s) {
+    const int nvx = dimx - 1, nvy = dimy - 1;
+    const int64_t noe = (int64_t)dimx * dimy;
+    if (nvx < 1 || nvy < 1) return SIPG_OK;          // no interior vertex: C = 0, no partials
+    coarse_args a{};
+    a.r = r;
+    a.xi = c.xi;
+    a.noe = noe;
+    a.E4 = c.E4;
+    a.s = s;
+    int rc = dispatch_p<1>(p, dtype, 0, a);
+    if (rc) return rc;
+    const int nt = nvx * nvy;
+    hipLaunchKernelGGL(gather_kernel, dim3(std::max(1, std::min(1024, (nt + VT - 1) / VT))), dim3(VT), 0, s, c.E4,
+                       dimx, dimy, c.S);
+    // T1 = S Qx; T2 = (Qy T1) ./ Lambda; T1 = T2 Qx; C = Qy T1 (into Cpad), partials of S.C
+    hipLaunchKernelGGL(coarse_gemm_kernel<0>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.S, c.Qx, nvy, nvx, nvx,
+                       nullptr, c.T1, nullptr);
+    hipLaunchKernelGGL(coarse_gemm_kernel<1>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.Qy, c.T1, nvy, nvy, nvx,
+                       c.Linv, c.T2, nullptr);
+    hipLaunchKernelGGL(coarse_gemm_kernel<0>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.T2, c.Qx, nvy, nvx, nvx,
+                       nullptr, c.T1, nullptr);
+    hipLaunchKernelGGL(coarse_gemm_kernel<2>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.Qy, c.T1, nvy, nvy, nvx, c.S,
+                       c.Cpad, c.part);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_coarse_add(int dtype, int p, void *v, int dimx, int64_t n, const coarse_tables &c, int nblk,
+                      hipStream_t s) {
+    coarse_args a{};
+    a.v = v;
+    a.Cpad = c.Cpad;
+    a.xi = c.xi;
+    a.dimx = dimx;
+    a.n = n;
+    a.nblk = nblk;
+    a.s = s;
+    return dispatch_p<1>(p, dtype, 1, a);
+}
+
+int launch_pcg2_update_p(int dtype, int p, void *x, void *pv, const void *z, int64_t n, const double *part_pAp,
+                         int np_pAp, const double *part_rr, const double *part_rz, int np, int npc, int dimx,
+                         const coarse_tables &c, int nblk, cg_state *st, int slot, hipStream_t s) {
+    coarse_args a{};
+    a.x = x;
+    a.v = pv;
+    a.z = z;
+    a.n = n;
+    a.ppAp = part_pAp;
+    a.np_pAp = np_pAp;
+    a.prr = part_rr;
+    a.prz = part_rz;
+    a.np = np;
+    a.pc = c.part;
+    a.npc = npc;
+    a.Cpad = c.Cpad;
+    a.xi = c.xi;
+    a.dimx = dimx;
+    a.nblk = nblk;
+    a.st = st;
+    a.slot = slot;
+    a.s = s;
+    return dispatch_p<1>(p, dtype, 2, a);
+}
+
+}  // namespace sipg_internal

@@ -398,9 +398,10 @@ __global__ __launch_bounds__(VT) void pcg_init_finalize_kernel(const double *__r
                                                                const double *__restrict__ prz, int np,
                                                                const double *__restrict__ pbb, int npb,
                                                                cg_state *st, double tol, int tol_mode,
-                                                               int maxit, int true_stop) {
+                                                               int maxit, int true_stop,
+                                                               const double *__restrict__ pc, int npc) {
     const double rr = vreduce(prr, np);
-    const double rz = vreduce(prz, np);
+    const double rz = vreduce(prz, np) + vreduce(pc, npc);     // + S.C of a coarse level
     const double bb = vreduce(pbb, npb);
     if (threadIdx.x == 0) {
         const double stop = tol_mode == SIPG_TOL_REL ? tol * sqrt(bb) : tol;
@@ -472,9 +473,9 @@ int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *par
 
 int launch_pcg_init_finalize(const double *part_rr, const double *part_rz, int np, const double *part_bb,
                              int npb, cg_state *st, double tol, int tol_mode, int maxit, int true_stop,
-                             hipStream_t s) {
+                             hipStream_t s, const double *part_c, int npc) {
     hipLaunchKernelGGL(pcg_init_finalize_kernel, dim3(1), dim3(VT), 0, s, part_rr, part_rz, np, part_bb, npb,
-                       st, tol, tol_mode, maxit, true_stop);
+                       st, tol, tol_mode, maxit, true_stop, part_c, npc);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

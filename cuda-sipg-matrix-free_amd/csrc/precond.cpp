@@ -16,6 +16,7 @@
 // symmetric matrix W^-1/2 K W^-1/2 with cyclic Jacobi rotations (n <= 17).
 #include "sipg_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -102,6 +103,35 @@ int build_block_jacobi(int p, double pen, std::vector<double> &tab) {
     for (int a = 0; a < n; ++a)
         for (int c = 0; c < n; ++c) tab[(size_t)n * n + (size_t)a * n + c] = 1.0 / (lam[a] + lam[c]);
     return SIPG_OK;
+}
+
+// Coarse level (kernels_coarse.hip): the sine transform of size n = dim - 1,
+// Q[j][k] = sqrt(2/dim) sin((j+1)(k+1) pi / dim) (symmetric, orthogonal),
+// and the eigenvalues of the 1-D Q1 stiffness tridiag(-1, 2, -1) and mass
+// tridiag(m1, 2 m0, m1) it diagonalises, the mass by the GLL quadrature of
+// degree p on [0, 1]: m0 = sum w/2 (1-xi)^2, m1 = sum w/2 xi (1-xi).
+void coarse_dst(int dim, std::vector<double> &Q, std::vector<double> &lamK, std::vector<double> &lamM, int p) {
+    const int n = dim - 1;
+    Q.assign((size_t)std::max(n, 0) * std::max(n, 0), 0.0);
+    lamK.assign(std::max(n, 0), 0.0);
+    lamM.assign(std::max(n, 0), 0.0);
+    if (n < 1) return;
+    std::vector<double> z(p + 1), w(p + 1);
+    gll(p, z.data(), w.data());
+    double m0 = 0.0, m1 = 0.0;
+    for (int q = 0; q <= p; ++q) {
+        const double xi = 0.5 * (z[q] + 1.0);
+        m0 += 0.5 * w[q] * (1.0 - xi) * (1.0 - xi);
+        m1 += 0.5 * w[q] * xi * (1.0 - xi);
+    }
+    const double pi = 3.14159265358979323846, sc = std::sqrt(2.0 / dim);
+    for (int j = 0; j < n; ++j)
+        for (int k = 0; k < n; ++k) Q[(size_t)j * n + k] = sc * std::sin((double)(j + 1) * (k + 1) * pi / dim);
+    for (int k = 0; k < n; ++k) {
+        const double c = std::cos((k + 1) * pi / dim);
+        lamK[k] = 2.0 - 2.0 * c;
+        lamM[k] = 2.0 * m0 + 2.0 * m1 * c;
+    }
 }
 
 }  // namespace sipg_internal

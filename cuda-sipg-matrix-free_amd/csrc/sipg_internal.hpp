@@ -186,7 +186,32 @@ int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *par
                  const cg_state *st, int slot, hipStream_t s);
 int launch_pcg_init_finalize(const double *part_rr, const double *part_rz, int np, const double *part_bb,
                              int npb, cg_state *st, double tol, int tol_mode, int maxit, int true_stop,
-                             hipStream_t s);
+                             hipStream_t s, const double *part_c = nullptr, int npc = 0);
+// ---- coarse level of the two-level preconditioner (kernels_coarse.hip) ----
+// Device pieces, in one allocation per op: DST matrices Qx ((dimx-1)^2),
+// Qy ((dimy-1)^2), 1/Lambda ((dimy-1) x (dimx-1)), the GLL positions
+// xi = (z+1)/2 (p+1), corner sums E4 (4 noe), S, T1, T2 ((dimy-1) x
+// (dimx-1)), the zero-bordered vertex grid Cpad ((dimy+1) x (dimx+1)) and
+// the S.C block partials.
+struct coarse_tables {
+    double *Qx = nullptr, *Qy = nullptr, *Linv = nullptr, *xi = nullptr, *E4 = nullptr, *S = nullptr,
+           *T1 = nullptr, *T2 = nullptr, *Cpad = nullptr, *part = nullptr;
+};
+int coarse_num_partials(int dimx, int dimy);
+// C = A0^-1 P0 r into Cpad, with the S.C partials (nothing when a side has
+// one element: no interior vertex)
+int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t s);
+// v += P0^T C
+int launch_coarse_add(int dtype, int p, void *v, int dimx, int64_t n, const coarse_tables &c, int nblk,
+                      hipStream_t s);
+// the PCG p update with the coarse level (r.z gains S.C, p gains P0^T C)
+int launch_pcg2_update_p(int dtype, int p, void *x, void *pv, const void *z, int64_t n, const double *part_pAp,
+                         int np_pAp, const double *part_rr, const double *part_rz, int np, int npc, int dimx,
+                         const coarse_tables &c, int nblk, cg_state *st, int slot, hipStream_t s);
+// host tables of the coarse solve: Q (n x n, n = dim - 1), and the 1-D
+// eigenvalues of K and of the GLL-quadrature Q1 mass
+void coarse_dst(int dim, std::vector<double> &Q, std::vector<double> &lamK, std::vector<double> &lamM, int p);
+
 // ---- block-Jacobi tables (precond.cpp) ------------------------------------
 // [S (row-major, S[i][a]), Linv[a][c] = 1/(lambda_a + lambda_c)] of the
 // interior element block K U W + W U K, K = K1 + own face terms (see
@@ -270,6 +295,8 @@ struct sipg_op {
     sipg_op *twin = nullptr;
     // block-Jacobi tables in the op's dtype, built on first use
     void *d_prec = nullptr;
+    void *d_coarse = nullptr;         // the coarse level's tables and work arrays (one allocation)
+    sipg_internal::coarse_tables coarse;   // pointers into d_coarse
 };
 
 namespace sipg_internal {

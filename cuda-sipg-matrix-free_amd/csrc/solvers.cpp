@@ -45,6 +45,9 @@ struct Engine {
     bool pcg = false;
     void *z = nullptr;
     const void *prec = nullptr;
+    // two-level: the coarse level of kernels_coarse.hip on top of the blocks
+    const coarse_tables *coarse = nullptr;
+    int ncp = 0;                     // its S.C partials
 
     hipStream_t s = nullptr;
     cg_state *st = nullptr;
@@ -71,8 +74,13 @@ struct Engine {
             if ((rc = launch_pcg_rz(opI->dtype, opI->p, opI->pen, 2, r, nullptr, z, p, noe(), prec, nullptr, 0, P.rr, P.rz,
                                     nullptr, 0, s)))
                 return rc;
+            if (coarse) {      // p = z_BJ + P0^T A0^-1 P0 r; r.z gains S.C
+                if ((rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s)) ||
+                    (rc = launch_coarse_add(opI->dtype, opI->p, p, opI->dimx, opI->n, *coarse, nb, s)))
+                    return rc;
+            }
             return launch_pcg_init_finalize(P.rr, P.rz, npc(2), P.bb, nb, st, tol, tol_mode, maxit,
-                                            true_stop ? 1 : 0, s);
+                                            true_stop ? 1 : 0, s, coarse ? coarse->part : nullptr, ncp);
         }
         if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, p, opI->n, P.rr, P.bb, nb, s)))
             return rc;
@@ -107,6 +115,8 @@ struct Engine {
         if ((rc = launch_pcg_rz(opI->dtype, opI->p, opI->pen, 1, r, Ap, z, nullptr, noe(), prec, src, np, P.rr, P.rz, st,
                                 slot, s)))
             return rc;
+        if (coarse && !replace && (rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s)))
+            return rc;
         int nrz = npc(1);
         if (replace) {
             if ((rc = launch_pcg_x(opI->dtype, xI, p, opI->n, src, np, nb, st, slot, s))) return rc;
@@ -120,11 +130,17 @@ struct Engine {
             if ((rc = launch_pcg_rz(opI->dtype, opI->p, opI->pen, 0, r, nullptr, z, nullptr, noe(), prec, nullptr, 0, P.rr,
                                     P.rz, nullptr, 0, s)))
                 return rc;
+            if (coarse && (rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s)))
+                return rc;
             nrz = npc(0);
             if ((rc = launch_mark_replaced(st, slot, restart_p ? 1 : 0, s))) return rc;
         }
-        if ((rc = launch_pcg_update_p(opI->dtype, xI, p, z, opI->n, src, np, P.rr, P.rz, nrz, nb, st, slot, s)))
-            return rc;
+        if (coarse)
+            rc = launch_pcg2_update_p(opI->dtype, opI->p, xI, p, z, opI->n, src, np, P.rr, P.rz, nrz, ncp, opI->dimx,
+                                      *coarse, nb, st, slot, s);
+        else
+            rc = launch_pcg_update_p(opI->dtype, xI, p, z, opI->n, src, np, P.rr, P.rz, nrz, nb, st, slot, s);
+        if (rc) return rc;
         ++launched;
         return SIPG_OK;
     }
@@ -294,13 +310,59 @@ int ensure_prec(sipg_op *op) {
     return SIPG_OK;
 }
 
+// Coarse-level tables and work arrays of the op (FP64), built once.
+int ensure_coarse(sipg_op *op) {
+    if (op->d_coarse) return SIPG_OK;
+    const int dx = op->dimx, dy = op->dimy, n1 = op->p + 1;
+    const int nvx = std::max(dx - 1, 0), nvy = std::max(dy - 1, 0);
+    const int64_t noe = (int64_t)dx * dy, nv = (int64_t)nvx * nvy;
+    std::vector<double> Qx, Qy, kx, mx, ky, my;
+    coarse_dst(dx, Qx, kx, mx, op->p);
+    coarse_dst(dy, Qy, ky, my, op->p);
+    std::vector<double> linv((size_t)nv);
+    for (int a = 0; a < nvy; ++a)
+        for (int b = 0; b < nvx; ++b) linv[(size_t)a * nvx + b] = 1.0 / (my[a] * kx[b] + ky[a] * mx[b]);
+    std::vector<double> xi(n1), w(n1);
+    gll(op->p, xi.data(), w.data());
+    for (double &v : xi) v = 0.5 * (v + 1.0);
+    const int npart = std::max(coarse_num_partials(dx, dy), 1);
+    // Qx, Qy, Linv, xi, E4, S, T1, T2, Cpad, part
+    const size_t len[10] = {(size_t)nvx * nvx, (size_t)nvy * nvy, (size_t)nv, (size_t)n1, (size_t)(4 * noe),
+                            (size_t)nv, (size_t)nv, (size_t)nv, (size_t)(dx + 1) * (dy + 1), (size_t)npart};
+    size_t total = 0;
+    for (size_t l : len) total += (l + 31) & ~size_t(31);
+    double *d = nullptr;
+    SIPG_HIP_TRY(hipMalloc(&d, total * sizeof(double)));
+    coarse_tables &c = op->coarse;
+    double **slots[10] = {&c.Qx, &c.Qy, &c.Linv, &c.xi, &c.E4, &c.S, &c.T1, &c.T2, &c.Cpad, &c.part};
+    size_t o = 0;
+    for (int k = 0; k < 10; ++k) {
+        *slots[k] = d + o;
+        o += (len[k] + 31) & ~size_t(31);
+    }
+    hipError_t e = hipSuccess;
+    const std::vector<double> *src[4] = {&Qx, &Qy, &linv, &xi};
+    for (int k = 0; k < 4 && e == hipSuccess; ++k)
+        if (!src[k]->empty()) e = hipMemcpy(*slots[k], src[k]->data(), src[k]->size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(c.Cpad, 0, len[8] * sizeof(double));
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        c = coarse_tables{};
+        set_last_hip_error(e);
+        return SIPG_ERR_HIP;
+    }
+    op->d_coarse = d;
+    return SIPG_OK;
+}
+
 int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg_cg_info *info,
-                bool pcg = false) {
+                bool pcg = false, bool two_level = false) {
     int rc = check_opts(o);
     if (rc) return rc;
     const size_t vb = (size_t)op->n * dtype_size(op->dtype);
     if (overlaps(x, b, vb)) return SIPG_ERR_ALIAS;
     if (pcg && (rc = ensure_prec(op))) return rc;
+    if (two_level && (rc = ensure_coarse(op))) return rc;
     // With true residuals the iterate is group-updated (van der Vorst & Ye
     // 2000): the CG increments go to a separate vector z that is folded
     // into x only when r is recomputed, so the rounding of x += alpha p
@@ -320,6 +382,8 @@ int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg
     E.pcg = pcg;
     E.z = pcg ? ws + (size_t)(nvec - 1) * vb : nullptr;
     E.prec = op->d_prec;
+    E.coarse = two_level ? &op->coarse : nullptr;
+    E.ncp = two_level ? coarse_num_partials(op->dimx, op->dimy) : 0;
     E.AxO = E.Ap;        // the initial and the replacement A x use the Ap slot
     E.b = b;
     E.replace_factor = o->replace_factor;
@@ -420,6 +484,25 @@ int sipg_prec_apply(sipg_op *op, const void *r, void *z) {
     parts P = part_layout(op);
     return launch_pcg_rz(op->dtype, op->p, op->pen, 0, const_cast<void *>(r), nullptr, z, nullptr,
                          (int64_t)op->dimx * op->dimy, op->d_prec, nullptr, 0, P.rr, P.rz, nullptr, 0, op->stream);
+}
+
+int sipg_pcg2(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {
+    if (!op || !x || !b) return SIPG_ERR_INVALID;
+    return solve_plain(op, x, b, opts, info, true, true);
+}
+
+int sipg_prec2_apply(sipg_op *op, const void *r, void *z) {
+    if (!op || !r || !z) return SIPG_ERR_INVALID;
+    const size_t vb = (size_t)op->n * dtype_size(op->dtype);
+    if (overlaps(r, z, vb)) return SIPG_ERR_ALIAS;
+    int rc = ensure_prec(op);
+    if (rc || (rc = ensure_coarse(op))) return rc;
+    parts P = part_layout(op);
+    if ((rc = launch_pcg_rz(op->dtype, op->p, op->pen, 0, const_cast<void *>(r), nullptr, z, nullptr,
+                            (int64_t)op->dimx * op->dimy, op->d_prec, nullptr, 0, P.rr, P.rz, nullptr, 0, op->stream)) ||
+        (rc = launch_coarse_solve(op->dtype, op->p, r, op->dimx, op->dimy, op->coarse, op->stream)))
+        return rc;
+    return launch_coarse_add(op->dtype, op->p, z, op->dimx, op->n, op->coarse, vec_num_blocks(op->n), op->stream);
 }
 
 int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {

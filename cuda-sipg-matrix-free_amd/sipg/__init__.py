@@ -97,6 +97,8 @@ def _declare(L):
         "sipg_pcg": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
         "sipg_pcg_mixed": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
         "sipg_prec_apply": (i, [_vp, _vp, _vp]),
+        "sipg_prec2_apply": (i, [_vp, _vp, _vp]),
+        "sipg_pcg2": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
         "sipg_vec_dot": (i, [i, i64, _vp, _vp, _dp, _vp]),
         "sipg_vec_axpy": (i, [i, i64, d, _vp, _vp, _vp]),
         "sipg_vec_copy": (i, [i, i64, _vp, _vp, _vp]),
@@ -306,6 +308,17 @@ class Operator:
         info = CgInfo()
         st = lib().sipg_pcg(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
         return st, info
+
+    def pcg2(self, x, b, tol, tol_mode=TOL_REL, maxit=0, replace_factor=0.0, true_stop=True, restart=0):
+        """PCG with the two-level preconditioner (sipg_pcg2); options as pcg."""
+        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1 if true_stop else 0, int(restart))
+        info = CgInfo()
+        st = lib().sipg_pcg2(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
+        return st, info
+
+    def prec2_apply(self, r, z):
+        """z = M2^-1 r with the two-level preconditioner of pcg2."""
+        check(lib().sipg_prec2_apply(self.h, r.ptr, z.ptr), "prec2_apply")
 
     def pcg_mixed(self, x, b, tol, tol_mode=TOL_REL, maxit=0, replace_factor=0.0, restart=True):
         """FP32 block-Jacobi PCG cycles inside FP64 refinement (sipg_pcg_mixed)."""

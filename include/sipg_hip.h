@@ -203,6 +203,20 @@ int sipg_pcg(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg
  * (FP32 tables of the op's FP32 twin).  op must be SIPG_F64. */
 int sipg_pcg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
 
+/* PCG with a two-level preconditioner: the block-Jacobi blocks of sipg_pcg
+ * plus an exact coarse solve on the continuous bilinear functions of the
+ * element vertices (zero on the boundary),
+ *   M2^-1 r = M_BJ^-1 r + P0^T A0^-1 P0 r,
+ * A0 = P0 A P0^T (the Q1 stiffness: on continuous functions every face term
+ * of the SIPG form vanishes), inverted by discrete sine transforms.  The
+ * method is preconditioned_conjugate_gradient (conjugate_gradient_gpu.hpp:
+ * 116-213); the coarse level has no counterpart in the reference, whose
+ * preconditioner is block-diagonal only.  The iteration count no longer
+ * grows with the mesh.  Options and results as sipg_pcg. */
+int sipg_pcg2(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
+/* z = M2^-1 r (two-level, as sipg_pcg2), asynchronous on the op's stream. */
+int sipg_prec2_apply(sipg_op *op, const void *r, void *z);
+
 /* z = M^-1 r with the preconditioner of sipg_pcg (device vectors of the
  * op's dtype, r and z distinct); replaces _prec_mvm (abs_mvm.hpp:15). */
 int sipg_prec_apply(sipg_op *op, const void *r, void *z);

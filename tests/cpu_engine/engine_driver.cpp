@@ -9,6 +9,72 @@
 
 extern "C" int orc_cg(int, int, int, double, const double *, double *, double, int, int, double *);
 extern "C" void orc_apply(int, int, int, double, const double *, double *);
+extern "C" void orc_gll(int, double *, double *);
+
+namespace sipg_internal {
+void coarse_dst(int dim, std::vector<double> &Q, std::vector<double> &lamK, std::vector<double> &lamM, int p);
+}
+
+// The coarse operator from first principles against the DST tables:
+// A0 = P0 A P0^T with A the oracle's operator and P0^T the bilinear
+// interpolation of interior vertex values at the GLL nodes (reference order,
+// element e = row * dimx + col, row 0 at the top, i1 the x node, i2 the y
+// node), compared with Qy diag(lamMy lamKx + lamKy lamMx) Qx applied to unit
+// vectors.  Prints the largest relative difference.
+static void coarse_check(int dx, int dy, int p) {
+    const int n1 = p + 1, nvx = dx - 1, nvy = dy - 1, nv = nvx * nvy;
+    const int64_t noe = (int64_t)dx * dy, n = noe * n1 * n1;
+    std::vector<double> z(n1), w(n1);
+    orc_gll(p, z.data(), w.data());
+    std::vector<double> xi(n1);
+    for (int i = 0; i < n1; ++i) xi[i] = 0.5 * (z[i] + 1.0);
+    auto interp = [&](const std::vector<double> &V, std::vector<double> &u) {   // V on (dy+1) x (dx+1)
+        u.assign(n, 0.0);
+        for (int row = 0; row < dy; ++row)
+            for (int col = 0; col < dx; ++col)
+                for (int i1 = 0; i1 < n1; ++i1)
+                    for (int i2 = 0; i2 < n1; ++i2) {
+                        const double X = xi[i1], Y = xi[i2];   // from the left, from the bottom
+                        const double tl = V[(size_t)row * (dx + 1) + col], tr = V[(size_t)row * (dx + 1) + col + 1];
+                        const double bl = V[(size_t)(row + 1) * (dx + 1) + col],
+                                     br = V[(size_t)(row + 1) * (dx + 1) + col + 1];
+                        u[((int64_t)i1 * n1 + i2) * noe + (int64_t)row * dx + col] =
+                            (1 - X) * (1 - Y) * bl + X * (1 - Y) * br + (1 - X) * Y * tl + X * Y * tr;
+                    }
+    };
+    std::vector<double> A0((size_t)nv * nv), V, u, Au(n), u2;
+    for (int j = 0; j < nv; ++j) {
+        V.assign((size_t)(dx + 1) * (dy + 1), 0.0);
+        V[(size_t)(j / nvx + 1) * (dx + 1) + j % nvx + 1] = 1.0;
+        interp(V, u);
+        orc_apply(dx, dy, p, 1e3, u.data(), Au.data());    // the penalty drops out on V0
+        for (int i = 0; i < nv; ++i) {
+            V.assign((size_t)(dx + 1) * (dy + 1), 0.0);
+            V[(size_t)(i / nvx + 1) * (dx + 1) + i % nvx + 1] = 1.0;
+            interp(V, u2);
+            double s = 0.0;
+            for (int64_t k = 0; k < n; ++k) s += u2[k] * Au[k];
+            A0[(size_t)i * nv + j] = s;
+        }
+    }
+    std::vector<double> Qx, Qy, kx, mx, ky, my;
+    sipg_internal::coarse_dst(dx, Qx, kx, mx, p);
+    sipg_internal::coarse_dst(dy, Qy, ky, my, p);
+    double worst = 0.0, scale = 0.0;
+    for (double v : A0) scale = std::fmax(scale, std::fabs(v));
+    for (int i = 0; i < nv; ++i)
+        for (int j = 0; j < nv; ++j) {
+            const int ai = i / nvx, bi = i % nvx, aj = j / nvx, bj = j % nvx;
+            double s = 0.0;
+            for (int a = 0; a < nvy; ++a)
+                for (int b = 0; b < nvx; ++b)
+                    s += Qy[(size_t)ai * nvy + a] * Qx[(size_t)bi * nvx + b] * (my[a] * kx[b] + ky[a] * mx[b]) *
+                         Qy[(size_t)a * nvy + aj] * Qx[(size_t)b * nvx + bj];
+            worst = std::fmax(worst, std::fabs(s - A0[(size_t)i * nv + j]));
+        }
+    printf("{\"case\":\"coarse_tables_%dx%d_p%d\",\"rel\":%.3e,\"center\":%.6f}\n", dx, dy, p, worst / scale,
+           A0[(size_t)(nv / 2) * nv + nv / 2]);
+}
 
 static std::vector<double> rhs(int64_t n, unsigned seed) {
     std::vector<double> b(n);
@@ -55,6 +121,8 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
         rc = sipg_cg_mixed(op, x, b, &o, &info);
     } else if (mode == 3) {
         rc = sipg_pcg(op, x, b, &o, &info);
+    } else if (mode == 5) {
+        rc = sipg_pcg2(op, x, b, &o, &info);
     } else {
         rc = sipg_pcg_mixed(op, x, b, &o, &info);
     }
@@ -86,5 +154,13 @@ int main() {
     run("pcg_other_pen", 8, 4, 100.0, 3, 1e-10, 0, 0.1, 0);
     run("pcg_cap", 8, 4, 125.0, 3, 1e-12, 5, 0.0, 0, 0);
     run("pcg_mixed", 8, 4, 125.0, 4, 1e-10, 0, 0.1);
+    run("pcg2_replace", 8, 4, 125.0, 5, 1e-10, 0, 0.1, 0);
+    run("pcg2_true_stop", 12, 3, 80.0, 5, 1e-10, 0, 0.0, 0);
+    run("pcg2_plain", 12, 3, 80.0, 5, 1e-10, 0, 0.0, 0, 0);
+    run("pcg2_vs_pcg", 12, 3, 80.0, 3, 1e-10, 0, 0.0, 0, 0);
+    run("pcg2_one_element", 1, 3, 80.0, 5, 1e-10, 0, 0.0, 0, 0);
+    coarse_check(5, 5, 3);
+    coarse_check(6, 4, 2);
+    coarse_check(4, 5, 1);
     return 0;
 }

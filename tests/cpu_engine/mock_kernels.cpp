@@ -420,6 +420,90 @@ int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, c
     return SIPG_OK;
 }
 
+// ---- coarse level (kernels_coarse.hip), FP64: the same DST solve in loops -
+int coarse_num_partials(int dimx, int dimy) { return (dimx > 1 && dimy > 1) ? 1 : 0; }
+
+static double corner_weight(int c, double xa, double yb) {
+    return ((c & 1) ? xa : 1.0 - xa) * ((c & 2) ? yb : 1.0 - yb);
+}
+
+int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t) {
+    if (dtype != SIPG_F64) return SIPG_ERR_UNSUPPORTED;
+    const int nvx = dimx - 1, nvy = dimy - 1, n1 = p + 1, nq = n1 * n1;
+    if (nvx < 1 || nvy < 1) return SIPG_OK;
+    const double *R = tp<const double>(r);
+    std::vector<double> S((size_t)nvx * nvy, 0.0);
+    // P0 r: each element adds its corner-weighted sums to its interior corners
+    for (int row = 0; row < dimy; ++row)
+        for (int col = 0; col < dimx; ++col) {
+            const int64_t e = (int64_t)row * dimx + col;
+            for (int cc = 0; cc < 4; ++cc) {
+                const int a = row + ((cc & 2) ? 0 : 1), b = col + ((cc & 1) ? 1 : 0);   // vertex of corner cc
+                if (a < 1 || a > nvy || b < 1 || b > nvx) continue;
+                double acc = 0.0;
+                for (int i1 = 0; i1 < n1; ++i1)
+                    for (int i2 = 0; i2 < n1; ++i2)
+                        acc += R[e * nq + i1 * n1 + i2] * corner_weight(cc, c.xi[i1], c.xi[i2]);
+                S[(size_t)(a - 1) * nvx + (b - 1)] += acc;
+            }
+        }
+    auto mul = [](const double *L, const double *Rm, int M, int K, int N, double *out) {
+        for (int i = 0; i < M; ++i)
+            for (int j = 0; j < N; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < K; ++k) acc += L[(size_t)i * K + k] * Rm[(size_t)k * N + j];
+                out[(size_t)i * N + j] = acc;
+            }
+    };
+    std::vector<double> T1(S.size()), T2(S.size()), C(S.size());
+    mul(S.data(), c.Qx, nvy, nvx, nvx, T1.data());
+    mul(c.Qy, T1.data(), nvy, nvy, nvx, T2.data());
+    for (size_t k = 0; k < T2.size(); ++k) T2[k] *= c.Linv[k];
+    mul(T2.data(), c.Qx, nvy, nvx, nvx, T1.data());
+    mul(c.Qy, T1.data(), nvy, nvy, nvx, C.data());
+    double dot = 0.0;
+    for (int a = 0; a < nvy; ++a)
+        for (int b = 0; b < nvx; ++b) {
+            c.Cpad[(size_t)(a + 1) * (dimx + 1) + b + 1] = C[(size_t)a * nvx + b];
+            dot += C[(size_t)a * nvx + b] * S[(size_t)a * nvx + b];
+        }
+    c.part[0] = dot;
+    return SIPG_OK;
+}
+
+static double mock_prolong(const coarse_tables &c, int dimx, int n1, int64_t idx) {
+    const int nq = n1 * n1;
+    const int64_t e = idx / nq;
+    const int node = (int)(idx - e * nq), i1 = node / n1, i2 = node % n1;
+    const int row = (int)(e / dimx), col = (int)(e % dimx), ld = dimx + 1;
+    double v = 0.0;
+    for (int cc = 0; cc < 4; ++cc) {
+        const int a = row + ((cc & 2) ? 0 : 1), b = col + ((cc & 1) ? 1 : 0);
+        v += c.Cpad[(size_t)a * ld + b] * corner_weight(cc, c.xi[i1], c.xi[i2]);
+    }
+    return v;
+}
+
+int launch_coarse_add(int dtype, int p, void *v, int dimx, int64_t n, const coarse_tables &c, int, hipStream_t) {
+    if (dtype != SIPG_F64) return SIPG_ERR_UNSUPPORTED;
+    for (int64_t i = 0; i < n; ++i) tp<double>(v)[i] += mock_prolong(c, dimx, p + 1, i);
+    return SIPG_OK;
+}
+
+int launch_pcg2_update_p(int dtype, int p, void *x, void *pv, const void *z, int64_t n, const double *part_pAp,
+                         int np_pAp, const double *part_rr, const double *part_rz, int np, int npc, int dimx,
+                         const coarse_tables &c, int, cg_state *st, int slot, hipStream_t) {
+    if (dtype != SIPG_F64) return SIPG_ERR_UNSUPPORTED;
+    // z + P0^T C through a scratch copy, then the plain update with r.z + S.C
+    std::vector<double> zc(tp<const double>(z), tp<const double>(z) + n);
+    for (int64_t i = 0; i < n; ++i) zc[i] += mock_prolong(c, dimx, p + 1, i);
+    std::vector<double> prz(part_rz, part_rz + np);
+    if (np > 0) prz[np - 1] += sum_partials(c.part, npc);
+    pcg_update_p_t<double>(tp<double>(x), tp<double>(pv), zc.data(), n, part_pAp, np_pAp, part_rr, prz.data(), np,
+                           st, slot);
+    return SIPG_OK;
+}
+
 int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *part_pAp, int np_pAp, int,
                  const cg_state *st, int slot, hipStream_t) {
     if (st->s[slot].done | st->err) return SIPG_OK;
@@ -432,8 +516,10 @@ int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *par
 }
 
 int launch_pcg_init_finalize(const double *prr, const double *prz, int np, const double *pbb, int npb,
-                             cg_state *st, double tol, int tol_mode, int maxit, int true_stop, hipStream_t) {
-    const double rr = sum_partials(prr, np), rz = sum_partials(prz, np), bb = sum_partials(pbb, npb);
+                             cg_state *st, double tol, int tol_mode, int maxit, int true_stop, hipStream_t,
+                             const double *pc, int npc) {
+    const double rr = sum_partials(prr, np), rz = sum_partials(prz, np) + sum_partials(pc, npc),
+                 bb = sum_partials(pbb, npb);
     const double stop = tol_mode == SIPG_TOL_REL ? tol * std::sqrt(bb) : tol;
     cg_slot s0{};
     s0.rr = rr;

@@ -84,6 +84,27 @@ def test_pcg_reaches_true_residual_in_fewer_iterations(results, case):
     assert r["iters"] < 0.8 * r["oracle_iters"], r
 
 
+@pytest.mark.parametrize("case", ["coarse_tables_5x5_p3", "coarse_tables_6x4_p2", "coarse_tables_4x5_p1"])
+def test_coarse_tables_reproduce_the_galerkin_coarse_operator(results, case):
+    # the sine-transform tables of the two-level preconditioner against
+    # P0 A P0^T built from the oracle's operator and bilinear interpolation
+    # (square, rectangular, and p = 1 with its lumped GLL mass)
+    r = results[case]
+    assert r["rel"] < 1e-13, r
+    assert abs(r["center"] - (4.0 if case.endswith("p1") else 8.0 / 3.0)) < 1e-6, r      # printed to 6 places
+
+
+@pytest.mark.parametrize("case", ["pcg2_replace", "pcg2_true_stop", "pcg2_plain", "pcg2_one_element"])
+def test_pcg2_converges_to_the_oracle_solution(results, case):
+    r = results[case]
+    assert r["rc"] == 0 and r["relres"] <= 1e-10 and r["diff"] < 1e-9, r
+
+
+def test_pcg2_needs_far_fewer_iterations(results):
+    two, one = results["pcg2_plain"], results["pcg2_vs_pcg"]
+    assert two["iters"] < 0.4 * one["iters"], (two, one)
+
+
 def test_pcg_plain_stops_on_the_recursion(results):
     r = results["pcg_plain"]
     assert r["rc"] == 0 and r["reps"] == 0 and r["relres"] <= 2e-10, r

@@ -148,3 +148,106 @@ def test_pcg_rejects_aliasing_and_bad_penalty(sipg):
     r, z = bad.vector(), bad.vector()
     with pytest.raises(Exception):
         bad.prec_apply(r, z)
+
+
+def _interp_matrix(oracle, dx, dy, p):
+    """P0^T: bilinear interpolation of the interior vertex values (row-major
+    (dy-1) x (dx-1), vertex (a, b) at x = b/dx, y = 1 - a/dy) at the GLL
+    nodes, in the reference order (i1 x node, i2 y node, e = row*dx + col,
+    row 0 at the top)."""
+    n1, noe = p + 1, dx * dy
+    z, _ = oracle.gll(p)
+    xi = (z + 1) / 2
+    nvx, nvy = dx - 1, dy - 1
+    P = np.zeros((n1 * n1 * noe, nvx * nvy))
+    for row in range(dy):
+        for col in range(dx):
+            e = row * dx + col
+            corners = {(row, col): lambda X, Y: (1 - X) * Y, (row, col + 1): lambda X, Y: X * Y,
+                       (row + 1, col): lambda X, Y: (1 - X) * (1 - Y), (row + 1, col + 1): lambda X, Y: X * (1 - Y)}
+            for (a, b), f in corners.items():
+                if not (1 <= a <= nvy and 1 <= b <= nvx):
+                    continue
+                k = (a - 1) * nvx + (b - 1)
+                for i1 in range(n1):
+                    for i2 in range(n1):
+                        P[(i1 * n1 + i2) * noe + e, k] += f(xi[i1], xi[i2])
+    return P
+
+
+@pytest.mark.parametrize("dx,dy,p", [(5, 4, 3), (6, 6, 8), (3, 7, 1), (4, 4, 14)])
+def test_prec2_adds_the_galerkin_coarse_solve(sipg, oracle, dx, dy, p):
+    # M2^-1 r - M_BJ^-1 r = P0^T (P0 A P0^T)^-1 P0 r, with A the oracle's
+    # operator: pins the sine-transform tables, the restriction, the
+    # vertex gather and the interpolation together
+    pen = sipg.pen_sigma(p)
+    op = sipg.Operator(dx, dy, p, pen, sipg.F64)
+    P = _interp_matrix(oracle, dx, dy, p)
+    AP = np.stack([oracle.apply(dx, dy, p, pen, P[:, k]) for k in range(P.shape[1])], axis=1)
+    A0 = P.T @ AP
+    r = np.random.default_rng(dx * dy + p).uniform(-1, 1, op.n)
+    rd, z1, z2 = op.vector(), op.vector(), op.vector()
+    op.upload(r, rd)
+    op.prec_apply(rd, z1)
+    op.prec2_apply(rd, z2)
+    coarse = op.download(z2) - op.download(z1)
+    want = P @ np.linalg.solve(A0, P.T @ r)
+    assert np.max(np.abs(coarse - want)) <= 1e-11 * np.max(np.abs(want)), np.max(np.abs(coarse - want))
+
+
+def test_prec2_fp32_matches_fp64_and_is_spd(sipg):
+    dim, p = 12, 5
+    ops = {t: sipg.Operator(dim, dim, p, sipg.pen_sigma(p), t) for t in (sipg.F64, sipg.F32)}
+    rng = np.random.default_rng(21)
+    a, b = rng.uniform(-1, 1, ops[sipg.F64].n), rng.uniform(-1, 1, ops[sipg.F64].n)
+    out = {}
+    for t, op in ops.items():
+        res = []
+        for v in (a, b):
+            vd, zd = op.vector(), op.vector()
+            op.upload(v.astype(np.float32) if t == sipg.F32 else v, vd)
+            op.prec2_apply(vd, zd)
+            res.append(op.download(zd))
+        out[t] = res
+    Ma, Mb = out[sipg.F64]
+    assert abs(a @ Mb - b @ Ma) <= 1e-12 * abs(a @ Mb) and a @ Ma > 0
+    assert np.max(np.abs(out[sipg.F32][0] - Ma)) <= 1e-5 * np.max(np.abs(Ma))
+
+
+@pytest.mark.parametrize("dx,dy,p", [(8, 8, 3), (16, 16, 8), (10, 6, 12), (1, 1, 4), (5, 1, 2)])
+def test_pcg2_matches_the_oracle_solution(sipg, oracle, dx, dy, p):
+    pen = sipg.pen_sigma(p)
+    op = sipg.Operator(dx, dy, p, pen, sipg.F64)
+    b_ref = np.random.default_rng(dx + dy + p).uniform(-1, 1, op.n)
+    b, x, x1 = op.vector(), op.vector(), op.vector()
+    op.upload(b_ref, b)
+    x.zero()
+    st, info = op.pcg2(x, b, 1e-10, sipg.TOL_REL, true_stop=False)
+    assert st == sipg.OK, (st, info.iters)
+    assert op.true_relres(x, b) <= 2e-10
+    xo, _, _ = oracle.cg(dx, dy, p, pen, b_ref, tol=1e-13)
+    assert np.linalg.norm(op.download(x) - xo) <= 1e-8 * np.linalg.norm(xo)
+    x1.zero()
+    st, info1 = op.pcg(x1, b, 1e-10, sipg.TOL_REL, true_stop=False)
+    if dx > 4 and dy > 4:
+        assert info.iters < 0.6 * info1.iters, (info.iters, info1.iters)
+
+
+def test_pcg2_c2_reaches_1e10_in_a_fraction_of_the_iterations(sipg):
+    # 256 x 256, p = 8: block-Jacobi PCG needs 1501 iterations
+    dim, p = 256, 8
+    op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F64)
+    b, x = op.vector(), op.vector()
+    modes = sipg.forcing_draw(0, 4, 4, 4)
+    op.rhs_fourier(modes, b)
+    x.zero()
+    # a true residual at every 10x drop keeps the recursion within reach of
+    # b - A x (the coarse solve's large early corrections leave a gap near
+    # 1e-9 relative otherwise, and rarer replacements then perturb r by a
+    # sizeable fraction of itself)
+    st, info = op.pcg2(x, b, 1e-10, sipg.TOL_REL, maxit=20000, replace_factor=0.1, true_stop=True)
+    assert st == sipg.OK, (st, info.iters)
+    assert op.true_relres(x, b) <= 1e-10
+    assert info.iters < 150, info.iters
+    err = op.err_norms_fourier(modes, x)
+    assert err["L2"] < 1e-10, err
