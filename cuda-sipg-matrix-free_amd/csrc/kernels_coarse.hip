@@ -35,7 +35,8 @@ constexpr int VT = kVecThreads;
 // elements per restrict workgroup: their tiles fill at most 32 KiB of LDS
 template <int P>
 constexpr int rne() { return 4096 / ((P + 1) * (P + 1)) < 32 ? 4096 / ((P + 1) * (P + 1)) : 32; }
-constexpr int GT = 32;           // gemm output tile (GT x GT), k chunk GT
+constexpr int GT = 16;           // gemm output tile (GT x GT)
+constexpr int KC = 256;          // gemm k chunk staged in LDS
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -117,7 +118,11 @@ __global__ __launch_bounds__(VT) void gather_kernel(const double *__restrict__ E
     }
 }
 
-// out = L (M x K) * R (K x N), row-major, tiled GT x GT with k chunks in LDS.
+// out = L (M x K) * R (K x N), row-major.  One workgroup per GT x GT output
+// tile (GT = 16, so a 255 x 255 product has 256 of them, one per CU); its
+// 16-row strip of L and 16-column strip of R are staged whole in LDS, KC
+// columns of k at a time (one chunk up to K = 256), so the kernel waits on
+// global memory once per chunk rather than once per small k tile.
 //   MODE 0: out = L R
 //   MODE 1: out = (L R) .* W               (W: M x N, the 1/Lambda table)
 //   MODE 2: Cpad[(i+1)*(N+2) + j+1] = L R, and per-workgroup partials of
@@ -126,46 +131,46 @@ template <int MODE>
 __global__ __launch_bounds__(256) void coarse_gemm_kernel(const double *__restrict__ L, const double *__restrict__ R,
                                                           int M, int K, int N, const double *__restrict__ W,
                                                           double *__restrict__ out, double *__restrict__ part) {
-    __shared__ double As[GT][GT + 1];
-    __shared__ double Bs[GT][GT + 1];
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;       // 16 x 16 threads, 2 x 2 outputs each
+    __shared__ double Ls[GT][KC + 1];
+    __shared__ double Rs[KC][GT];
+    const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
     const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
-    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-    for (int k0 = 0; k0 < K; k0 += GT) {
-        for (int i = threadIdx.x; i < GT * GT; i += 256) {
-            const int rr = i / GT, cc = i % GT;
-            const int gr = row0 + rr, gk = k0 + cc;
-            As[rr][cc] = (gr < M && gk < K) ? L[(int64_t)gr * K + gk] : 0.0;
-            const int kr = k0 + rr, gc = col0 + cc;
-            Bs[rr][cc] = (kr < K && gc < N) ? R[(int64_t)kr * N + gc] : 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        const int kc = min(KC, K - k0);
+        for (int i = threadIdx.x; i < GT * KC; i += 256) {
+            const int r = i / KC, kk = i - (i / KC) * KC;
+            const int gr = row0 + r;
+            Ls[r][kk] = (gr < M && kk < kc) ? L[(int64_t)gr * K + k0 + kk] : 0.0;
+        }
+        for (int i = threadIdx.x; i < KC * GT; i += 256) {
+            const int kk = i / GT, c = i % GT;
+            const int gc = col0 + c;
+            Rs[kk][c] = (kk < kc && gc < N) ? R[(int64_t)(k0 + kk) * N + gc] : 0.0;
         }
         __syncthreads();
-#pragma unroll 8
-        for (int kk = 0; kk < GT; ++kk) {
-            const double a0 = As[ty][kk], a1 = As[ty + 16][kk];
-            const double b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
-            acc[0][0] += a0 * b0;
-            acc[0][1] += a0 * b1;
-            acc[1][0] += a1 * b0;
-            acc[1][1] += a1 * b1;
+        // four independent partial sums over k mod 4 (zero-padded to KC)
+#pragma unroll 4
+        for (int kk = 0; kk < KC; kk += 4) {
+            acc[0] += Ls[ty][kk] * Rs[kk][tx];
+            acc[1] += Ls[ty][kk + 1] * Rs[kk + 1][tx];
+            acc[2] += Ls[ty][kk + 2] * Rs[kk + 2][tx];
+            acc[3] += Ls[ty][kk + 3] * Rs[kk + 3][tx];
         }
         __syncthreads();
     }
+    const double v = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    const int gr = row0 + ty, gc = col0 + tx;
     double dot = 0.0;
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            const int gr = row0 + ty + 16 * u, gc = col0 + tx + 16 * v;
-            if (gr >= M || gc >= N) continue;
-            const int64_t o = (int64_t)gr * N + gc;
-            if (MODE == 0) out[o] = acc[u][v];
-            if (MODE == 1) out[o] = acc[u][v] * W[o];
-            if (MODE == 2) {
-                out[(int64_t)(gr + 1) * (N + 2) + gc + 1] = acc[u][v];
-                dot += acc[u][v] * W[o];
-            }
+    if (gr < M && gc < N) {
+        const int64_t o = (int64_t)gr * N + gc;
+        if (MODE == 0) out[o] = v;
+        if (MODE == 1) out[o] = v * W[o];
+        if (MODE == 2) {
+            out[(int64_t)(gr + 1) * (N + 2) + gc + 1] = v;
+            dot = v * W[o];
         }
+    }
     if (MODE == 2) {
         __shared__ double red[4];
         dot = wsum(dot);
