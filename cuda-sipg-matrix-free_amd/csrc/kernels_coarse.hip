@@ -18,9 +18,12 @@
 // four dense products of (dim-1)^2 matrices.  One coarse step per
 // preconditioner application:
 //   restrict  -- per element the four corner-weighted sums of r (E4)
-//   gather    -- S(a,b) = sum of the four elements meeting at vertex (a,b)
-//   4 x gemm  -- the DST solve; the last writes C into the zero-bordered
-//                vertex grid Cpad and the block partials of S.C
+//   4 x gemm  -- the DST solve; the first gathers S(a,b), the sum of the
+//                four elements meeting at interior vertex (a,b) (vertex
+//                (a,b) sits at x = b hx, y = 1 - a hy; element (row, col)
+//                has its top-left corner at vertex (row, col)), as it loads
+//                it; the last writes C into the zero-bordered vertex grid
+//                Cpad and the block partials of S.C
 // and the p update interpolates Cpad into every node (p = z + P0^T C + beta p)
 // with r.z = r.z_BJ + S.C.
 #include "sipg_internal.hpp"
@@ -100,24 +103,6 @@ __global__ __launch_bounds__(VT) void restrict_kernel(const T *__restrict__ r, c
     }
 }
 
-// S[(a-1)*nvx + (b-1)] for interior vertex (a, b): vertex (a, b) sits at
-// x = b hx, y = 1 - a hy; element (row, col) has its top-left corner at
-// vertex (row, col).
-__global__ __launch_bounds__(VT) void gather_kernel(const double *__restrict__ E4, int dimx, int dimy,
-                                                    double *__restrict__ S) {
-    const int nvx = dimx - 1, nvy = dimy - 1;
-    const int64_t noe = (int64_t)dimx * dimy;
-    const int total = nvx * nvy;
-    for (int k = blockIdx.x * VT + threadIdx.x; k < total; k += gridDim.x * VT) {
-        const int a = k / nvx + 1, b = k % nvx + 1;
-        const double br = E4[1 * noe + (int64_t)(a - 1) * dimx + (b - 1)];   // element (a-1, b-1)
-        const double bl = E4[0 * noe + (int64_t)(a - 1) * dimx + b];         // element (a-1, b)
-        const double tr = E4[3 * noe + (int64_t)a * dimx + (b - 1)];         // element (a, b-1)
-        const double tl = E4[2 * noe + (int64_t)a * dimx + b];               // element (a, b)
-        S[k] = (br + bl) + (tr + tl);
-    }
-}
-
 // out = L (M x K) * R (K x N), row-major.  One workgroup per GT x GT output
 // tile (GT = 16, so a 255 x 255 product has 256 of them, one per CU); its
 // 16-row strip of L and 16-column strip of R are staged whole in LDS, KC
@@ -127,9 +112,11 @@ __global__ __launch_bounds__(VT) void gather_kernel(const double *__restrict__ E
 //   MODE 1: out = (L R) .* W               (W: M x N, the 1/Lambda table)
 //   MODE 2: Cpad[(i+1)*(N+2) + j+1] = L R, and per-workgroup partials of
 //           sum (L R) .* W into part[blockIdx.y * gridDim.x + blockIdx.x]
+//   MODE 3: as MODE 0 with L = S gathered from the corner sums E4 (L is
+//           E4; the workgroups of the first tile column also store S into W)
 template <int MODE>
 __global__ __launch_bounds__(256) void coarse_gemm_kernel(const double *__restrict__ L, const double *__restrict__ R,
-                                                          int M, int K, int N, const double *__restrict__ W,
+                                                          int M, int K, int N, double *__restrict__ W,
                                                           double *__restrict__ out, double *__restrict__ part) {
     __shared__ double Ls[GT][KC + 1];
     __shared__ double Rs[KC][GT];
@@ -141,7 +128,21 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const double *__restri
         for (int i = threadIdx.x; i < GT * KC; i += 256) {
             const int r = i / KC, kk = i - (i / KC) * KC;
             const int gr = row0 + r;
-            Ls[r][kk] = (gr < M && kk < kc) ? L[(int64_t)gr * K + k0 + kk] : 0.0;
+            double v = 0.0;
+            if (gr < M && kk < kc) {
+                if (MODE == 3) {
+                    // vertex (a, b) = (gr + 1, k + 1) of the (M+1) x (K+1)
+                    // element grid: the four elements around it
+                    const int a = gr + 1, b = k0 + kk + 1, dimx = K + 1;
+                    const int64_t noe = (int64_t)dimx * (M + 1);
+                    v = (L[1 * noe + (int64_t)(a - 1) * dimx + (b - 1)] + L[0 * noe + (int64_t)(a - 1) * dimx + b]) +
+                        (L[3 * noe + (int64_t)a * dimx + (b - 1)] + L[2 * noe + (int64_t)a * dimx + b]);
+                    if (blockIdx.x == 0) W[(int64_t)gr * K + k0 + kk] = v;
+                } else {
+                    v = L[(int64_t)gr * K + k0 + kk];
+                }
+            }
+            Ls[r][kk] = v;
         }
         for (int i = threadIdx.x; i < KC * GT; i += 256) {
             const int kk = i / GT, c = i % GT;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const double *__restri
     double dot = 0.0;
     if (gr < M && gc < N) {
         const int64_t o = (int64_t)gr * N + gc;
-        if (MODE == 0) out[o] = v;
+        if (MODE == 0 || MODE == 3) out[o] = v;
         if (MODE == 1) out[o] = v * W[o];
         if (MODE == 2) {
             out[(int64_t)(gr + 1) * (N + 2) + gc + 1] = v;
@@ -344,12 +345,10 @@ int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, con
     a.s = s;
     int rc = dispatch_p<1>(p, dtype, 0, a);
     if (rc) return rc;
-    const int nt = nvx * nvy;
-    hipLaunchKernelGGL(gather_kernel, dim3(std::max(1, std::min(1024, (nt + VT - 1) / VT))), dim3(VT), 0, s, c.E4,
-                       dimx, dimy, c.S);
-    // T1 = S Qx; T2 = (Qy T1) ./ Lambda; T1 = T2 Qx; C = Qy T1 (into Cpad), partials of S.C
-    hipLaunchKernelGGL(coarse_gemm_kernel<0>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.S, c.Qx, nvy, nvx, nvx,
-                       nullptr, c.T1, nullptr);
+    // T1 = S Qx (S gathered from E4 on the fly, and stored); T2 = (Qy T1) ./
+    // Lambda; T1 = T2 Qx; C = Qy T1 (into Cpad), partials of S.C
+    hipLaunchKernelGGL(coarse_gemm_kernel<3>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.E4, c.Qx, nvy, nvx, nvx,
+                       c.S, c.T1, nullptr);
     hipLaunchKernelGGL(coarse_gemm_kernel<1>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.Qy, c.T1, nvy, nvy, nvx,
                        c.Linv, c.T2, nullptr);
     hipLaunchKernelGGL(coarse_gemm_kernel<0>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.T2, c.Qx, nvy, nvx, nvx,
