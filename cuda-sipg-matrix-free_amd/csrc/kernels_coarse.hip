@@ -40,6 +40,8 @@ template <int P>
 constexpr int rne() { return 4096 / ((P + 1) * (P + 1)) < 32 ? 4096 / ((P + 1) * (P + 1)) : 32; }
 constexpr int GT = 16;           // gemm output tile (GT x GT)
 constexpr int KC = 256;          // gemm k chunk staged in LDS
+constexpr int KS = 4;            // gemm k slices per output
+constexpr int GEMM_T = GT * GT * KS;
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -106,8 +108,10 @@ __global__ __launch_bounds__(VT) void restrict_kernel(const T *__restrict__ r, c
 // out = L (M x K) * R (K x N), row-major.  One workgroup per GT x GT output
 // tile (GT = 16, so a 255 x 255 product has 256 of them, one per CU); its
 // 16-row strip of L and 16-column strip of R are staged whole in LDS, KC
-// columns of k at a time (one chunk up to K = 256), so the kernel waits on
-// global memory once per chunk rather than once per small k tile.
+// columns of k at a time (one chunk up to K = 256).  The workgroup's 1024
+// threads split k four ways (GT x GT outputs x 4 slices: four waves per
+// SIMD, 64 k each), and the four partial sums are added in LDS in a fixed
+// order.
 //   MODE 0: out = L R
 //   MODE 1: out = (L R) .* W               (W: M x N, the 1/Lambda table)
 //   MODE 2: Cpad[(i+1)*(N+2) + j+1] = L R, and per-workgroup partials of
@@ -115,17 +119,20 @@ __global__ __launch_bounds__(VT) void restrict_kernel(const T *__restrict__ r, c
 //   MODE 3: as MODE 0 with L = S gathered from the corner sums E4 (L is
 //           E4; the workgroups of the first tile column also store S into W)
 template <int MODE>
-__global__ __launch_bounds__(256) void coarse_gemm_kernel(const double *__restrict__ L, const double *__restrict__ R,
-                                                          int M, int K, int N, double *__restrict__ W,
-                                                          double *__restrict__ out, double *__restrict__ part) {
+__global__ __launch_bounds__(GEMM_T) void coarse_gemm_kernel(const double *__restrict__ L,
+                                                             const double *__restrict__ R, int M, int K, int N,
+                                                             double *__restrict__ W, double *__restrict__ out,
+                                                             double *__restrict__ part) {
     __shared__ double Ls[GT][KC + 1];
     __shared__ double Rs[KC][GT];
-    const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
+    __shared__ double ps[KS][GT][GT + 1];
+    const int o = threadIdx.x % (GT * GT), ks = threadIdx.x / (GT * GT);
+    const int tx = o % GT, ty = o / GT;
     const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[2] = {0.0, 0.0};
     for (int k0 = 0; k0 < K; k0 += KC) {
         const int kc = min(KC, K - k0);
-        for (int i = threadIdx.x; i < GT * KC; i += 256) {
+        for (int i = threadIdx.x; i < GT * KC; i += GEMM_T) {
             const int r = i / KC, kk = i - (i / KC) * KC;
             const int gr = row0 + r;
             double v = 0.0;
@@ -144,38 +151,41 @@ __global__ __launch_bounds__(256) void coarse_gemm_kernel(const double *__restri
             }
             Ls[r][kk] = v;
         }
-        for (int i = threadIdx.x; i < KC * GT; i += 256) {
+        for (int i = threadIdx.x; i < KC * GT; i += GEMM_T) {
             const int kk = i / GT, c = i % GT;
             const int gc = col0 + c;
             Rs[kk][c] = (kk < kc && gc < N) ? R[(int64_t)(k0 + kk) * N + gc] : 0.0;
         }
         __syncthreads();
-        // four independent partial sums over k mod 4 (zero-padded to KC)
-#pragma unroll 4
-        for (int kk = 0; kk < KC; kk += 4) {
-            acc[0] += Ls[ty][kk] * Rs[kk][tx];
-            acc[1] += Ls[ty][kk + 1] * Rs[kk + 1][tx];
-            acc[2] += Ls[ty][kk + 2] * Rs[kk + 2][tx];
-            acc[3] += Ls[ty][kk + 3] * Rs[kk + 3][tx];
+        // slice ks of k (zero-padded to KC), two partial sums
+        const int kb = ks * (KC / KS);
+#pragma unroll 8
+        for (int kk = 0; kk < KC / KS; kk += 2) {
+            acc[0] += Ls[ty][kb + kk] * Rs[kb + kk][tx];
+            acc[1] += Ls[ty][kb + kk + 1] * Rs[kb + kk + 1][tx];
         }
         __syncthreads();
     }
-    const double v = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-    const int gr = row0 + ty, gc = col0 + tx;
+    ps[ks][ty][tx] = acc[0] + acc[1];
+    __syncthreads();
     double dot = 0.0;
-    if (gr < M && gc < N) {
-        const int64_t o = (int64_t)gr * N + gc;
-        if (MODE == 0 || MODE == 3) out[o] = v;
-        if (MODE == 1) out[o] = v * W[o];
-        if (MODE == 2) {
-            out[(int64_t)(gr + 1) * (N + 2) + gc + 1] = v;
-            dot = v * W[o];
+    if (ks == 0) {
+        const double v = (ps[0][ty][tx] + ps[1][ty][tx]) + (ps[2][ty][tx] + ps[3][ty][tx]);
+        const int gr = row0 + ty, gc = col0 + tx;
+        if (gr < M && gc < N) {
+            const int64_t oo = (int64_t)gr * N + gc;
+            if (MODE == 0 || MODE == 3) out[oo] = v;
+            if (MODE == 1) out[oo] = v * W[oo];
+            if (MODE == 2) {
+                out[(int64_t)(gr + 1) * (N + 2) + gc + 1] = v;
+                dot = v * W[oo];
+            }
         }
     }
     if (MODE == 2) {
-        __shared__ double red[4];
+        __shared__ double red[GT * GT / 64];
         dot = wsum(dot);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
+        if (ks == 0 && (threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
         __syncthreads();
         if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
     }
@@ -347,13 +357,13 @@ int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, con
     if (rc) return rc;
     // T1 = S Qx (S gathered from E4 on the fly, and stored); T2 = (Qy T1) ./
     // Lambda; T1 = T2 Qx; C = Qy T1 (into Cpad), partials of S.C
-    hipLaunchKernelGGL(coarse_gemm_kernel<3>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.E4, c.Qx, nvy, nvx, nvx,
+    hipLaunchKernelGGL(coarse_gemm_kernel<3>, gemm_grid(nvy, nvx), dim3(GEMM_T), 0, s, c.E4, c.Qx, nvy, nvx, nvx,
                        c.S, c.T1, nullptr);
-    hipLaunchKernelGGL(coarse_gemm_kernel<1>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.Qy, c.T1, nvy, nvy, nvx,
+    hipLaunchKernelGGL(coarse_gemm_kernel<1>, gemm_grid(nvy, nvx), dim3(GEMM_T), 0, s, c.Qy, c.T1, nvy, nvy, nvx,
                        c.Linv, c.T2, nullptr);
-    hipLaunchKernelGGL(coarse_gemm_kernel<0>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.T2, c.Qx, nvy, nvx, nvx,
+    hipLaunchKernelGGL(coarse_gemm_kernel<0>, gemm_grid(nvy, nvx), dim3(GEMM_T), 0, s, c.T2, c.Qx, nvy, nvx, nvx,
                        nullptr, c.T1, nullptr);
-    hipLaunchKernelGGL(coarse_gemm_kernel<2>, gemm_grid(nvy, nvx), dim3(256), 0, s, c.Qy, c.T1, nvy, nvy, nvx, c.S,
+    hipLaunchKernelGGL(coarse_gemm_kernel<2>, gemm_grid(nvy, nvx), dim3(GEMM_T), 0, s, c.Qy, c.T1, nvy, nvy, nvx, c.S,
                        c.Cpad, c.part);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
