@@ -401,7 +401,8 @@ int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg
 }
 
 // FP32 inner (P)CG cycles on the op's FP32 twin inside FP64 refinement.
-int solve_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info, bool pcg) {
+int solve_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info, bool pcg,
+                bool two_level = false) {
     if (!op || !x || !b || op->dtype != SIPG_F64) return SIPG_ERR_INVALID;
     int rc = check_opts(opts);
     if (rc) return rc;
@@ -413,6 +414,7 @@ int solve_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, s
     sipg_op *tw = op->twin;
     tw->stream = op->stream;
     if (pcg && (rc = ensure_prec(tw))) return rc;
+    if (two_level && (rc = ensure_coarse(tw))) return rc;       // coarse solve in FP64, vectors in FP32
     const size_t v32 = (size_t)op->n * 4, v64 = (size_t)op->n * 8;
     if ((rc = ensure_ws(tw, (pcg ? 5 : 4) * v32))) return rc;    // r^, p^, A p^, x^ (, z^)
     if ((rc = ensure_ws(op, v64))) return rc;           // A x in FP64
@@ -421,6 +423,8 @@ int solve_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, s
     E.pcg = pcg;
     E.z = pcg ? w + 4 * v32 : nullptr;
     E.prec = tw->d_prec;
+    E.coarse = two_level ? &tw->coarse : nullptr;
+    E.ncp = two_level ? coarse_num_partials(tw->dimx, tw->dimy) : 0;
     E.opI = tw;
     E.opO = op;
     E.r = w;
@@ -511,6 +515,10 @@ int sipg_cg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts,
 
 int sipg_pcg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {
     return solve_mixed(op, x, b, opts, info, true);
+}
+
+int sipg_pcg2_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info) {
+    return solve_mixed(op, x, b, opts, info, true, true);
 }
 
 int sipg_true_relres(sipg_op *op, const void *x, const void *b, double *relres) {

@@ -99,6 +99,7 @@ def _declare(L):
         "sipg_prec_apply": (i, [_vp, _vp, _vp]),
         "sipg_prec2_apply": (i, [_vp, _vp, _vp]),
         "sipg_pcg2": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
+        "sipg_pcg2_mixed": (i, [_vp, _vp, _vp, ctypes.POINTER(CgOpts), ctypes.POINTER(CgInfo)]),
         "sipg_vec_dot": (i, [i, i64, _vp, _vp, _dp, _vp]),
         "sipg_vec_axpy": (i, [i, i64, d, _vp, _vp, _vp]),
         "sipg_vec_copy": (i, [i, i64, _vp, _vp, _vp]),
@@ -314,6 +315,13 @@ class Operator:
         o = CgOpts(tol, tol_mode, maxit, replace_factor, 1 if true_stop else 0, int(restart))
         info = CgInfo()
         st = lib().sipg_pcg2(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
+        return st, info
+
+    def pcg2_mixed(self, x, b, tol, tol_mode=TOL_REL, maxit=0, replace_factor=0.0, restart=True):
+        """FP32 two-level PCG cycles inside FP64 refinement (sipg_pcg2_mixed)."""
+        o = CgOpts(tol, tol_mode, maxit, replace_factor, 1, int(restart))
+        info = CgInfo()
+        st = lib().sipg_pcg2_mixed(self.h, x.ptr, b.ptr, ctypes.byref(o), ctypes.byref(info))
         return st, info
 
     def prec2_apply(self, r, z):

@@ -216,6 +216,10 @@ int sipg_pcg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts
 int sipg_pcg2(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
 /* z = M2^-1 r (two-level, as sipg_pcg2), asynchronous on the op's stream. */
 int sipg_prec2_apply(sipg_op *op, const void *r, void *z);
+/* sipg_pcg_mixed with the two-level preconditioner: FP32 two-level PCG
+ * cycles (the coarse solve in FP64) inside FP64 iterative refinement on the
+ * true residual.  Options and results as sipg_pcg_mixed. */
+int sipg_pcg2_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
 
 /* z = M^-1 r with the preconditioner of sipg_pcg (device vectors of the
  * op's dtype, r and z distinct); replaces _prec_mvm (abs_mvm.hpp:15). */

@@ -251,3 +251,20 @@ def test_pcg2_c2_reaches_1e10_in_a_fraction_of_the_iterations(sipg):
     assert info.iters < 150, info.iters
     err = op.err_norms_fourier(modes, x)
     assert err["L2"] < 1e-10, err
+
+
+def test_pcg2_mixed_reaches_fp64_tolerance(sipg):
+    # FP32 two-level PCG cycles (restarted each decade) inside FP64
+    # refinement: the FP64 solution to 1e-10 on the true residual
+    dim, p = 64, 8
+    op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F64)
+    b, x, x64 = op.vector(), op.vector(), op.vector()
+    op.rhs_fourier(sipg.forcing_draw(2, 4, 4, 4), b)
+    x.zero()
+    st, info = op.pcg2_mixed(x, b, 1e-10, sipg.TOL_REL, maxit=5000, replace_factor=0.1, restart=1)
+    assert st == sipg.OK, (st, info.iters)
+    assert op.true_relres(x, b) <= 1e-10
+    x64.zero()
+    st, _ = op.pcg2(x64, b, 1e-12, sipg.TOL_REL, replace_factor=0.1, true_stop=True)
+    u, u64 = op.download(x), op.download(x64)
+    assert np.linalg.norm(u - u64) <= 1e-8 * np.linalg.norm(u64)
