@@ -343,7 +343,8 @@ int coarse_num_partials(int dimx, int dimy) {
     return (int)(g.x * g.y);
 }
 
-int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t s) {
+int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t s,
+                        bool have_e4) {
     const int nvx = dimx - 1, nvy = dimy - 1;
     const int64_t noe = (int64_t)dimx * dimy;
     if (nvx < 1 || nvy < 1) return SIPG_OK;          // no interior vertex: C = 0, no partials
@@ -353,7 +354,7 @@ int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, con
     a.noe = noe;
     a.E4 = c.E4;
     a.s = s;
-    int rc = dispatch_p<1>(p, dtype, 0, a);
+    int rc = have_e4 ? SIPG_OK : dispatch_p<1>(p, dtype, 0, a);
     if (rc) return rc;
     // T1 = S Qx (S gathered from E4 on the fly, and stored); T2 = (Qy T1) ./
     // Lambda; T1 = T2 Qx; C = Qy T1 (into Cpad), partials of S.C

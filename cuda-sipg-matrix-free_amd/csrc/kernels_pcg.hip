@@ -42,6 +42,14 @@ struct PcgShape {
     static constexpr int KR = (NV + NT - 1) / NT;       // = N1
 };
 
+// The restriction's line shares (4 (p+1) doubles per element) are summed in
+// the element tiles' LDS once z has left it, where they fit: FP64 from
+// p = 3, FP32 from p = 7; elsewhere the separate restriction kernel runs.
+// es: bytes per value.
+__host__ __device__ constexpr bool rz_fuses_restrict(int p, int es) {
+    return 4 * (p + 1) * 8 <= tile_stride((p + 1) * (p + 1)) * es;
+}
+
 // Sum over the workgroup, returned to every thread (fixed order).
 template <int NT>
 __device__ __forceinline__ double group_sum_all(double v, double *red, double *total) {
@@ -62,18 +70,26 @@ __device__ __forceinline__ double group_sum_all(double v, double *red, double *t
 // 1 / (lambda_a + lambda_c)], in T.
 //   CX: the coefficients are the compile-time tables of the standard penalty
 //   pen = 5 (p+1)^2 (prec is then unused); otherwise they are read from prec.
-template <typename T, int P, bool UPDATE, bool COPYP, bool CX>
+//   RESTR: with E4, also the two-level restriction's corner sums (its own
+//   instantiation, so that the plain kernel keeps its registers).
+template <typename T, int P, bool UPDATE, bool COPYP, bool CX, bool RESTR = false>
 __global__ __launch_bounds__(PcgShape<P>::NT)
 void pcg_rz_kernel(T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ z,
                    T *__restrict__ pcopy, int64_t noe, const T *__restrict__ prec,
                    const double *__restrict__ ppAp, int np, double *__restrict__ prr,
-                   double *__restrict__ prz, cg_state *st, int slot)
+                   double *__restrict__ prz, cg_state *st, int slot, double *__restrict__ E4)
 {
     using G = PcgShape<P>;
     constexpr int N1 = G::N1, NQ = G::NQ, TS = G::TS, NE = G::NE, NT = G::NT, KR = G::KR;
-    __shared__ T lds[NE * TS];
+    __shared__ __attribute__((aligned(16))) T lds[NE * TS];
     __shared__ double red[(NT + 63) / 64];
     __shared__ double total;
+    // with E4 (the two-level preconditioner's restriction): each line's
+    // share of its element's four corner-weighted sums, per corner c
+    // (0 bottom-left, 1 bottom-right, 2 top-left, 3 top-right), held in
+    // registers and summed over the lines in the tile LDS at the end
+    constexpr bool FUSE = RESTR && rz_fuses_restrict(P, (int)sizeof(T));
+    constexpr cx::Tables<P> TB = cx::make_tables<P>();
 
     T alpha = T(0);
     if (UPDATE) {
@@ -110,6 +126,7 @@ void pcg_rz_kernel(T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ 
     for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
         const int64_t base = g * (int64_t)(NE * NQ);
         T rv[KR];
+        double cv[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
             const int idx = tid + k * NT;
@@ -133,6 +150,22 @@ void pcg_rz_kernel(T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ 
             T c[N1], t[N1];
 #pragma unroll
             for (int i = 0; i < N1; ++i) c[i] = tile[i * N1 + j];
+            if (FUSE && E4) {
+                // column j is the x line r(., j) at y node j: x weights here,
+                // the y weight of line j, summed over the lines below
+                double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+                for (int i = 0; i < N1; ++i) {
+                    const double xi = 0.5 * (TB.z[i] + 1.0);
+                    a0 += (1.0 - xi) * (double)c[i];
+                    a1 += xi * (double)c[i];
+                }
+                const double eta = 0.5 * (TB.z[j] + 1.0);
+                cv[0] = (1.0 - eta) * a0;
+                cv[1] = (1.0 - eta) * a1;
+                cv[2] = eta * a0;
+                cv[3] = eta * a1;
+            }
 #pragma unroll
             for (int a = 0; a < N1; ++a) {
                 T acc = T(0);
@@ -195,6 +228,22 @@ void pcg_rz_kernel(T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ 
             }
         }
         __syncthreads();                               // LDS free for the next group
+        if (FUSE && E4) {
+            // E4[c * noe + e]: the lines' shares summed in a fixed order
+            double *cs = reinterpret_cast<double *>(lds);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) cs[(c * N1 + j) * NE + s_el] = cv[c];
+            __syncthreads();
+            for (int q = tid; q < 4 * NE; q += NT) {
+                const int c = q / NE, el = q - (q / NE) * NE;
+                const int64_t e = g * NE + el;
+                double sum = 0.0;
+#pragma unroll
+                for (int jj = 0; jj < N1; ++jj) sum += cs[(c * N1 + jj) * NE + el];
+                if (e < noe) E4[(int64_t)c * noe + e] = sum;
+            }
+            __syncthreads();
+        }
     }
     const double a = block_sum<NT>(srr, red);
     __syncthreads();
@@ -233,11 +282,19 @@ int pcg_grid(int64_t noe) {
                                   kMaxPartials);
 }
 
+// grid: that of the plain kernel (the block-sum count the solver expects);
+// with E4 on a degree where it fits, the restricting instantiation
 template <typename T, int P, bool UPDATE, bool COPYP, bool CX>
 void rz_launch(dim3 grid, void *r, const void *Ap, void *z, void *pcopy, int64_t noe, const void *prec,
-               const double *ppAp, int np, double *prr, double *prz, cg_state *st, int slot, hipStream_t s) {
-    hipLaunchKernelGGL((pcg_rz_kernel<T, P, UPDATE, COPYP, CX>), grid, dim3(PcgShape<P>::NT), 0, s, (T *)r,
-                       (const T *)Ap, (T *)z, (T *)pcopy, noe, (const T *)prec, ppAp, np, prr, prz, st, slot);
+               const double *ppAp, int np, double *prr, double *prz, cg_state *st, int slot, double *E4,
+               hipStream_t s) {
+    if (CX && E4 && rz_fuses_restrict(P, (int)sizeof(T)))
+        hipLaunchKernelGGL((pcg_rz_kernel<T, P, UPDATE, COPYP, CX, CX>), grid, dim3(PcgShape<P>::NT), 0, s, (T *)r,
+                           (const T *)Ap, (T *)z, (T *)pcopy, noe, (const T *)prec, ppAp, np, prr, prz, st, slot, E4);
+    else
+        hipLaunchKernelGGL((pcg_rz_kernel<T, P, UPDATE, COPYP, CX>), grid, dim3(PcgShape<P>::NT), 0, s, (T *)r,
+                           (const T *)Ap, (T *)z, (T *)pcopy, noe, (const T *)prec, ppAp, np, prr, prz, st, slot,
+                           nullptr);
 }
 
 // The standard penalty 5 (p+1)^2 takes the kernel with compile-time
@@ -247,28 +304,28 @@ inline bool standard_pen(int p, double pen) { return pen == 5.0 * (p + 1) * (p +
 template <typename T, int P>
 int launch_rz_one(int mode, bool cx, void *r, const void *Ap, void *z, void *pcopy, int64_t noe,
                   const void *prec, const double *ppAp, int np, double *prr, double *prz, cg_state *st,
-                  int slot, hipStream_t s)
+                  int slot, double *E4, hipStream_t s)
 {
     if (cx) {
         if (mode == 1)
             rz_launch<T, P, true, false, true>(dim3(pcg_grid<T, P, true, false, true>(noe)), r, Ap, z, pcopy,
-                                               noe, prec, ppAp, np, prr, prz, st, slot, s);
+                                               noe, prec, ppAp, np, prr, prz, st, slot, E4, s);
         else if (mode == 2)
             rz_launch<T, P, false, true, true>(dim3(pcg_grid<T, P, false, true, true>(noe)), r, Ap, z, pcopy,
-                                               noe, prec, ppAp, np, prr, prz, st, slot, s);
+                                               noe, prec, ppAp, np, prr, prz, st, slot, E4, s);
         else
             rz_launch<T, P, false, false, true>(dim3(pcg_grid<T, P, false, false, true>(noe)), r, Ap, z,
-                                                pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+                                                pcopy, noe, prec, ppAp, np, prr, prz, st, slot, E4, s);
     } else {
         if (mode == 1)
             rz_launch<T, P, true, false, false>(dim3(pcg_grid<T, P, true, false, false>(noe)), r, Ap, z,
-                                                pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+                                                pcopy, noe, prec, ppAp, np, prr, prz, st, slot, E4, s);
         else if (mode == 2)
             rz_launch<T, P, false, true, false>(dim3(pcg_grid<T, P, false, true, false>(noe)), r, Ap, z,
-                                                pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+                                                pcopy, noe, prec, ppAp, np, prr, prz, st, slot, E4, s);
         else
             rz_launch<T, P, false, false, false>(dim3(pcg_grid<T, P, false, false, false>(noe)), r, Ap, z,
-                                                 pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+                                                 pcopy, noe, prec, ppAp, np, prr, prz, st, slot, E4, s);
     }
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
@@ -291,7 +348,7 @@ struct PcgDispatch;
 template <typename T>
 struct PcgDispatch<T> {
     static int launch(int, int, bool, void *, const void *, void *, void *, int64_t, const void *,
-                      const double *, int, double *, double *, cg_state *, int, hipStream_t) {
+                      const double *, int, double *, double *, cg_state *, int, double *, hipStream_t) {
         return SIPG_ERR_UNSUPPORTED;
     }
     static int grid(int, int, bool, int64_t) { return -1; }
@@ -301,11 +358,11 @@ template <typename T, int P0, int... Ps>
 struct PcgDispatch<T, P0, Ps...> {
     static int launch(int p, int mode, bool cx, void *r, const void *Ap, void *z, void *pcopy, int64_t noe,
                       const void *prec, const double *ppAp, int np, double *prr, double *prz, cg_state *st,
-                      int slot, hipStream_t s) {
+                      int slot, double *E4, hipStream_t s) {
         if (p == P0)
-            return launch_rz_one<T, P0>(mode, cx, r, Ap, z, pcopy, noe, prec, ppAp, np, prr, prz, st, slot, s);
+            return launch_rz_one<T, P0>(mode, cx, r, Ap, z, pcopy, noe, prec, ppAp, np, prr, prz, st, slot, E4, s);
         return PcgDispatch<T, Ps...>::launch(p, mode, cx, r, Ap, z, pcopy, noe, prec, ppAp, np, prr, prz, st,
-                                             slot, s);
+                                             slot, E4, s);
     }
     static int grid(int p, int mode, bool cx, int64_t noe) {
         if (p == P0) return grid_one<T, P0>(mode, cx, noe);
@@ -435,15 +492,19 @@ int pcg_num_blocks(int dtype, int p, double pen, int mode, int64_t noe) {
                              : PcgDegrees<float>::grid(p, mode, cx, noe);
 }
 
+bool pcg_rz_fuses_restrict(int dtype, int p, double pen) {
+    return p >= 1 && p <= kMaxDegree && standard_pen(p, pen) && rz_fuses_restrict(p, dtype == SIPG_F64 ? 8 : 4);
+}
+
 int launch_pcg_rz(int dtype, int p, double pen, int mode, void *r, const void *Ap, void *z, void *pcopy,
                   int64_t noe, const void *prec, const double *part_pAp, int np_pAp, double *part_rr,
-                  double *part_rz, cg_state *st, int slot, hipStream_t s) {
+                  double *part_rz, cg_state *st, int slot, hipStream_t s, double *E4) {
     const bool cx = standard_pen(p, pen);
     if (dtype == SIPG_F64)
         return PcgDegrees<double>::launch(p, mode, cx, r, Ap, z, pcopy, noe, prec, part_pAp, np_pAp, part_rr,
-                                          part_rz, st, slot, s);
+                                          part_rz, st, slot, E4, s);
     return PcgDegrees<float>::launch(p, mode, cx, r, Ap, z, pcopy, noe, prec, part_pAp, np_pAp, part_rr,
-                                     part_rz, st, slot, s);
+                                     part_rz, st, slot, E4, s);
 }
 
 int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, const double *part_pAp,

@@ -173,9 +173,14 @@ int launch_mark_replaced(cg_state *st, int slot, int restart, hipStream_t s);
 // (pen selects the kernel: the standard 5 (p+1)^2 has its coefficients
 // compiled in, any other reads them from prec)
 int pcg_num_blocks(int dtype, int p, double pen, int mode, int64_t noe);
+// E4 non-null: also the two-level restriction's corner sums of r (as
+// launch_coarse_solve's first step), from the same read of r, where
+// pcg_rz_fuses_restrict (the standard penalty's kernels, where the LDS has
+// room); ignored elsewhere
+bool pcg_rz_fuses_restrict(int dtype, int p, double pen);
 int launch_pcg_rz(int dtype, int p, double pen, int mode, void *r, const void *Ap, void *z, void *pcopy,
                   int64_t noe, const void *prec, const double *part_pAp, int np_pAp, double *part_rr,
-                  double *part_rz, cg_state *st, int slot, hipStream_t s);
+                  double *part_rz, cg_state *st, int slot, hipStream_t s, double *E4 = nullptr);
 // p = z + beta p (and x += alpha p unless r was replaced this iteration);
 // writes the next slot
 int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, const double *part_pAp,
@@ -199,8 +204,10 @@ struct coarse_tables {
 };
 int coarse_num_partials(int dimx, int dimy);
 // C = A0^-1 P0 r into Cpad, with the S.C partials (nothing when a side has
-// one element: no interior vertex)
-int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t s);
+// one element: no interior vertex); have_e4: the corner sums are already in
+// c.E4 (launch_pcg_rz wrote them)
+int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t s,
+                        bool have_e4 = false);
 // v += P0^T C
 int launch_coarse_add(int dtype, int p, void *v, int dimx, int64_t n, const coarse_tables &c, int nblk,
                       hipStream_t s);

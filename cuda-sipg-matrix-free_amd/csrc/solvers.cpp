@@ -48,6 +48,11 @@ struct Engine {
     // two-level: the coarse level of kernels_coarse.hip on top of the blocks
     const coarse_tables *coarse = nullptr;
     int ncp = 0;                     // its S.C partials
+    // where the block-Jacobi kernel leaves the restriction's corner sums
+    // (null: the coarse solve restricts r itself)
+    double *e4() const {
+        return coarse && pcg_rz_fuses_restrict(opI->dtype, opI->p, opI->pen) ? coarse->E4 : nullptr;
+    }
 
     hipStream_t s = nullptr;
     cg_state *st = nullptr;
@@ -72,10 +77,11 @@ struct Engine {
             if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, nullptr, opI->n, P.rr, P.bb, nb, s)))
                 return rc;
             if ((rc = launch_pcg_rz(opI->dtype, opI->p, opI->pen, 2, r, nullptr, z, p, noe(), prec, nullptr, 0, P.rr, P.rz,
-                                    nullptr, 0, s)))
+                                    nullptr, 0, s, e4())))
                 return rc;
             if (coarse) {      // p = z_BJ + P0^T A0^-1 P0 r; r.z gains S.C
-                if ((rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s)) ||
+                if ((rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s,
+                                              e4() != nullptr)) ||
                     (rc = launch_coarse_add(opI->dtype, opI->p, p, opI->dimx, opI->n, *coarse, nb, s)))
                     return rc;
             }
@@ -113,9 +119,10 @@ struct Engine {
             np = 1;
         }
         if ((rc = launch_pcg_rz(opI->dtype, opI->p, opI->pen, 1, r, Ap, z, nullptr, noe(), prec, src, np, P.rr, P.rz, st,
-                                slot, s)))
+                                slot, s, replace ? nullptr : e4())))
             return rc;
-        if (coarse && !replace && (rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s)))
+        if (coarse && !replace &&
+            (rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s, e4() != nullptr)))
             return rc;
         int nrz = npc(1);
         if (replace) {
@@ -128,9 +135,10 @@ struct Engine {
                                            nullptr, nb, s)))
                 return rc;
             if ((rc = launch_pcg_rz(opI->dtype, opI->p, opI->pen, 0, r, nullptr, z, nullptr, noe(), prec, nullptr, 0, P.rr,
-                                    P.rz, nullptr, 0, s)))
+                                    P.rz, nullptr, 0, s, e4())))
                 return rc;
-            if (coarse && (rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s)))
+            if (coarse && (rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s,
+                                                    e4() != nullptr)))
                 return rc;
             nrz = npc(0);
             if ((rc = launch_mark_replaced(st, slot, restart_p ? 1 : 0, s))) return rc;
@@ -502,9 +510,11 @@ int sipg_prec2_apply(sipg_op *op, const void *r, void *z) {
     int rc = ensure_prec(op);
     if (rc || (rc = ensure_coarse(op))) return rc;
     parts P = part_layout(op);
+    const bool fused = pcg_rz_fuses_restrict(op->dtype, op->p, op->pen);
     if ((rc = launch_pcg_rz(op->dtype, op->p, op->pen, 0, const_cast<void *>(r), nullptr, z, nullptr,
-                            (int64_t)op->dimx * op->dimy, op->d_prec, nullptr, 0, P.rr, P.rz, nullptr, 0, op->stream)) ||
-        (rc = launch_coarse_solve(op->dtype, op->p, r, op->dimx, op->dimy, op->coarse, op->stream)))
+                            (int64_t)op->dimx * op->dimy, op->d_prec, nullptr, 0, P.rr, P.rz, nullptr, 0, op->stream,
+                            fused ? op->coarse.E4 : nullptr)) ||
+        (rc = launch_coarse_solve(op->dtype, op->p, r, op->dimx, op->dimy, op->coarse, op->stream, fused)))
         return rc;
     return launch_coarse_add(op->dtype, op->p, z, op->dimx, op->n, op->coarse, vec_num_blocks(op->n), op->stream);
 }

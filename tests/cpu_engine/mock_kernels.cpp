@@ -366,7 +366,7 @@ static void pcg_rz_t(int p, int mode, T *r, const T *Ap, T *z, T *pcopy, int64_t
 
 int launch_pcg_rz(int dtype, int p, double, int mode, void *r, const void *Ap, void *z, void *pcopy,
                   int64_t noe, const void *prec, const double *part_pAp, int np_pAp, double *part_rr,
-                  double *part_rz, cg_state *st, int slot, hipStream_t) {
+                  double *part_rz, cg_state *st, int slot, hipStream_t, double *) {
     if (p < 1 || p > kMaxDegree || !prec) return SIPG_ERR_INVALID;
     if (dtype == SIPG_F64)
         pcg_rz_t<double>(p, mode, tp<double>(r), tp<double>(Ap), tp<double>(z), tp<double>(pcopy), noe,
@@ -420,6 +420,8 @@ int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, c
     return SIPG_OK;
 }
 
+bool pcg_rz_fuses_restrict(int, int, double) { return true; }
+
 // ---- coarse level (kernels_coarse.hip), FP64: the same DST solve in loops -
 int coarse_num_partials(int dimx, int dimy) { return (dimx > 1 && dimy > 1) ? 1 : 0; }
 
@@ -427,7 +429,8 @@ static double corner_weight(int c, double xa, double yb) {
     return ((c & 1) ? xa : 1.0 - xa) * ((c & 2) ? yb : 1.0 - yb);
 }
 
-int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t) {
+int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t,
+                        bool) {
     if (dtype != SIPG_F64) return SIPG_ERR_UNSUPPORTED;
     const int nvx = dimx - 1, nvy = dimy - 1, n1 = p + 1, nq = n1 * n1;
     if (nvx < 1 || nvy < 1) return SIPG_OK;

@@ -175,7 +175,7 @@ def _interp_matrix(oracle, dx, dy, p):
     return P
 
 
-@pytest.mark.parametrize("dx,dy,p", [(5, 4, 3), (6, 6, 8), (3, 7, 1), (4, 4, 14)])
+@pytest.mark.parametrize("dx,dy,p", [(5, 4, 3), (6, 6, 8), (3, 7, 1), (4, 4, 14), (3, 4, 16)])
 def test_prec2_adds_the_galerkin_coarse_solve(sipg, oracle, dx, dy, p):
     # M2^-1 r - M_BJ^-1 r = P0^T (P0 A P0^T)^-1 P0 r, with A the oracle's
     # operator: pins the sine-transform tables, the restriction, the
