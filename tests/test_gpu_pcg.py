@@ -268,3 +268,30 @@ def test_pcg2_mixed_reaches_fp64_tolerance(sipg):
     st, _ = op.pcg2(x64, b, 1e-12, sipg.TOL_REL, replace_factor=0.1, true_stop=True)
     u, u64 = op.download(x), op.download(x64)
     assert np.linalg.norm(u - u64) <= 1e-8 * np.linalg.norm(u64)
+
+
+@pytest.mark.parametrize("dx,dy", [(40, 40), (37, 22)])
+def test_prec2_coarse_solve_on_several_tiles(sipg, oracle, dx, dy):
+    # vertex grids of several 16 x 16 product tiles with ragged edges,
+    # against A0 = Kx (x) My + Mx (x) Ky (the form the CPU engine pins
+    # against P0 A P0^T) and the first-principles interpolation
+    p = 2
+    op = sipg.Operator(dx, dy, p, sipg.pen_sigma(p), sipg.F64)
+    P = _interp_matrix(oracle, dx, dy, p)
+
+    def factors(n):
+        K = 2 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+        M = (2 / 3) * np.eye(n) + (1 / 6) * (np.eye(n, k=1) + np.eye(n, k=-1))
+        return K, M
+
+    Kx, Mx = factors(dx - 1)
+    Ky, My = factors(dy - 1)
+    A0 = np.kron(My, Kx) + np.kron(Ky, Mx)      # vertex (a, b) -> a * (dx-1) + b
+    r = np.random.default_rng(dx + dy).uniform(-1, 1, op.n)
+    rd, z1, z2 = op.vector(), op.vector(), op.vector()
+    op.upload(r, rd)
+    op.prec_apply(rd, z1)
+    op.prec2_apply(rd, z2)
+    coarse = op.download(z2) - op.download(z1)
+    want = P @ np.linalg.solve(A0, P.T @ r)
+    assert np.max(np.abs(coarse - want)) <= 1e-11 * np.max(np.abs(want))
