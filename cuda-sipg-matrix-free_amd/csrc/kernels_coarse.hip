@@ -39,7 +39,7 @@ constexpr int VT = kVecThreads;
 template <int P>
 constexpr int rne() { return 4096 / ((P + 1) * (P + 1)) < 32 ? 4096 / ((P + 1) * (P + 1)) : 32; }
 constexpr int GT = 16;           // gemm output tile (GT x GT)
-constexpr int KC = 256;          // gemm k chunk staged in LDS
+constexpr int KC2 = 128;         // folded k chunk staged in LDS
 constexpr int KS = 4;            // gemm k slices per output
 constexpr int GEMM_T = GT * GT * KS;
 
@@ -105,64 +105,110 @@ __global__ __launch_bounds__(VT) void restrict_kernel(const T *__restrict__ r, c
     }
 }
 
-// out = L (M x K) * R (K x N), row-major.  One workgroup per GT x GT output
-// tile (GT = 16, so a 255 x 255 product has 256 of them, one per CU); its
-// 16-row strip of L and 16-column strip of R are staged whole in LDS, KC
-// columns of k at a time (one chunk up to K = 256).  The workgroup's 1024
-// threads split k four ways (GT x GT outputs x 4 slices: four waves per
-// SIMD, 64 k each), and the four partial sums are added in LDS in a fixed
-// order.
-//   MODE 0: out = L R
-//   MODE 1: out = (L R) .* W               (W: M x N, the 1/Lambda table)
-//   MODE 2: Cpad[(i+1)*(N+2) + j+1] = L R, and per-workgroup partials of
-//           sum (L R) .* W into part[blockIdx.y * gridDim.x + blockIdx.x]
-//   MODE 3: as MODE 0 with L = S gathered from the corner sums E4 (L is
-//           E4; the workgroups of the first tile column also store S into W)
-template <int MODE>
-__global__ __launch_bounds__(GEMM_T) void coarse_gemm_kernel(const double *__restrict__ L,
-                                                             const double *__restrict__ R, int M, int K, int N,
-                                                             double *__restrict__ W, double *__restrict__ out,
-                                                             double *__restrict__ part) {
-    __shared__ double Ls[GT][KC + 1];
-    __shared__ double Rs[KC][GT];
+// The sine matrix Q (n x n, symmetric) has Q[j][n-1-k] = (-1)^j Q[j][k],
+// so each product folds its k range in half: with h = n / 2,
+//   (Q X)[j][m] = sum_{k<h} Q[j][k] (X[k][m] +- X[n-1-k][m]) (+ Q[j][h] X[h][m], n odd)
+//   (X Q)[m][j] = sum_{k<h} (X[m][k] +- X[m][n-1-k]) Q[k][j]  (+ X[m][h] Q[h][j], n odd)
+// with + for even j and - for odd j.  One workgroup per GT x GT output tile
+// (GT = 16, so a 255 x 255 product has 256 of them, one per CU); the folded
+// strips (KC2 values of the half range at a time, one chunk up to n = 256)
+// are staged in LDS, and the 1024 threads split that range four ways (four
+// waves per SIMD), the partial sums added in LDS in a fixed order.
+//   SIDE 0: out (n x N) = Q X (X: n x N);  SIDE 1: out (M x n) = X Q (X: M x n)
+//   MODE 0: out = the product                (SIDE 1)
+//   MODE 1: out = product .* W               (SIDE 0; W the 1/Lambda table)
+//   MODE 2: Cpad[(i+1)*(N+2) + j+1] = product, and per-workgroup partials of
+//           sum product .* W into part[blockIdx.y * gridDim.x + blockIdx.x]   (SIDE 0)
+//   MODE 3: as MODE 0 with X = S gathered from the corner sums E4 (X is E4,
+//           M + 1 by n + 1 elements; the first tile column also stores S
+//           into W)
+template <int SIDE, int MODE>
+__global__ __launch_bounds__(GEMM_T) void dst_gemm_kernel(const double *__restrict__ X,
+                                                          const double *__restrict__ Q, int M, int n, int N,
+                                                          double *__restrict__ W, double *__restrict__ out,
+                                                          double *__restrict__ part) {
+    // SIDE 0: A = Q rows (plain), B0/B1 = folded X (+/-); SIDE 1: A0/A1 =
+    // folded X rows (+/-), B = Q columns
+    __shared__ double Aq[SIDE == 1 ? 2 : 1][GT][KC2 + 1];
+    __shared__ double Bq[SIDE == 0 ? 2 : 1][KC2][GT];
     __shared__ double ps[KS][GT][GT + 1];
     const int o = threadIdx.x % (GT * GT), ks = threadIdx.x / (GT * GT);
     const int tx = o % GT, ty = o / GT;
     const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
-    double acc[2] = {0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += KC) {
-        const int kc = min(KC, K - k0);
-        for (int i = threadIdx.x; i < GT * KC; i += GEMM_T) {
-            const int r = i / KC, kk = i - (i / KC) * KC;
-            const int gr = row0 + r;
-            double v = 0.0;
-            if (gr < M && kk < kc) {
-                if (MODE == 3) {
-                    // vertex (a, b) = (gr + 1, k + 1) of the (M+1) x (K+1)
-                    // element grid: the four elements around it
-                    const int a = gr + 1, b = k0 + kk + 1, dimx = K + 1;
-                    const int64_t noe = (int64_t)dimx * (M + 1);
-                    v = (L[1 * noe + (int64_t)(a - 1) * dimx + (b - 1)] + L[0 * noe + (int64_t)(a - 1) * dimx + b]) +
-                        (L[3 * noe + (int64_t)a * dimx + (b - 1)] + L[2 * noe + (int64_t)a * dimx + b]);
-                    if (blockIdx.x == 0) W[(int64_t)gr * K + k0 + kk] = v;
-                } else {
-                    v = L[(int64_t)gr * K + k0 + kk];
-                }
-            }
-            Ls[r][kk] = v;
+    const int rows = SIDE == 0 ? n : M, cols = SIDE == 0 ? N : n;
+    const int h = n / 2, kh = h + (n & 1);          // folded length (with the middle)
+    // X element (r, k) of the (rows x n) or (n x N) operand
+    auto xval = [&](int r, int k) -> double {
+        if (MODE == 3) {
+            const int a = r + 1, b = k + 1, dimx = n + 1;
+            const int64_t noe = (int64_t)dimx * (M + 1);
+            return (X[1 * noe + (int64_t)(a - 1) * dimx + (b - 1)] + X[0 * noe + (int64_t)(a - 1) * dimx + b]) +
+                   (X[3 * noe + (int64_t)a * dimx + (b - 1)] + X[2 * noe + (int64_t)a * dimx + b]);
         }
-        for (int i = threadIdx.x; i < KC * GT; i += GEMM_T) {
-            const int kk = i / GT, c = i % GT;
-            const int gc = col0 + c;
-            Rs[kk][c] = (kk < kc && gc < N) ? R[(int64_t)(k0 + kk) * N + gc] : 0.0;
+        return SIDE == 0 ? X[(int64_t)k * N + r] : X[(int64_t)r * n + k];
+    };
+    // output (gr, gc): its parity picks the folded operand
+    const int gr = row0 + ty, gc = col0 + tx;
+    const bool odd = SIDE == 0 ? (gr & 1) : (gc & 1);
+    double acc[2] = {0.0, 0.0};
+    for (int k0 = 0; k0 < kh; k0 += KC2) {
+        const int kc = min(KC2, kh - k0);
+        for (int i = threadIdx.x; i < GT * KC2; i += GEMM_T) {
+            const int r = i / KC2, kk = i - (i / KC2) * KC2, k = k0 + kk;
+            if (SIDE == 0) {
+                const int gq = row0 + r;
+                Aq[0][r][kk] = (gq < rows && kk < kc) ? Q[(int64_t)gq * n + k] : 0.0;
+            } else {
+                const int gx = row0 + r;
+                double p = 0.0, q = 0.0;
+                if (gx < rows && kk < kc) {
+                    const double a = xval(gx, k);
+                    if (k < h) {
+                        const double b = xval(gx, n - 1 - k);
+                        p = a + b;
+                        q = a - b;
+                        if (MODE == 3 && blockIdx.x == 0) {
+                            W[(int64_t)gx * n + k] = a;
+                            W[(int64_t)gx * n + n - 1 - k] = b;
+                        }
+                    } else {                                  // the middle (n odd)
+                        p = q = a;
+                        if (MODE == 3 && blockIdx.x == 0) W[(int64_t)gx * n + k] = a;
+                    }
+                }
+                Aq[0][r][kk] = p;
+                Aq[SIDE == 1 ? 1 : 0][r][kk] = q;
+            }
+        }
+        for (int i = threadIdx.x; i < KC2 * GT; i += GEMM_T) {
+            const int kk = i / GT, c = i % GT, k = k0 + kk;
+            if (SIDE == 0) {
+                const int gx = col0 + c;
+                double p = 0.0, q = 0.0;
+                if (kk < kc && gx < cols) {
+                    const double a = xval(gx, k);
+                    if (k < h) {
+                        const double b = xval(gx, n - 1 - k);
+                        p = a + b;
+                        q = a - b;
+                    } else {
+                        p = q = a;
+                    }
+                }
+                Bq[0][kk][c] = p;
+                Bq[SIDE == 0 ? 1 : 0][kk][c] = q;
+            } else {
+                const int gq = col0 + c;
+                Bq[0][kk][c] = (kk < kc && gq < cols) ? Q[(int64_t)k * n + gq] : 0.0;
+            }
         }
         __syncthreads();
-        // slice ks of k (zero-padded to KC), two partial sums
-        const int kb = ks * (KC / KS);
+        const int kb = ks * (KC2 / KS);
+        const int ia = SIDE == 1 ? (int)odd : 0, ib = SIDE == 0 ? (int)odd : 0;
 #pragma unroll 8
-        for (int kk = 0; kk < KC / KS; kk += 2) {
-            acc[0] += Ls[ty][kb + kk] * Rs[kb + kk][tx];
-            acc[1] += Ls[ty][kb + kk + 1] * Rs[kb + kk + 1][tx];
+        for (int kk = 0; kk < KC2 / KS; kk += 2) {
+            acc[0] += Aq[ia][ty][kb + kk] * Bq[ib][kb + kk][tx];
+            acc[1] += Aq[ia][ty][kb + kk + 1] * Bq[ib][kb + kk + 1][tx];
         }
         __syncthreads();
     }
@@ -171,13 +217,12 @@ __global__ __launch_bounds__(GEMM_T) void coarse_gemm_kernel(const double *__res
     double dot = 0.0;
     if (ks == 0) {
         const double v = (ps[0][ty][tx] + ps[1][ty][tx]) + (ps[2][ty][tx] + ps[3][ty][tx]);
-        const int gr = row0 + ty, gc = col0 + tx;
-        if (gr < M && gc < N) {
-            const int64_t oo = (int64_t)gr * N + gc;
+        if (gr < rows && gc < cols) {
+            const int64_t oo = (int64_t)gr * cols + gc;
             if (MODE == 0 || MODE == 3) out[oo] = v;
             if (MODE == 1) out[oo] = v * W[oo];
             if (MODE == 2) {
-                out[(int64_t)(gr + 1) * (N + 2) + gc + 1] = v;
+                out[(int64_t)(gr + 1) * (cols + 2) + gc + 1] = v;
                 dot = v * W[oo];
             }
         }
@@ -358,14 +403,11 @@ int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, con
     if (rc) return rc;
     // T1 = S Qx (S gathered from E4 on the fly, and stored); T2 = (Qy T1) ./
     // Lambda; T1 = T2 Qx; C = Qy T1 (into Cpad), partials of S.C
-    hipLaunchKernelGGL(coarse_gemm_kernel<3>, gemm_grid(nvy, nvx), dim3(GEMM_T), 0, s, c.E4, c.Qx, nvy, nvx, nvx,
-                       c.S, c.T1, nullptr);
-    hipLaunchKernelGGL(coarse_gemm_kernel<1>, gemm_grid(nvy, nvx), dim3(GEMM_T), 0, s, c.Qy, c.T1, nvy, nvy, nvx,
-                       c.Linv, c.T2, nullptr);
-    hipLaunchKernelGGL(coarse_gemm_kernel<0>, gemm_grid(nvy, nvx), dim3(GEMM_T), 0, s, c.T2, c.Qx, nvy, nvx, nvx,
-                       nullptr, c.T1, nullptr);
-    hipLaunchKernelGGL(coarse_gemm_kernel<2>, gemm_grid(nvy, nvx), dim3(GEMM_T), 0, s, c.Qy, c.T1, nvy, nvy, nvx, c.S,
-                       c.Cpad, c.part);
+    const dim3 g = gemm_grid(nvy, nvx), b(GEMM_T);
+    hipLaunchKernelGGL((dst_gemm_kernel<1, 3>), g, b, 0, s, c.E4, c.Qx, nvy, nvx, nvx, c.S, c.T1, nullptr);
+    hipLaunchKernelGGL((dst_gemm_kernel<0, 1>), g, b, 0, s, c.T1, c.Qy, nvy, nvy, nvx, c.Linv, c.T2, nullptr);
+    hipLaunchKernelGGL((dst_gemm_kernel<1, 0>), g, b, 0, s, c.T2, c.Qx, nvy, nvx, nvx, nullptr, c.T1, nullptr);
+    hipLaunchKernelGGL((dst_gemm_kernel<0, 2>), g, b, 0, s, c.T1, c.Qy, nvy, nvy, nvx, c.S, c.Cpad, c.part);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }
