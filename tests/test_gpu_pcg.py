@@ -295,3 +295,44 @@ def test_prec2_coarse_solve_on_several_tiles(sipg, oracle, dx, dy):
     coarse = op.download(z2) - op.download(z1)
     want = P @ np.linalg.solve(A0, P.T @ r)
     assert np.max(np.abs(coarse - want)) <= 1e-11 * np.max(np.abs(want))
+
+
+def test_prec2_coarse_solve_on_a_grid_wider_than_one_chunk(sipg, oracle):
+    # 300 x 259 elements: the folded products take two k chunks in both
+    # directions.  The coarse correction against numpy's evaluation of the
+    # same sine-transform solve (its tables and the restriction and
+    # interpolation are pinned from first principles by the smaller cases)
+    dx, dy, p = 300, 259, 2
+    n1 = p + 1
+    op = sipg.Operator(dx, dy, p, sipg.pen_sigma(p), sipg.F64)
+    r = np.random.default_rng(9).uniform(-1, 1, op.n)
+    rd, z1, z2 = op.vector(), op.vector(), op.vector()
+    op.upload(r, rd)
+    op.prec_apply(rd, z1)
+    op.prec2_apply(rd, z2)
+    coarse = op.download(z2) - op.download(z1)
+    z, _ = oracle.gll(p)
+    xi = (z + 1) / 2
+    R = r.reshape(n1, n1, dy, dx)                       # (x node, y node, row, col)
+    w0, w1 = 1 - xi, xi
+    V = np.zeros((dy + 1, dx + 1))                      # vertex (a, b) at x = b/dx, y = 1 - a/dy
+    V[1:, :-1] += np.einsum("i,j,ijrc->rc", w0, w0, R)  # bottom-left corners
+    V[1:, 1:] += np.einsum("i,j,ijrc->rc", w1, w0, R)   # bottom-right
+    V[:-1, :-1] += np.einsum("i,j,ijrc->rc", w0, w1, R)  # top-left
+    V[:-1, 1:] += np.einsum("i,j,ijrc->rc", w1, w1, R)   # top-right
+    S = V[1:dy, 1:dx]
+
+    def dst(dim):
+        k = np.arange(1, dim)
+        Q = np.sqrt(2.0 / dim) * np.sin(np.outer(k, k) * np.pi / dim)
+        c = np.cos(k * np.pi / dim)
+        return Q, 2 - 2 * c, 2 / 3 + c / 3              # GLL mass exact for p >= 2
+    Qx, kx, mx = dst(dx)
+    Qy, ky, my = dst(dy)
+    C = Qy @ ((Qy @ S @ Qx) / (np.outer(my, kx) + np.outer(ky, mx))) @ Qx
+    Cv = np.zeros((dy + 1, dx + 1))
+    Cv[1:dy, 1:dx] = C
+    want = (np.einsum("i,j,rc->ijrc", w0, w0, Cv[1:, :-1]) + np.einsum("i,j,rc->ijrc", w1, w0, Cv[1:, 1:]) +
+            np.einsum("i,j,rc->ijrc", w0, w1, Cv[:-1, :-1]) + np.einsum("i,j,rc->ijrc", w1, w1, Cv[:-1, 1:]))
+    want = want.reshape(-1)
+    assert np.max(np.abs(coarse - want)) <= 1e-10 * np.max(np.abs(want)), np.max(np.abs(coarse - want))
