@@ -17,8 +17,12 @@
 //
 // four dense products of (dim-1)^2 matrices.  One coarse step per
 // preconditioner application:
-//   restrict  -- per element the four corner-weighted sums of r (E4)
-//   4 x gemm  -- the DST solve; the first gathers S(a,b), the sum of the
+//   restrict  -- per element the four corner-weighted sums of r (E4); for
+//                the standard penalty the block-Jacobi kernel forms them
+//                from its own read of r (pcg_rz_kernel with RESTR), and
+//                restrict_kernel runs only elsewhere
+//   4 x gemm  -- the DST solve (dst_gemm_kernel, each product over half
+//                the k range); the first gathers S(a,b), the sum of the
 //                four elements meeting at interior vertex (a,b) (vertex
 //                (a,b) sits at x = b hx, y = 1 - a hy; element (row, col)
 //                has its top-left corner at vertex (row, col)), as it loads
