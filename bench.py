@@ -129,6 +129,19 @@ def profile_apply(sipg, op, reps):
         f"({8 * op.n} bytes per vector)")
 
 
+def profile_pcg2(sipg, op, b, reps):
+    """N two-level PCG solves of the 256x256 p=8 FP64 problem (for counter
+    runs over its kernels) and N copies of one vector of the same length,
+    which calibrate FETCH_SIZE / WRITE_SIZE as in profile_apply."""
+    x, z = op.vector(), op.vector()
+    for _ in range(reps):
+        gpu_solve(sipg, op, b, x, "pcg2")
+    for _ in range(reps):
+        sipg.check(sipg.lib().sipg_vec_copy(sipg.F64, op.n, x.ptr, z.ptr, None), "vec_copy")
+    sipg.synchronize()
+    log(f"profile-pcg2: {reps} two-level PCG solves and {reps} copies of {op.n} doubles")
+
+
 def batched_leg(sipg, log_every=True):
     """2048 independent FP64 solves in one batch: mesh side ~ U{4..64},
     degree ~ U{2..10} (numpy default_rng(4)), instance i forced by
@@ -331,6 +344,8 @@ def main():
                     help="also serve the same arrivals by CG (the served solver is PCG)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs that measure roofline.traffic")
+    ap.add_argument("--profile-pcg2", type=int, default=0, metavar="N",
+                    help="only N two-level PCG solves and N vector copies (for counter runs)")
     ap.add_argument("--profile-apply", type=int, default=0, metavar="N",
                     help="only launch N operator applies and N copies of one vector "
                          "(for hardware-counter runs), then exit")
@@ -352,6 +367,9 @@ def main():
 
     if args.profile_apply:
         profile_apply(sipg, op, args.profile_apply)
+        return
+    if args.profile_pcg2:
+        profile_pcg2(sipg, op, b, args.profile_pcg2)
         return
 
     # ---- operator-apply roofline (dominant kernel) -----------------------
