@@ -451,7 +451,8 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
     // j+1 runs, and whatever it decides (replacement set, shorter lists) is
     // enqueued before replay j+2 and copy j+1.
     int64_t replays = 0;
-    const int64_t max_replays = (*std::max_element(maxit.begin(), maxit.end()) + kGraphIters) / kGraphIters + 64;
+    const int64_t max_replays =
+        ((int64_t)*std::max_element(maxit.begin(), maxit.end()) + kGraphIters) / kGraphIters + 64;
     int finished_since = 0;
     int64_t open_work = 0;
     for (int i = 0; i < ni; ++i)
@@ -465,6 +466,7 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
     std::vector<double> check_ref(ni, -1.0);
     std::vector<int> check_stale(ni, 0);
     std::vector<char> check_req(ni, 0);
+    std::vector<int> rep_it(ni, 0);       // iteration count when the last true residual was requested
     bool stopped = false;
     const double rf2 = opts->replace_factor * opts->replace_factor;
     auto launch_replay = [&]() -> int {
@@ -485,7 +487,12 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
             return rc;
         }
         for (;;) {
-            SIPG_HIP_TRY(hipEventSynchronize(ev[cur]));
+            const hipError_t es = hipEventSynchronize(ev[cur]);
+            if (es != hipSuccess) {          // cleanup() still drains the stream and frees the graph
+                set_last_hip_error(es);
+                rc = SIPG_ERR_HIP;
+                break;
+            }
             const cg_state *hs = cur ? hs1 : hs0;
             std::vector<char> repl(ni, 0);
             bool any_repl = false;
@@ -517,13 +524,15 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
                     }
                     if (gain) check_ref[i] = hs[i].rr_ref;
                 }
-                const bool check = true_stop && (c.flags & kSlotWantCheck) && c.rr <= 0.0625 * hs[i].rr_ref;
+                const bool check = true_stop && (c.flags & kSlotWantCheck) &&
+                                   (c.rr <= 0.0625 * hs[i].rr_ref || c.it - rep_it[i] >= force_check_interval(c.it));
                 const bool periodic = rf2 > 0.0 && c.rr <= rf2 * hs[i].rr_ref;
                 if (check || periodic) {
                     repl[i] = 1;
                     any_repl = true;
                     ++nrep_req[i];
                     check_req[i] = check ? 1 : 0;
+                    rep_it[i] = c.it;
                 }
             }
             // the replacement set of replay j+2 (empty most of the time)

@@ -61,6 +61,7 @@ struct Request {
     bool check_req = false;          // the last one requested is a final check
     double check_ref = -1.0;         // smallest true r.r of the failed final checks
     int check_stale = 0;             // failed checks without a 10% gain on it
+    int rep_it = 0;                  // iteration count when the last true residual was requested
     sipg_cg_info info{};
 };
 
@@ -631,12 +632,14 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
                 if (gain) R.check_ref = st.rr_ref;
             }
             still.push_back(id);
-            const bool check = (c.flags & kSlotWantCheck) && c.rr <= 0.0625 * st.rr_ref;
+            const bool check = (c.flags & kSlotWantCheck) &&
+                               (c.rr <= 0.0625 * st.rr_ref || c.it - R.rep_it >= force_check_interval(c.it));
             const bool periodic = c.rr <= kServeReplace * kServeReplace * st.rr_ref;
             if (check || periodic) {
                 repl.push_back(id);
                 ++R.nrep_req;
                 R.check_req = check;
+                R.rep_it = c.it;
             }
         }
         S->active.swap(still);

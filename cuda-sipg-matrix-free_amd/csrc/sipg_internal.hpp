@@ -80,6 +80,13 @@ constexpr int kStallChecks = 2;
 // of that swing: two checks in a row can miss a 10% gain well above the
 // FP64 floor.
 constexpr int kStallChecksPcg = 4;
+// Once the recursion has met the tolerance, a final check is requested when
+// it has fallen 4x below the last true residual, and also after this many
+// iterations since the last true residual was requested: near the FP64
+// floor that 4x drop can take thousands of iterations while the true
+// residual creeps down by more than 10% per check, and the stall count
+// above only advances with checks.
+inline int force_check_interval(int it) { return it / 8 > 64 ? it / 8 : 64; }
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b

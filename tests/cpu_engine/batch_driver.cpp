@@ -83,6 +83,8 @@ static void run(const char *name, const std::vector<int> &dim, const std::vector
 int main() {
     run("batch", {6, 4, 5, 3, 7, 4, 2}, {3, 2, 4, 5, 2, 13, 16}, {}, 0);
     run("batch_cap", {6, 4, 5}, {3, 2, 4}, {}, 5);
+    // maxit = INT_MAX as "no cap": the replay bound must not overflow
+    run("batch_maxit_intmax", {6, 4, 5}, {3, 2, 4}, {}, 2147483647);
     run("batch_breakdown", {6, 6, 5}, {3, 3, 4}, {80.0, -500.0, 125.0}, 0);
     run("batch_pcg", {6, 4, 5, 3, 7, 4, 2}, {3, 2, 4, 5, 2, 13, 16}, {}, 0, true);
     run("batch_pcg_other_pen", {6, 4}, {3, 2}, {81.0, 45.0}, 0, true);

@@ -145,6 +145,12 @@ def test_batch_engine_iteration_cap(batch_results):
         assert r["rc"] == 6 and r["status"] == 6 and r["iters"] == 5, r
 
 
+def test_batch_engine_unbounded_maxit(batch_results):
+    # maxit = INT_MAX: every instance converges as with the default cap
+    for r in batch_results["batch_maxit_intmax"]:
+        assert r["rc"] == 0 and r["status"] == 0 and r["relres"] <= 1e-10, r
+
+
 def test_batch_engine_breakdown_is_per_instance(batch_results):
     rows = batch_results["batch_breakdown"]
     assert all(r["rc"] == 8 for r in rows)
