@@ -121,6 +121,14 @@ int sipg_vec_xpby(int dtype, int64_t n, const void *r, double beta, void *p, voi
 int sipg_vec_copy(int dtype, int64_t n, const void *x, void *y, void *stream);
 int sipg_vec_alpha_update(int dtype, int64_t n, double alpha, void *x, const void *p, void *r,
                           const void *Ap, void *stream);
+/* sipg_vec_dot without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops
+ * that must not call hipMalloc, harnesses that forbid it).  Same arguments,
+ * result and summation order as sipg_vec_dot, for any n >= 0.  Its scratch
+ * (the block sums and the sum, 1025 doubles) is a fixed device array of
+ * the library, one per device, held until the process ends; calls that
+ * overlap in time on one device race on it. */
+int sipg_vec_dot_noalloc(int dtype, int64_t n, const void *a, const void *b, double *result, void *stream);
 
 /* ---- CG solve ---------------------------------------------------------- */
 /* Unpreconditioned CG on A x = b with device-resident scalars.  Replaces
