@@ -33,7 +33,7 @@ int sipg_internal::ensure_ws(sipg_op *op, size_t bytes) {
 
 sipg_internal::parts sipg_internal::part_layout(sipg_op *op) {
     parts p;
-    p.napply = apply_num_blocks(op->dtype, op->p, op->dimx, op->dimy);
+    p.napply = apply_num_blocks(op->dtype, op->p, op->dimx, op->dimy, op->kernel);
     p.pAp = op->d_part;
     p.rr = op->d_part + std::max(p.napply, kMaxPartials);
     p.bb = p.rr + kMaxPartials;
@@ -159,7 +159,9 @@ int sipg_op_create(sipg_op **out, int dimx, int dimy, int p, double pen, int dty
         if ((e = hipMalloc(&op->d_tab, tl * sizeof(float))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
         if ((e = hipMemcpy(op->d_tab, tf.data(), tl * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_HIP); }
     }
-    const int napply = apply_num_blocks(dtype, p, dimx, dimy);
+    // room for the block sums of either contraction kernel (sipg_op_set_kernel)
+    const int napply = std::max(apply_num_blocks(dtype, p, dimx, dimy, SIPG_KERNEL_VALU),
+                                apply_num_blocks(dtype, p, dimx, dimy, SIPG_KERNEL_MFMA));
     if (napply < 1) return fail(SIPG_ERR_UNSUPPORTED);
     const size_t npart = (size_t)std::max(napply, kMaxPartials) + 4 * (size_t)kMaxPartials;
     if ((e = hipMalloc(&op->d_part, npart * sizeof(double))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
@@ -210,7 +212,20 @@ int sipg_apply(sipg_op *op, const void *x, void *y) {
     if (!op || !x || !y) return SIPG_ERR_INVALID;
     if (overlaps(x, y, (size_t)op->n * dtype_size(op->dtype))) return SIPG_ERR_ALIAS;
     return launch_apply(op->dtype, op->p, x, y, op->d_tab, op->dimx, op->dimy, op->pen, nullptr,
-                        nullptr, 0, op->stream);
+                        nullptr, 0, op->stream, op->kernel);
+}
+
+int sipg_op_set_kernel(sipg_op *op, int kernel) {
+    if (!op || (kernel != SIPG_KERNEL_AUTO && kernel != SIPG_KERNEL_VALU && kernel != SIPG_KERNEL_MFMA))
+        return SIPG_ERR_INVALID;
+    op->kernel = kernel;
+    return SIPG_OK;
+}
+
+int sipg_op_kernel(const sipg_op *op, int dtype, int *kernel) {
+    if (!op || !kernel || (dtype != SIPG_F64 && dtype != SIPG_F32)) return SIPG_ERR_INVALID;
+    *kernel = resolve_kernel(dtype, op->p, op->kernel);
+    return SIPG_OK;
 }
 
 int sipg_dot(sipg_op *op, const void *a, const void *b, double *result) {
@@ -365,7 +380,7 @@ int sipg_time_apply(sipg_op *op, const void *x, void *y, int reps, double *ms_pe
     SIPG_HIP_TRY(hipEventRecord(a, op->stream));
     for (int i = 0; i < reps && rc == SIPG_OK; ++i)
         rc = launch_apply(op->dtype, op->p, x, y, op->d_tab, op->dimx, op->dimy, op->pen, nullptr,
-                          nullptr, 0, op->stream);
+                          nullptr, 0, op->stream, op->kernel);
     SIPG_HIP_TRY(hipEventRecord(b, op->stream));
     SIPG_HIP_TRY(hipEventSynchronize(b));
     float ms = 0.f;

@@ -218,14 +218,23 @@ using AllDegrees = Dispatch<T, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15
 
 }  // namespace
 
-int apply_num_blocks(int dtype, int p, int dimx, int dimy) {
+int resolve_kernel(int dtype, int p, int kernel) {
+    // AUTO: the matrix-core kernel wherever one exists
+    if (kernel != SIPG_KERNEL_VALU && mfma_apply_available(dtype, p)) return SIPG_KERNEL_MFMA;
+    return SIPG_KERNEL_VALU;
+}
+
+int apply_num_blocks(int dtype, int p, int dimx, int dimy, int kernel) {
+    if (resolve_kernel(dtype, p, kernel) == SIPG_KERNEL_MFMA) return mfma_apply_num_blocks(dimx, dimy);
     return dtype == SIPG_F64 ? AllDegrees<double>::nblocks(p, dimx, dimy)
                              : AllDegrees<float>::nblocks(p, dimx, dimy);
 }
 
 int launch_apply(int dtype, int p, const void *x, void *y, const void *tab, int dimx, int dimy,
-                 double pen, double *partials, const cg_state *st, int slot, hipStream_t s)
+                 double pen, double *partials, const cg_state *st, int slot, hipStream_t s, int kernel)
 {
+    if (resolve_kernel(dtype, p, kernel) == SIPG_KERNEL_MFMA)
+        return launch_apply_mfma(dtype, p, x, y, dimx, dimy, pen, partials, st, slot, s);
     if (dtype == SIPG_F64)
         return AllDegrees<double>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, slot, s);
     return AllDegrees<float>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, slot, s);

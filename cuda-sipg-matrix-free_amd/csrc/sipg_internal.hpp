@@ -134,13 +134,22 @@ bool batch_shape_supported(int p, int shape);
 constexpr int kMaxPartials = 2048;
 constexpr int kVecThreads = 256;
 
-// ---- launchers (kernels_apply.hip) ----------------------------------------
+// ---- launchers (kernels_apply.hip, kernels_mfma.hip) ----------------------
+// The contraction kernel an apply of (dtype, p) runs for a requested
+// SIPG_KERNEL_* setting: SIPG_KERNEL_VALU or SIPG_KERNEL_MFMA.
+int resolve_kernel(int dtype, int p, int kernel);
 // Number of thread blocks the fused apply uses for a mesh (= partial count).
-int apply_num_blocks(int dtype, int p, int dimx, int dimy);
+int apply_num_blocks(int dtype, int p, int dimx, int dimy, int kernel = SIPG_KERNEL_AUTO);
 // y = A x; with partials != nullptr also writes per-block sums of x.y and
 // returns early when st->s[slot].done is set.
 int launch_apply(int dtype, int p, const void *x, void *y, const void *tab, int dimx, int dimy,
-                 double pen, double *partials, const cg_state *st, int slot, hipStream_t s);
+                 double pen, double *partials, const cg_state *st, int slot, hipStream_t s,
+                 int kernel = SIPG_KERNEL_AUTO);
+// the matrix-core apply (FP32, p = 15)
+bool mfma_apply_available(int dtype, int p);
+int mfma_apply_num_blocks(int dimx, int dimy);
+int launch_apply_mfma(int dtype, int p, const void *x, void *y, int dimx, int dimy, double pen, double *partials,
+                      const cg_state *st, int slot, hipStream_t s);
 
 // ---- launchers (kernels_vec.hip) ------------------------------------------
 int vec_num_blocks(int64_t n);
@@ -294,6 +303,7 @@ struct sipg_op {
     int dimx = 0, dimy = 0, p = 0, dtype = SIPG_F64;
     double pen = 0;
     int64_t n = 0;
+    int kernel = SIPG_KERNEL_AUTO;    // contraction kernel setting (sipg_op_set_kernel)
     hipStream_t stream = nullptr;
     std::vector<double> h_tab;        // host copy (double) of the tables
     void *d_tab = nullptr;            // tables in the op's dtype

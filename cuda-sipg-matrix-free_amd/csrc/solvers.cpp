@@ -68,7 +68,7 @@ struct Engine {
         int rc;
         // r = b - A x ; p = r ; r.r ; b.b    (conjugate_gradient_gpu.hpp:40-60)
         if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
-                               opO->pen, nullptr, nullptr, 0, s)))
+                               opO->pen, nullptr, nullptr, 0, s, opO->kernel)))
             return rc;
         if (mixed) SIPG_HIP_TRY(hipMemsetAsync(xI, 0, (size_t)opI->n * dtype_size(opI->dtype), s));
         if (pcg) {
@@ -109,7 +109,7 @@ struct Engine {
         const int slot = (int)(launched & 1);
         int rc;
         if ((rc = launch_apply(opI->dtype, opI->p, p, Ap, opI->d_tab, opI->dimx, opI->dimy, opI->pen,
-                               P.pAp, st, slot, s)))
+                               P.pAp, st, slot, s, opI->kernel)))
             return rc;
         const double *src = P.pAp;
         int np = P.napply;
@@ -129,7 +129,7 @@ struct Engine {
             if ((rc = launch_pcg_x(opI->dtype, xI, p, opI->n, src, np, nb, st, slot, s))) return rc;
             if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s))) return rc;
             if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
-                                   opO->pen, nullptr, nullptr, 0, s)))
+                                   opO->pen, nullptr, nullptr, 0, s, opO->kernel)))
                 return rc;
             if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, nullptr, opI->n, P.rr,
                                            nullptr, nb, s)))
@@ -158,7 +158,7 @@ struct Engine {
         const int slot = (int)(launched & 1);
         int rc;
         if ((rc = launch_apply(opI->dtype, opI->p, p, Ap, opI->d_tab, opI->dimx, opI->dimy, opI->pen,
-                               P.pAp, st, slot, s)))
+                               P.pAp, st, slot, s, opI->kernel)))
             return rc;
         const double *src = P.pAp;
         int np = P.napply;
@@ -172,7 +172,7 @@ struct Engine {
         if (replace) {
             if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s))) return rc;
             if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
-                                   opO->pen, nullptr, nullptr, 0, s)))
+                                   opO->pen, nullptr, nullptr, 0, s, opO->kernel)))
                 return rc;
             if ((rc = launch_true_residual(opO->dtype, opI->dtype, b, AxO, r, nullptr, opI->n, P.rr,
                                            nullptr, nb, s)))
@@ -421,6 +421,7 @@ int solve_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, s
     }
     sipg_op *tw = op->twin;
     tw->stream = op->stream;
+    tw->kernel = op->kernel;
     if (pcg && (rc = ensure_prec(tw))) return rc;
     if (two_level && (rc = ensure_coarse(tw))) return rc;       // coarse solve in FP64, vectors in FP32
     const size_t v32 = (size_t)op->n * 4, v64 = (size_t)op->n * 8;
@@ -541,7 +542,7 @@ int sipg_true_relres(sipg_op *op, const void *x, const void *b, double *relres) 
     void *Ax = ws + 2 * vb;
     void *r = ws;
     if ((rc = launch_apply(op->dtype, op->p, x, Ax, op->d_tab, op->dimx, op->dimy, op->pen, nullptr,
-                           nullptr, 0, op->stream)))
+                           nullptr, 0, op->stream, op->kernel)))
         return rc;
     if ((rc = launch_residual(op->dtype, b, Ax, r, op->n, op->stream))) return rc;
     double rr = 0, bb = 0;

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libsipg_hip.so")
 F64, F32 = 0, 1
 TOL_ABS, TOL_REL = 0, 1
 METHOD_CG, METHOD_PCG = 0, 1
+KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA = 0, 1, 2
 
 OK = 0
 ERR_INVALID = 1
@@ -85,6 +86,8 @@ def _declare(L):
         "sipg_op_size": (i, [_vp, ctypes.POINTER(i64)]),
         "sipg_op_set_stream": (i, [_vp, _vp]),
         "sipg_op_tables": (i, [_vp, _dp, _dp, _dp, _dp]),
+        "sipg_op_set_kernel": (i, [_vp, i]),
+        "sipg_op_kernel": (i, [_vp, i, _ip]),
         "sipg_apply": (i, [_vp, _vp, _vp]),
         "sipg_dot": (i, [_vp, _vp, _vp, _dp]),
         "sipg_axpy": (i, [_vp, d, _vp, _vp]),
@@ -257,6 +260,17 @@ class Operator:
     def to_reference_layout(self, dev, ref):
         """Element-major device array -> the reference order (async)."""
         return lib().sipg_to_reference_layout(self.h, dev.ptr, ref.ptr)
+
+    def set_kernel(self, kernel):
+        """KERNEL_AUTO, KERNEL_VALU or KERNEL_MFMA (sipg_op_set_kernel)."""
+        check(lib().sipg_op_set_kernel(self.h, kernel), "op_set_kernel")
+
+    def kernel(self, dtype=None):
+        """The contraction kernel the op's applies (or, with dtype=F32 on an
+        FP64 op, its mixed-precision inner cycles) use."""
+        k = ctypes.c_int()
+        check(lib().sipg_op_kernel(self.h, self.dtype if dtype is None else dtype, ctypes.byref(k)), "op_kernel")
+        return k.value
 
     def apply(self, x, y):
         return lib().sipg_apply(self.h, x.ptr, y.ptr)

@@ -93,6 +93,18 @@ int sipg_op_set_stream(sipg_op *op, void *stream);
  * (device_const.hpp:115-127), K1 (laplacian_operator.hpp:31-43). */
 int sipg_op_tables(const sipg_op *op, double *nodes, double *weights, double *D, double *K1);
 
+/* Contraction kernel of the op's applies.  No counterpart in the
+ * reference: SIPG_KERNEL_VALU runs the element contractions on the vector
+ * ALUs, SIPG_KERNEL_MFMA on the matrix cores where an MFMA kernel exists
+ * for the op's dtype and degree (FP32, p = 15) and on the vector ALUs
+ * elsewhere, SIPG_KERNEL_AUTO (the default) picks the faster one.  A mixed-
+ * precision solve passes the setting of the FP64 op to its FP32 inner
+ * cycles.  sipg_op_kernel reports the kernel the op's own applies use
+ * (SIPG_KERNEL_VALU or SIPG_KERNEL_MFMA); FP32 twin: pass its dtype. */
+enum { SIPG_KERNEL_AUTO = 0, SIPG_KERNEL_VALU = 1, SIPG_KERNEL_MFMA = 2 };
+int sipg_op_set_kernel(sipg_op *op, int kernel);
+int sipg_op_kernel(const sipg_op *op, int dtype, int *kernel);
+
 /* y = A x.  Replaces sipg_sem_2d<T>::_mvm (sipg_sem_2d.hpp:167-201): the
  * three launches volume + flux_term6a + flux_term6b become one fused
  * kernel.  x, y: device arrays of n elements of the op's dtype; they must

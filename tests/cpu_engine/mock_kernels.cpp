@@ -23,7 +23,9 @@ namespace sipg_internal {
 static const int kMockApplyBlocks = 3;
 static const int kMockVecBlocks = 2;
 
-int apply_num_blocks(int, int p, int, int) { return p <= kMaxDegree ? kMockApplyBlocks : -1; }
+int apply_num_blocks(int, int p, int, int, int) { return p <= kMaxDegree ? kMockApplyBlocks : -1; }
+// the CPU stand-in has one contraction path; the setting is only recorded
+int resolve_kernel(int, int, int) { return SIPG_KERNEL_VALU; }
 int vec_num_blocks(int64_t) { return kMockVecBlocks; }
 
 template <typename T> static T *tp(void *p) { return (T *)p; }
@@ -62,7 +64,7 @@ static void apply_aos(int dimx, int dimy, int p, double pen, const T *x, T *y) {
 }
 
 int launch_apply(int dtype, int p, const void *x, void *y, const void *, int dimx, int dimy,
-                 double pen, double *partials, const cg_state *st, int slot, hipStream_t) {
+                 double pen, double *partials, const cg_state *st, int slot, hipStream_t, int) {
     if (partials && (st->s[slot].done | st->err)) return SIPG_OK;
     const int64_t n = (int64_t)dimx * dimy * (p + 1) * (p + 1);
     if (dtype == SIPG_F64) apply_aos<double>(dimx, dimy, p, pen, tp<double>(x), tp<double>(y));
