@@ -80,11 +80,24 @@ def gpu_solve(sipg, op, b, x, method):
     return t1 - t0, info
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(pen, modes, gpu_iters, n_dof):
-    """The oracle (the reference's kernels restated in C, OpenMP) on the
-    host cores: per-CG-iteration time on the same workload, extrapolated
-    to the iteration count of the GPU's unpreconditioned CG solve (the
-    oracle restates the reference's CG, not its preconditioned variant)."""
+    """The oracle (the reference's kernels and its unpreconditioned CG
+    restated in C, OpenMP over elements) on the host cores: the median of 5
+    timed runs of k CG iterations on the same workload, extrapolated to the
+    iteration count of the GPU's unpreconditioned CG solve.  The GPU's
+    `value` comes from its fastest solver (two-level PCG), a different
+    algorithm; the GPU CG solve is config.fp64_plain."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     b = O.rhs(DIM, P, pen, O.problem(0, modes))
@@ -96,22 +109,29 @@ def cpu_baseline(pen, modes, gpu_iters, n_dof):
         O.apply(DIM, DIM, P, pen, b)
         reps.append(time.perf_counter() - t0)
     t_apply = float(np.median(reps))
-    # about 15 s of CG iterations, at least 5
+    # 5 runs of k iterations, about 15 s in all
     t0 = time.perf_counter()
     O.cg(DIM, DIM, P, pen, b, x0=x, tol=0.0, tol_mode=1, maxit=3)
-    k = int(max(5, min(1000, 15.0 / ((time.perf_counter() - t0) / 3))))
-    t0 = time.perf_counter()
-    O.cg(DIM, DIM, P, pen, b, x0=x, tol=0.0, tol_mode=1, maxit=k)
-    t_iter = (time.perf_counter() - t0) / k
+    k = int(max(3, min(200, 3.0 / ((time.perf_counter() - t0) / 3))))
+    runs = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.cg(DIM, DIM, P, pen, b, x0=x, tol=0.0, tol_mode=1, maxit=k)
+        runs.append((time.perf_counter() - t0) / k)
+    t_iter = float(np.median(runs))
     t_solve = t_iter * gpu_iters
     return {
         "value": n_dof / t_solve,
         "unit": "DoF/s",
         "cores": O.num_threads(),
         "kind": "port",
-        "sample": (f"{k} CG iterations of the 256x256 p=8 FP64 solve with the OpenMP oracle "
-                   f"({t_iter * 1e3:.1f} ms/iteration, apply {t_apply * 1e3:.1f} ms), "
-                   f"extrapolated to the GPU CG solve's {gpu_iters} iterations"),
+        "cpu_model": _cpu_model(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS", "unset"),
+        "nproc": os.cpu_count(),
+        "algorithm": "unpreconditioned CG (the reference's conjugate_gradient), FP64",
+        "sample": (f"median of 5 runs of {k} CG iterations of the 256x256 p=8 FP64 solve with the OpenMP oracle "
+                   f"({t_iter * 1e3:.1f} ms/iteration, min {min(runs) * 1e3:.1f}, max {max(runs) * 1e3:.1f}; "
+                   f"apply {t_apply * 1e3:.1f} ms), extrapolated to the GPU CG solve's {gpu_iters} iterations"),
     }
 
 
@@ -140,6 +160,127 @@ def profile_pcg2(sipg, op, b, reps):
         sipg.check(sipg.lib().sipg_vec_copy(sipg.F64, op.n, x.ptr, z.ptr, None), "vec_copy")
     sipg.synchronize()
     log(f"profile-pcg2: {reps} two-level PCG solves and {reps} copies of {op.n} doubles")
+
+
+C3_DIM, C3_P = 128, 15
+
+
+def c3_problem(sipg):
+    """C3: 128 x 128 elements, p = 15, seeded Fourier forcing with 1-8 modes,
+    m, n <= 12 (seed 0), pen = 5 (p+1)^2."""
+    pen = sipg.pen_sigma(C3_P)
+    modes = sipg.forcing_draw(0, 1, 8, 12)
+    op = sipg.Operator(C3_DIM, C3_DIM, C3_P, pen, sipg.F64)
+    b = op.vector()
+    op.rhs_fourier(modes, b)
+    return op, modes, b
+
+
+def c3_leg(sipg, steps):
+    """C3 in full: FP32 apply times of both contraction kernels, the FP64
+    two-level PCG solve, and the mixed solve (FP32 two-level PCG cycles in
+    FP64 refinement) with the FP32 contractions on the matrix cores and, for
+    comparison, on the vector ALUs; errors against the FP64 solve and the
+    exact solution."""
+    op, modes, b = c3_problem(sipg)
+    n = op.n
+    op32 = sipg.Operator(C3_DIM, C3_DIM, C3_P, op.pen, sipg.F32)
+    xa, ya = op32.vector(), op32.vector()
+    xa.upload(np.random.default_rng(1).uniform(-1, 1, n).astype(np.float32))
+    apply_us = {}
+    for name, k in (("mfma", sipg.KERNEL_MFMA), ("valu", sipg.KERNEL_VALU)):
+        op32.set_kernel(k)
+        op32.time_apply(xa, ya, 20)
+        apply_us["f32_" + name] = 1e3 * op32.time_apply(xa, ya, 200)
+    xa64, ya64 = op.vector(), op.vector()
+    xa64.upload(np.random.default_rng(1).uniform(-1, 1, n))
+    op.time_apply(xa64, ya64, 20)
+    apply_us["f64"] = 1e3 * op.time_apply(xa64, ya64, 200)
+    del xa, ya, xa64, ya64, op32
+
+    def timed(method, kernel=None):
+        if kernel is not None:
+            op.set_kernel(kernel)
+        x = op.vector()
+        gpu_solve(sipg, op, b, x, method)                      # warm-up (tables, FP32 twin)
+        ts = []
+        for _ in range(max(steps, 3)):
+            t, info = gpu_solve(sipg, op, b, x, method)
+            ts.append(t)
+        rel = op.true_relres(x, b)
+        if not rel <= TOL:
+            raise RuntimeError(f"C3 {method} true relative residual {rel:.3e} above {TOL}")
+        return x, 1e3 * float(np.median(ts)), info, rel
+
+    x64, ms64, i64, r64 = timed("pcg2", sipg.KERNEL_AUTO)
+    h64 = op.download(x64)
+    out = {"workload": "128x128 elements, p=15, 4,194,304 DoF, seeded Fourier forcing (seed 0, 1-8 modes, "
+                       "m,n<=12), pen=5(p+1)^2, 1e-10 true relative residual",
+           "dof": n, "apply_us": apply_us,
+           "apply_gbs": {k: 2 * (8 if k == "f64" else 4) * n / (v * 1e-6) / 1e9 for k, v in apply_us.items()},
+           "fp64_pcg2": {"ms_per_solve": ms64, "iterations": i64.iters, "true_relres": r64,
+                         "dof_per_s": n / (ms64 * 1e-3)}}
+    for name, k in (("mixed", sipg.KERNEL_MFMA), ("mixed_valu", sipg.KERNEL_VALU)):
+        xm, msm, im, rm = timed("mixed", k)
+        hm = op.download(xm)
+        out[name] = {"contraction_kernel": "MFMA" if op.kernel(sipg.F32) == sipg.KERNEL_MFMA else "VALU",
+                     "ms_per_solve": msm, "iterations_fp32": im.iters, "refinement_cycles": im.replacements,
+                     "true_relres": rm, "dof_per_s": n / (msm * 1e-3),
+                     "time_vs_fp64_pcg2": msm / ms64,
+                     "error_vs_fp64_solution": {
+                         "rel_l2": float(np.linalg.norm(hm - h64) / np.linalg.norm(h64)),
+                         "rel_max": float(np.max(np.abs(hm - h64)) / np.max(np.abs(h64)))},
+                     "error_vs_exact": {k2: float(v) for k2, v in op.err_norms_fourier(modes, xm).items()}}
+        log(f"c3 {name}: {msm:.1f} ms/solve ({im.iters} FP32 iterations, {im.replacements} cycles), "
+            f"FP64 two-level PCG {ms64:.1f} ms ({i64.iters} iterations), ratio {msm / ms64:.2f}, "
+            f"rel L2 vs FP64 {out[name]['error_vs_fp64_solution']['rel_l2']:.2e}")
+    op.set_kernel(sipg.KERNEL_AUTO)
+    out["fp64_pcg2"]["error_vs_exact"] = {k2: float(v) for k2, v in op.err_norms_fourier(modes, x64).items()}
+    log(f"c3 apply: FP32 MFMA {apply_us['f32_mfma']:.2f} us, FP32 VALU {apply_us['f32_valu']:.2f} us, "
+        f"FP64 {apply_us['f64']:.2f} us")
+    return out
+
+
+def c1_leg(sipg):
+    """C1 (8 x 8, p = 3, 1,024 DoF): the reference's CG (absolute tolerance
+    semantics, relative 1e-10 here), two-level PCG and the mixed solver,
+    against the dense-LAPACK solution committed with the tests."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "c1_solution.npz"))
+    dim, p = 8, 3
+    op = sipg.Operator(dim, dim, p, sipg.pen_sigma(p), sipg.F64)
+    b = op.vector()
+    op.rhs_fourier((g["a"], g["m"], g["n"]), b)
+    rhs_err = float(np.max(np.abs(op.download(b) - g["rhs"])) / np.max(np.abs(g["rhs"])))
+    out = {"workload": "8x8 elements, p=3, 1,024 DoF, seeded Fourier forcing (seed 0, 4 modes, m,n<=4)",
+           "rhs_rel_diff_vs_golden": rhs_err}
+    u = g["u"]
+    for method in ("fp64", "pcg2", "mixed"):
+        x = op.vector()
+        t, info = gpu_solve(sipg, op, b, x, method)
+        t, info = gpu_solve(sipg, op, b, x, method)
+        xh = op.download(x)
+        out[method] = {"ms_per_solve": 1e3 * t, "iterations": info.iters, "true_relres": op.true_relres(x, b),
+                       "rel_l2_vs_lapack": float(np.linalg.norm(xh - u) / np.linalg.norm(u)),
+                       "rel_max_vs_lapack": float(np.max(np.abs(xh - u)) / np.max(np.abs(u)))}
+    log(f"c1: CG {out['fp64']['iterations']} it, rel L2 vs LAPACK {out['fp64']['rel_l2_vs_lapack']:.2e}; "
+        f"mixed rel L2 {out['mixed']['rel_l2_vs_lapack']:.2e}")
+    return out
+
+
+def profile_apply_c3(sipg, reps):
+    """N FP32 applies of the C3 operator (128 x 128, p = 15) with each
+    contraction kernel and N copies of one FP32 vector of the same length
+    (for the kernel-trace summary of both kernels)."""
+    op = sipg.Operator(C3_DIM, C3_DIM, C3_P, sipg.pen_sigma(C3_P), sipg.F32)
+    x, y, z = op.vector(), op.vector(), op.vector()
+    x.upload(np.random.default_rng(1).uniform(-1, 1, op.n).astype(np.float32))
+    for k in (sipg.KERNEL_MFMA, sipg.KERNEL_VALU):
+        op.set_kernel(k)
+        op.time_apply(x, y, reps)
+    for _ in range(reps):
+        sipg.check(sipg.lib().sipg_vec_copy(sipg.F32, op.n, x.ptr, z.ptr, None), "vec_copy")
+    sipg.synchronize()
+    log(f"profile-apply-c3: {reps} MFMA applies, {reps} VALU applies and {reps} copies of {op.n} floats")
 
 
 def batched_leg(sipg, log_every=True):
@@ -349,6 +490,9 @@ def main():
     ap.add_argument("--profile-apply", type=int, default=0, metavar="N",
                     help="only launch N operator applies and N copies of one vector "
                          "(for hardware-counter runs), then exit")
+    ap.add_argument("--profile-apply-c3", type=int, default=0, metavar="N",
+                    help="only N FP32 p=15 applies with each contraction kernel and N copies, then exit")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 128x128 p=15 (C3) leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -357,6 +501,9 @@ def main():
 
     import sipg
     sipg.check(sipg.lib().sipg_set_device(0), "set_device")
+    if args.profile_apply_c3:
+        profile_apply_c3(sipg, args.profile_apply_c3)
+        return
     pen = sipg.pen_sigma(P)
     modes = sipg.forcing_draw(0, 4, 4, 4)
     op = sipg.Operator(DIM, DIM, P, pen, sipg.F64)
@@ -440,25 +587,38 @@ def main():
         f"vs PCG {pcg2['time_vs_pcg']:.3f}")
 
     # ---- mixed-precision solver -------------------------------------------
+    # time_vs_fp64 is like for like: the mixed solver runs two-level PCG
+    # cycles, so it is set against the FP64 two-level PCG solve; the ratios
+    # against plain FP64 CG (the reference's solver) and block-Jacobi PCG are
+    # kept under their own names
     mixed = {"status": "not run"}
     xm = op.vector()
     try:
         gpu_solve(sipg, op, b, xm, "mixed")                         # warm-up (creates the FP32 twin)
-        tm, infom = gpu_solve(sipg, op, b, xm, "mixed")
+        tms = []
+        for _ in range(max(args.steps, 3)):
+            tm, infom = gpu_solve(sipg, op, b, xm, "mixed")
+            tms.append(tm)
+        tm = float(np.median(tms))
         relm = op.true_relres(xm, b)
-        xm_h, x64_h = op.download(xm), op.download(x64)
+        xm_h, x64_h, x2_h = op.download(xm), op.download(x64), op.download(x2)
         mixed = {"status": "ok", "ms_per_solve": 1e3 * tm, "dof_per_s": n / tm,
                  "iterations_fp32": infom.iters, "refinement_cycles": infom.replacements,
                  "true_relres": relm,
-                 "error_vs_fp64_solution": {
+                 "error_vs_fp64_pcg2_solution": {
+                     "rel_l2": float(np.linalg.norm(xm_h - x2_h) / np.linalg.norm(x2_h)),
+                     "rel_max": float(np.max(np.abs(xm_h - x2_h)) / np.max(np.abs(x2_h)))},
+                 "error_vs_fp64_cg_solution": {
                      "rel_l2": float(np.linalg.norm(xm_h - x64_h) / np.linalg.norm(x64_h)),
-                     "max_abs": float(np.max(np.abs(xm_h - x64_h)))},
-                 "time_vs_fp64": tm / (ms64 * 1e-3), "time_vs_fp64_pcg": tm / (msp * 1e-3),
-                 "time_vs_fp64_pcg2": tm / (ms2 * 1e-3)}
+                     "rel_max": float(np.max(np.abs(xm_h - x64_h)) / np.max(np.abs(x64_h)))},
+                 "error_vs_exact": {k: float(v) for k, v in op.err_norms_fourier(modes, xm).items()},
+                 "time_vs_fp64": tm / (ms2 * 1e-3),
+                 "time_vs_fp64_note": "mixed two-level PCG / FP64 two-level PCG (same algorithm)",
+                 "time_vs_fp64_plain_cg": tm / (ms64 * 1e-3), "time_vs_fp64_pcg": tm / (msp * 1e-3)}
         log(f"mixed: {1e3 * tm:.1f} ms/solve, {infom.iters} FP32 iterations, "
             f"{infom.replacements} refinement cycles, true relres {relm:.2e}, "
-            f"time ratio vs FP64 CG {mixed['time_vs_fp64']:.3f}, vs FP64 PCG {mixed['time_vs_fp64_pcg']:.2f}, "
-            f"vs FP64 two-level PCG {mixed['time_vs_fp64_pcg2']:.2f}")
+            f"time ratio vs FP64 two-level PCG {mixed['time_vs_fp64']:.2f} (vs FP64 CG "
+            f"{mixed['time_vs_fp64_plain_cg']:.3f}, vs FP64 PCG {mixed['time_vs_fp64_pcg']:.2f})")
     except RuntimeError as exc:
         mixed = {"status": f"failed: {exc}"}
         log(f"mixed: {exc}")
@@ -492,6 +652,8 @@ def main():
     err = op.err_norms_fourier(modes, xs)
     log(f"error vs exact solution: max {err['max']:.3e}, L2 {err['L2']:.3e}, H1 {err['H1']:.3e}")
 
+    c1 = c1_leg(sipg)
+    c3 = None if args.no_c3 else c3_leg(sipg, args.steps)
     batched = None if args.no_batch else batched_leg(sipg)
     serving = None
     if args.serve_seconds > 0:
@@ -530,6 +692,8 @@ def main():
             "pcg2": pcg2,
             "mixed": mixed,
             "apply_us_f64": ms_apply * 1e3,
+            "c1": c1,
+            "c3": c3,
             "batched": batched,
             "serving": serving,
         },
