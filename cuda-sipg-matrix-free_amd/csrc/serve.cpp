@@ -26,6 +26,7 @@
 #include "sipg_internal.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -62,6 +63,8 @@ struct Request {
     double check_ref = -1.0;         // smallest true r.r of the failed final checks
     int check_stale = 0;             // failed checks without a 10% gain on it
     int rep_it = 0;                  // iteration count when the last true residual was requested
+    bool cancel = false;             // cancelled while running: retired at the next progress
+    sipg_request_times times{-1.0, -1.0, -1.0};
     sipg_cg_info info{};
 };
 
@@ -121,6 +124,10 @@ struct sipg_server {
     std::deque<int64_t> queue, done_q;
     std::vector<int64_t> active;             // request ids RUNNING
     int64_t next_id = 1;
+
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double timeout = 0.0;                    // seconds from submission; 0: none
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 
     static constexpr int NC = 2 + kMaxDegree + 1;
     int *count(int set, int k) { return d_counts + set * NC + k; }
@@ -398,12 +405,14 @@ int capture_graph(sipg_server &S, const std::vector<int> &degrees) {
     return SIPG_OK;
 }
 
-void finish(sipg_server &S, Request &R, const cg_state &st) {
+void finish(sipg_server &S, Request &R, const cg_state &st, bool cancelled = false) {
     const cg_slot &c = st.s[0];
     int status = SIPG_OK;
-    if (st.err == 2) status = SIPG_ERR_BREAKDOWN;
+    if (cancelled) status = SIPG_ERR_CANCELLED;
+    else if (st.err == 2) status = SIPG_ERR_BREAKDOWN;
     else if (st.err == 3) status = SIPG_ERR_NONFINITE;
     else if (!(std::sqrt(c.rr) <= st.stop) || !c.checked) status = SIPG_NOT_CONVERGED;
+    R.times.completed = S.now();
     R.info.status = status;
     R.info.iters = c.it;
     R.info.replacements = st.nrep;
@@ -416,6 +425,23 @@ void finish(sipg_server &S, Request &R, const cg_state &st) {
     S.free_slots.push_back(R.slot);
     R.slot = -1;
     S.done_q.push_back(R.id);
+}
+
+// A queued request that is cancelled or timed out completes without having
+// run (it holds no slot and no memory).
+void finish_queued(sipg_server &S, Request &R) {
+    R.info = sipg_cg_info{};
+    R.info.status = SIPG_ERR_CANCELLED;
+    R.info.res_norm = -1.0;
+    R.info.true_res_norm = -1.0;
+    R.info.b_norm = -1.0;
+    R.times.completed = S.now();
+    R.phase = DONE;
+    S.done_q.push_back(R.id);
+}
+
+bool timed_out(const sipg_server &S, const Request &R, double now) {
+    return S.timeout > 0.0 && now - R.times.submitted > S.timeout;
 }
 
 }  // namespace
@@ -587,6 +613,7 @@ int sipg_server_submit(sipg_server *S, int dim, int p, double pen, int nmodes, c
     R.tol_mode = tol_mode;
     R.maxit = (int)std::min<int64_t>(maxit > 0 ? (int64_t)maxit : len, INT_MAX);
     R.len = len;
+    R.times.submitted = S->now();
     S->reqs[R.id] = R;
     S->queue.push_back(R.id);
     if (id) *id = R.id;
@@ -602,12 +629,28 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
     // once it meets the tolerance -- the request then finishes only on
     // that true residual)
     std::vector<int64_t> repl;
+    const double now = S->now();
+    // 0: queued requests past the time limit complete as cancelled
+    if (S->timeout > 0.0) {
+        std::deque<int64_t> keep;
+        for (int64_t id : S->queue) {
+            Request &R = S->reqs.at(id);
+            if (timed_out(*S, R, now)) finish_queued(*S, R);
+            else keep.push_back(id);
+        }
+        S->queue.swap(keep);
+    }
     if (S->replayed) {
         std::vector<int64_t> still;
         for (int64_t id : S->active) {
             Request &R = S->reqs.at(id);
             const cg_state &st = S->h_state[R.slot];
             const cg_slot &c = st.s[0];
+            if (R.cancel || timed_out(*S, R, now)) {      // retired with its iterate so far
+                finish(*S, R, st, true);
+                lists_changed = true;
+                continue;
+            }
             if (st.err || c.done) {
                 finish(*S, R, st);
                 lists_changed = true;
@@ -658,6 +701,7 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
         S->free_slots.pop_back();
         S->slot_req[R.slot] = R.id;
         R.phase = RUNNING;
+        R.times.admitted = S->now();
         S->queue.pop_front();
         admitted.push_back(R.id);
         slots.push_back(R.slot);
@@ -716,11 +760,47 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
 }
 
 int sipg_server_take(sipg_server *S, int64_t *id, sipg_cg_info *info, double *x_host, int64_t n_cap) {
+    return sipg_server_take_ex(S, id, info, x_host, n_cap, nullptr);
+}
+
+int sipg_server_peek(const sipg_server *S, int64_t *id, int64_t *n) {
+    if (!S || !id) return SIPG_ERR_INVALID;
+    if (S->done_q.empty()) return SIPG_ERR_NOT_FOUND;
+    *id = S->done_q.front();
+    if (n) *n = S->reqs.at(*id).len;
+    return SIPG_OK;
+}
+
+int sipg_server_cancel(sipg_server *S, int64_t id) {
+    if (!S) return SIPG_ERR_INVALID;
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end() || it->second.phase == DONE) return SIPG_ERR_NOT_FOUND;
+    Request &R = it->second;
+    if (R.phase == QUEUED) {
+        S->queue.erase(std::find(S->queue.begin(), S->queue.end(), id));
+        finish_queued(*S, R);
+    } else {
+        R.cancel = true;
+    }
+    return SIPG_OK;
+}
+
+int sipg_server_set_timeout(sipg_server *S, double seconds) {
+    if (!S || !(seconds >= 0.0) || !std::isfinite(seconds)) return SIPG_ERR_INVALID;
+    S->timeout = seconds;
+    return SIPG_OK;
+}
+
+int sipg_server_take_ex(sipg_server *S, int64_t *id, sipg_cg_info *info, double *x_host, int64_t n_cap,
+                        sipg_request_times *times) {
     if (!S || !id) return SIPG_ERR_INVALID;
     if (S->done_q.empty()) return SIPG_ERR_NOT_FOUND;
     const int64_t rid = S->done_q.front();
     Request &R = S->reqs.at(rid);
-    if (x_host) {
+    if (x_host && R.off < 0) {                          // never ran: the initial guess x = 0
+        if (n_cap < R.len) return SIPG_ERR_INVALID;
+        std::fill(x_host, x_host + R.len, 0.0);
+    } else if (x_host) {
         if (n_cap < R.len) return SIPG_ERR_INVALID;
         const int64_t noe = (int64_t)R.dim * R.dim, nq = (int64_t)(R.p + 1) * (R.p + 1);
         // x + z: the increments after the last true residual (none unless the
@@ -734,7 +814,8 @@ int sipg_server_take(sipg_server *S, int64_t *id, sipg_cg_info *info, double *x_
     }
     *id = rid;
     if (info) *info = R.info;
-    range_free(*S, R.off, R.len);
+    if (times) *times = R.times;
+    if (R.off >= 0) range_free(*S, R.off, R.len);
     S->reqs.erase(rid);
     S->done_q.pop_front();
     return SIPG_OK;

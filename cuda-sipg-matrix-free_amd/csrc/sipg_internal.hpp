@@ -85,8 +85,10 @@ constexpr int kStallChecksPcg = 4;
 // iterations since the last true residual was requested: near the FP64
 // floor that 4x drop can take thousands of iterations while the true
 // residual creeps down by more than 10% per check, and the stall count
-// above only advances with checks.
-inline int force_check_interval(int it) { return it / 8 > 64 ? it / 8 : 64; }
+// above only advances with checks.  it / 40 is about a quarter of a decade
+// of the solve's own average rate (ten decades in `it` iterations), so a
+// solve that is still converging gains more than 10% between checks.
+inline int force_check_interval(int it) { return it / 40 > 64 ? it / 40 : 64; }
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b

@@ -57,6 +57,10 @@ class CgInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class RequestTimes(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_double), ("admitted", ctypes.c_double), ("completed", ctypes.c_double)]
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -138,6 +142,12 @@ def _declare(L):
         "sipg_server_progress": (i, [_vp, _ip]),
         "sipg_server_take": (i, [_vp, ctypes.POINTER(i64), ctypes.POINTER(CgInfo), _dp, i64]),
         "sipg_server_stats": (i, [_vp, _ip, _ip, _ip, _dp]),
+        "sipg_server_take_ex": (i, [_vp, ctypes.POINTER(i64), ctypes.POINTER(CgInfo), _dp, i64,
+                                    ctypes.POINTER(RequestTimes)]),
+        "sipg_server_cancel": (i, [_vp, i64]),
+        "sipg_server_peek": (i, [_vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "sipg_server_set_timeout": (i, [_vp, d]),
+        "sipg_vec_dot_noalloc": (i, [i, i64, _vp, _vp, _dp, _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -562,6 +572,42 @@ class Server:
             return None
         check(st, "server_take")
         return rid.value, info, x
+
+    def peek(self):
+        """(id, unknowns) of the oldest completed request, or None."""
+        rid, n = ctypes.c_int64(), ctypes.c_int64()
+        st = lib().sipg_server_peek(self.h, ctypes.byref(rid), ctypes.byref(n))
+        if st == ERR_NOT_FOUND:
+            return None
+        check(st, "server_peek")
+        return rid.value, n.value
+
+    def take_ex(self, n_solution=0):
+        """take() plus the request's times {submitted, admitted, completed}
+        (seconds since the server's creation; admitted -1: never ran).
+        n_solution = -1: a buffer of exactly the request's size."""
+        if n_solution < 0:
+            pk = self.peek()
+            if pk is None:
+                return None
+            n_solution = pk[1]
+        rid = ctypes.c_int64()
+        info = CgInfo()
+        t = RequestTimes()
+        x = np.empty(n_solution) if n_solution else None
+        st = lib().sipg_server_take_ex(self.h, ctypes.byref(rid), ctypes.byref(info),
+                                       x.ctypes.data_as(_dp) if x is not None else None, n_solution, ctypes.byref(t))
+        if st == ERR_NOT_FOUND:
+            return None
+        check(st, "server_take_ex")
+        return rid.value, info, x, {"submitted": t.submitted, "admitted": t.admitted, "completed": t.completed}
+
+    def cancel(self, rid):
+        """Status of sipg_server_cancel (OK or ERR_NOT_FOUND)."""
+        return lib().sipg_server_cancel(self.h, rid)
+
+    def set_timeout(self, seconds):
+        check(lib().sipg_server_set_timeout(self.h, float(seconds)), "server_set_timeout")
 
     def stats(self):
         q, r, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

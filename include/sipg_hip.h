@@ -340,6 +340,33 @@ int sipg_server_progress(sipg_server *s, int *n_done);
  * its memory.  SIPG_ERR_NOT_FOUND if none has completed. */
 int sipg_server_take(sipg_server *s, int64_t *id, sipg_cg_info *info, double *x_host, int64_t n_cap);
 int sipg_server_stats(const sipg_server *s, int *queued, int *running, int *done_held, double *bytes_used);
+/* When a request was submitted, admitted (-1: never, it was cancelled in
+ * the queue) and completed, in seconds on a monotonic clock that starts at
+ * sipg_server_create: admitted - submitted is the queueing delay,
+ * completed - admitted the service time. */
+typedef struct {
+    double submitted;
+    double admitted;
+    double completed;
+} sipg_request_times;
+/* Id and number of unknowns of the oldest completed request, without
+ * taking it (so a caller can size the solution buffer, or take it without
+ * one); SIPG_ERR_NOT_FOUND if none has completed. */
+int sipg_server_peek(const sipg_server *s, int64_t *id, int64_t *n);
+/* sipg_server_take that also returns the request's times (times may be
+ * null). */
+int sipg_server_take_ex(sipg_server *s, int64_t *id, sipg_cg_info *info, double *x_host, int64_t n_cap,
+                        sipg_request_times *times);
+/* Cancels request id.  A queued request completes at once, a running one
+ * at the next sipg_server_progress with its iterate so far; either is then
+ * taken like any other request, with status SIPG_ERR_CANCELLED (a request
+ * cancelled in the queue returns x = 0).  SIPG_ERR_NOT_FOUND for an unknown
+ * or already completed id. */
+int sipg_server_cancel(sipg_server *s, int64_t id);
+/* Requests not completed within `seconds` of their submission are
+ * cancelled as by sipg_server_cancel, checked at each
+ * sipg_server_progress; 0 (the default) means no limit. */
+int sipg_server_set_timeout(sipg_server *s, double seconds);
 
 /* ---- right-hand side and error norms (device) -------------------------- */
 /* Seeded Fourier forcing u = sum_k a_k sin(m_k pi x) sin(n_k pi y) drawn

@@ -3,6 +3,7 @@
 // tests/test_cpu_engine.py.
 #include "../../include/sipg_hip.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -154,6 +155,54 @@ int main() {
         }
         printf("{\"case\":\"serve_pcg_refusals\",\"create\":%d,\"pen\":%d,\"busy\":%d}\n", rc_create, rc_pen,
                rc_busy);
+    }
+    // cancellation: one request cancelled in the queue, one while running,
+    // the rest served; then a time limit that retires running requests
+    for (int variant = 0; variant < 2; ++variant) {
+        sipg_server *S = nullptr;
+        int64_t need = 0;
+        for (int k = 0; k < 6; ++k) need = std::max<int64_t>(need, (int64_t)reqs[k].dim * reqs[k].dim * (reqs[k].p + 1) * (reqs[k].p + 1));
+        // room for the largest and a little more: the others queue
+        if (sipg_server_create(&S, 48.0 * 1.2 * need, nullptr)) return 1;
+        std::vector<int64_t> ids;
+        for (int k = 0; k < 6; ++k) {
+            const Req &r = reqs[k];
+            int64_t id = 0;
+            sipg_server_submit(S, r.dim, r.p, 5.0 * (r.p + 1) * (r.p + 1), r.nm, r.a, r.m, r.n, 1e-10, SIPG_TOL_REL,
+                               0, &id);
+            ids.push_back(id);
+        }
+        int c_queued = -1, c_running = -1, c_again = -1, c_unknown = -1;
+        if (variant == 0) c_queued = sipg_server_cancel(S, ids[5]);       // still queued
+        int nd = 0;
+        sipg_server_progress(S, &nd);                                       // admits and runs a replay
+        if (variant == 0) {
+            c_running = sipg_server_cancel(S, ids[0]);                      // running
+            c_again = sipg_server_cancel(S, ids[5]);                        // already completed
+            c_unknown = sipg_server_cancel(S, 987654);
+        } else {
+            sipg_server_set_timeout(S, 1e-12);                              // everything is overdue now
+        }
+        std::map<int64_t, std::pair<sipg_cg_info, sipg_request_times>> got;
+        for (int round = 0; round < 100000 && got.size() < ids.size(); ++round) {
+            if (sipg_server_progress(S, &nd)) break;
+            for (;;) {
+                int64_t id = 0;
+                sipg_cg_info info{};
+                sipg_request_times t{};
+                std::vector<double> x(64 * 64 * 49);
+                if (sipg_server_take_ex(S, &id, &info, x.data(), (int64_t)x.size(), &t)) break;
+                got[id] = {info, t};
+            }
+        }
+        for (size_t k = 0; k < ids.size(); ++k) {
+            const auto &g = got[ids[k]];
+            printf("{\"case\":\"serve_%s\",\"k\":%zu,\"status\":%d,\"iters\":%d,\"submitted\":%.6f,\"admitted\":%.6f,"
+                   "\"completed\":%.6f,\"c_queued\":%d,\"c_running\":%d,\"c_again\":%d,\"c_unknown\":%d}\n",
+                   variant == 0 ? "cancel" : "timeout", k, g.first.status, g.first.iters, g.second.submitted,
+                   g.second.admitted, g.second.completed, c_queued, c_running, c_again, c_unknown);
+        }
+        sipg_server_destroy(S);
     }
     return 0;
 }

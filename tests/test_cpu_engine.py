@@ -213,6 +213,33 @@ def test_server_pcg_finishes_all_like_cg(serve_results):
     assert sum(r["iters"] for r in ones) < 0.8 * sum(r["cg_iters"] for r in ones)
 
 
+def test_server_cancellation(serve_results):
+    rows = [r for r in serve_results if r["case"] == "serve_cancel"]
+    assert len(rows) == 6
+    r0 = rows[0]
+    assert (r0["c_queued"], r0["c_running"], r0["c_again"], r0["c_unknown"]) == (0, 0, 11, 11)
+    # request 5, cancelled in the queue: never admitted, no iterations
+    assert rows[5]["status"] == 10 and rows[5]["admitted"] == -1.0 and rows[5]["iters"] == 0
+    # request 0, cancelled while running: retired with its iterate so far
+    assert rows[0]["status"] == 10 and rows[0]["admitted"] >= 0.0 and rows[0]["iters"] > 0
+    for r in rows[1:5]:
+        assert r["status"] == 0, r
+    for r in rows:
+        assert r["submitted"] >= 0.0 and r["completed"] >= r["submitted"], r
+        if r["admitted"] >= 0.0:
+            assert r["submitted"] <= r["admitted"] <= r["completed"], r
+
+
+def test_server_timeout(serve_results):
+    rows = [r for r in serve_results if r["case"] == "serve_timeout"]
+    assert len(rows) == 6
+    # the limit is set after the first replay: running requests retire with
+    # their iterations so far, queued ones without running
+    assert all(r["status"] == 10 for r in rows), rows
+    assert any(r["admitted"] >= 0.0 and r["iters"] > 0 for r in rows)
+    assert any(r["admitted"] == -1.0 and r["iters"] == 0 for r in rows)
+
+
 def test_server_pcg_refusals(serve_results):
     r = next(r for r in serve_results if r["case"] == "serve_pcg_refusals")
     assert r["create"] == 0 and r["pen"] == 3 and r["busy"] == 1      # UNSUPPORTED, INVALID
