@@ -159,9 +159,10 @@ int sipg_op_create(sipg_op **out, int dimx, int dimy, int p, double pen, int dty
         if ((e = hipMalloc(&op->d_tab, tl * sizeof(float))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
         if ((e = hipMemcpy(op->d_tab, tf.data(), tl * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_HIP); }
     }
-    // room for the block sums of either contraction kernel (sipg_op_set_kernel)
-    const int napply = std::max(apply_num_blocks(dtype, p, dimx, dimy, SIPG_KERNEL_VALU),
-                                apply_num_blocks(dtype, p, dimx, dimy, SIPG_KERNEL_MFMA));
+    // room for the block sums of any contraction kernel (sipg_op_set_kernel)
+    const int napply = std::max({apply_num_blocks(dtype, p, dimx, dimy, SIPG_KERNEL_VALU),
+                                 apply_num_blocks(dtype, p, dimx, dimy, SIPG_KERNEL_MFMA),
+                                 apply_num_blocks(dtype, p, dimx, dimy, SIPG_KERNEL_WAVE)});
     if (napply < 1) return fail(SIPG_ERR_UNSUPPORTED);
     const size_t npart = (size_t)std::max(napply, kMaxPartials) + 4 * (size_t)kMaxPartials;
     if ((e = hipMalloc(&op->d_part, npart * sizeof(double))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
@@ -216,7 +217,7 @@ int sipg_apply(sipg_op *op, const void *x, void *y) {
 }
 
 int sipg_op_set_kernel(sipg_op *op, int kernel) {
-    if (!op || (kernel != SIPG_KERNEL_AUTO && kernel != SIPG_KERNEL_VALU && kernel != SIPG_KERNEL_MFMA))
+    if (!op || kernel < SIPG_KERNEL_AUTO || kernel > SIPG_KERNEL_WAVE)
         return SIPG_ERR_INVALID;
     op->kernel = kernel;
     return SIPG_OK;

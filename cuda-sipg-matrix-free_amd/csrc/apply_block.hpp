@@ -277,6 +277,252 @@ __device__ __forceinline__ void block_stage(T *lds, const BlockRegs<T, P, BX, BY
     }
 }
 
+// The operator for line j of one element staged in LDS, in three parts:
+//   line_load        column j (x direction, flux_term6a) and row j (y
+//                    direction, flux_term6b) of the element's tile, their
+//                    even-odd split and the element's own face derivatives;
+//   line_neighbours  the neighbours' face derivative and trace on line j,
+//                    from their staged tiles (boundary sides are not read);
+//   line_finish      the sum-factorised contractions in even-odd form and
+//                    the face terms: Yx[a] is this line's contribution to
+//                    Y[a][j], Yy[c] to Y[j][c].
+// line_operator runs the three.
+//
+// Even-odd form: K1 is centrosymmetric and the derivative table
+// antisymmetric (DP[i] = -D0[p-i], exactly), so the contraction, the
+// element's own face derivatives and the face terms all act on the pair
+// sums ue = u[i] + u[p-i] and differences uo = u[i] - u[p-i].
+//
+// The coefficients come from a policy class C: CxCoef has them as
+// compile-time constants (immediates built in scalar registers at each
+// use), TabCoef reads them from a __constant__ table (scalar loads, several
+// coefficients per load; the lane-dependent w[j] by one vector load).
+template <typename T, int P>
+struct CxCoef {
+    static constexpr cx::Tables<P> CT = cx::make_tables<P>();
+    static constexpr cx::EvenOdd<P> EO = cx::make_even_odd<P>();
+    __device__ static T E(int a, int i) { return (T)EO.E[a][i]; }
+    __device__ static T O(int a, int i) { return (T)EO.O[a][i]; }
+    __device__ static T DA(int i) { return (T)EO.DA[i]; }
+    __device__ static T DB(int i) { return (T)EO.DB[i]; }
+    __device__ static T DM() { return (T)EO.DM; }
+    __device__ static T D0(int i) { return (T)CT.D[i][0]; }
+    __device__ static T DP(int i) { return (T)CT.D[i][P]; }
+    __device__ static T w(int i) { return (T)CT.w[i]; }
+};
+
+template <typename T, int P>
+struct CoefTable {
+    static constexpr int N1 = P + 1, H = N1 / 2, ME = (N1 + 1) / 2;
+    T E[ME][ME];
+    T O[H > 0 ? H : 1][H > 0 ? H : 1];
+    T DA[H > 0 ? H : 1], DB[H > 0 ? H : 1];
+    T DM;
+    T D0[N1], DP[N1], w[N1];
+};
+
+template <typename T, int P>
+constexpr CoefTable<T, P> make_coef_table() {
+    const cx::Tables<P> t = cx::make_tables<P>();
+    const cx::EvenOdd<P> eo = cx::make_even_odd<P>();
+    CoefTable<T, P> c{};
+    constexpr int N1 = P + 1, H = N1 / 2, ME = (N1 + 1) / 2;
+    for (int a = 0; a < ME; ++a)
+        for (int i = 0; i < ME; ++i) c.E[a][i] = (T)eo.E[a][i];
+    for (int a = 0; a < H; ++a)
+        for (int i = 0; i < H; ++i) c.O[a][i] = (T)eo.O[a][i];
+    for (int i = 0; i < H; ++i) {
+        c.DA[i] = (T)eo.DA[i];
+        c.DB[i] = (T)eo.DB[i];
+    }
+    c.DM = (T)eo.DM;
+    for (int i = 0; i < N1; ++i) {
+        c.D0[i] = (T)t.D[i][0];
+        c.DP[i] = (T)t.D[i][P];
+        c.w[i] = (T)t.w[i];
+    }
+    return c;
+}
+
+template <typename T, int P>
+__constant__ CoefTable<T, P> kCoefTable = make_coef_table<T, P>();
+
+template <typename T, int P>
+struct TabCoef {
+    __device__ static T E(int a, int i) { return kCoefTable<T, P>.E[a][i]; }
+    __device__ static T O(int a, int i) { return kCoefTable<T, P>.O[a][i]; }
+    __device__ static T DA(int i) { return kCoefTable<T, P>.DA[i]; }
+    __device__ static T DB(int i) { return kCoefTable<T, P>.DB[i]; }
+    __device__ static T DM() { return kCoefTable<T, P>.DM; }
+    __device__ static T D0(int i) { return kCoefTable<T, P>.D0[i]; }
+    __device__ static T DP(int i) { return kCoefTable<T, P>.DP[i]; }
+    __device__ static T w(int i) { return kCoefTable<T, P>.w[i]; }
+};
+
+template <typename T, int P>
+struct LineState {
+    static constexpr int N1 = P + 1, H = N1 / 2, ME = (N1 + 1) / 2;
+    T ue[ME], uo[H > 0 ? H : 1], ve[ME], vo[H > 0 ? H : 1];
+    T u0, uP, v0, vP;          // traces of the line at the element's faces
+    T dL, dR, dD, dU;          // the element's own face derivatives on the line
+};
+
+// Face derivative n* and trace t* of each neighbour on the shared face, line j.
+template <typename T>
+struct LineNbr {
+    T nL = 0, tL = 0, nR = 0, tR = 0, nD = 0, tD = 0, nU = 0, tU = 0;
+};
+
+template <typename T, int P, class C = CxCoef<T, P>>
+__device__ __forceinline__ void line_load(const T *tile, int j, LineState<T, P> &S, T (&v)[P + 1])
+{
+    constexpr int N1 = P + 1;
+    constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
+    T u[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+        u[i] = tile[i * N1 + j];
+        v[i] = tile[j * N1 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        S.ue[i] = u[i] + u[P - i];
+        S.uo[i] = u[i] - u[P - i];
+        S.ve[i] = v[i] + v[P - i];
+        S.vo[i] = v[i] - v[P - i];
+    }
+    if (ME > H) { S.ue[ME - 1] = u[H]; S.ve[ME - 1] = v[H]; }
+    S.u0 = u[0];
+    S.uP = u[P];
+    S.v0 = v[0];
+    S.vP = v[P];
+    // own face derivatives: dL = sum D0[i] u[i] = S + B, dR = sum DP[i] u[i]
+    // = B - S with S = sum DA[i] ue[i] (+ D0[h] u[h]), B = sum DB[i] uo[i]
+    T su = 0, bu = 0, sv = 0, bv = 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        su += C::DA(i) * S.ue[i];
+        bu += C::DB(i) * S.uo[i];
+        sv += C::DA(i) * S.ve[i];
+        bv += C::DB(i) * S.vo[i];
+    }
+    if (ME > H) {
+        su += C::DM() * S.ue[ME - 1];
+        sv += C::DM() * S.ve[ME - 1];
+    }
+    S.dL = su + bu;
+    S.dR = bu - su;
+    S.dD = sv + bv;
+    S.dU = bv - sv;
+}
+
+template <typename T, int P, class C = CxCoef<T, P>>
+__device__ __forceinline__ void line_neighbours(const T *tlL, const T *tlR, const T *tlD, const T *tlU, int j,
+                                                bool Lb, bool Rb, bool Db, bool Ub, LineNbr<T> &N)
+{
+    constexpr int N1 = P + 1;
+    if (!Lb) {
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+            const T t = tlL[i * N1 + j];
+            N.nL += C::DP(i) * t;
+            if (i == P) N.tL = t;
+        }
+    }
+    if (!Rb) {
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+            const T t = tlR[i * N1 + j];
+            N.nR += C::D0(i) * t;
+            if (i == 0) N.tR = t;
+        }
+    }
+    if (!Db) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            const T t = tlD[j * N1 + k];
+            N.nD += C::DP(k) * t;
+            if (k == P) N.tD = t;
+        }
+    }
+    if (!Ub) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            const T t = tlU[j * N1 + k];
+            N.nU += C::D0(k) * t;
+            if (k == 0) N.tU = t;
+        }
+    }
+}
+
+template <typename T, int P, class C = CxCoef<T, P>>
+__device__ __forceinline__ void line_finish(const LineState<T, P> &S, const LineNbr<T> &N, int j, bool Lb, bool Rb,
+                                            bool Db, bool Ub, T pen, T (&Yx)[P + 1], T (&Yy)[P + 1])
+{
+    constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
+    const T wj = C::w(j);
+    const T pen2 = pen + pen;
+    const T ev = wj * (Lb ? S.u0 : T(0.5) * (S.u0 - N.tL));
+    const T fv = wj * (Rb ? -S.uP : T(0.5) * (N.tR - S.uP));
+    const T Ax = Lb ? S.dL : T(0.5) * (S.dL + N.nL);
+    const T Bx = Rb ? -S.dR : T(-0.5) * (S.dR + N.nR);
+    const T g = wj * (Db ? S.v0 : T(0.5) * (S.v0 - N.tD));
+    const T h = wj * (Ub ? -S.vP : T(0.5) * (N.tU - S.vP));
+    const T Cy = Db ? S.dD : T(0.5) * (S.dD + N.nD);
+    const T Dy = Ub ? -S.dU : T(-0.5) * (S.dU + N.nU);
+
+    // Yx[a] = wj (K1 u)[a] + D0[a] ev + DP[a] fv, and the same for Yy with
+    // (v, g, h).  In even-odd form the face part of pair (a, p-a) is
+    // DA[a] (ev - fv) +/- DB[a] (ev + fv), and D0[h] (ev - fv) at the middle
+    // node; each constant of the E/O blocks serves both directions while live.
+    {
+        const T s0x = ev - fv, s1x = ev + fv, s0y = g - h, s1y = g + h;
+#pragma unroll
+        for (int a = 0; a < ME; ++a) {
+            T ex_ = 0, ey_ = 0, ox_ = 0, oy_ = 0;
+#pragma unroll
+            for (int i = 0; i < ME; ++i) {
+                const T c = C::E(a, i);
+                ex_ += c * S.ue[i];
+                ey_ += c * S.ve[i];
+            }
+            if (a < H) {
+#pragma unroll
+                for (int i = 0; i < H; ++i) {
+                    const T c = C::O(a, i);
+                    ox_ += c * S.uo[i];
+                    oy_ += c * S.vo[i];
+                }
+                const T e1x = wj * ex_ + C::DA(a) * s0x, o1x = wj * ox_ + C::DB(a) * s1x;
+                const T e1y = wj * ey_ + C::DA(a) * s0y, o1y = wj * oy_ + C::DB(a) * s1y;
+                Yx[a] = e1x + o1x;
+                Yx[P - a] = e1x - o1x;
+                Yy[a] = e1y + o1y;
+                Yy[P - a] = e1y - o1y;
+            } else {
+                Yx[a] = wj * ex_ + C::DM() * s0x;
+                Yy[a] = wj * ey_ + C::DM() * s0y;
+            }
+        }
+    }
+    Yx[0] += wj * Ax + (Lb ? pen : pen2) * ev;
+    Yx[P] += wj * Bx - (Rb ? pen : pen2) * fv;
+    Yy[0] += wj * Cy + (Db ? pen : pen2) * g;
+    Yy[P] += wj * Dy - (Ub ? pen : pen2) * h;
+}
+
+template <typename T, int P, class C = CxCoef<T, P>>
+__device__ __forceinline__ void line_operator(const T *tile, const T *tlL, const T *tlR, const T *tlD, const T *tlU,
+                                              int j, bool Lb, bool Rb, bool Db, bool Ub, T pen, T (&Yx)[P + 1],
+                                              T (&Yy)[P + 1], T (&v)[P + 1])
+{
+    LineState<T, P> S;
+    LineNbr<T> N;
+    line_load<T, P, C>(tile, j, S, v);
+    line_neighbours<T, P, C>(tlL, tlR, tlD, tlU, j, Lb, Rb, Db, Ub, N);
+    line_finish<T, P, C>(S, N, j, Lb, Rb, Db, Ub, pen, Yx, Yy);
+}
+
 // The operator on the staged block (x0, y0), result written to y; returns
 // this thread's share of x.y over valid elements when DOT.  Contains the
 // block's barriers; with TAIL it ends with one more, after which the LDS
@@ -297,19 +543,11 @@ __device__ __forceinline__ double block_compute(T *lds, T *__restrict__ y, int d
     constexpr int N1 = G::N1, NQ = G::NQ, TS = G::TS, NE = G::NE, NT = G::NT;
     constexpr int SL_COL = G::SL_COL, RUN = G::RUN;
 
-    // degree-p tables as compile-time constants (see basis_constexpr.hpp)
-    constexpr cx::Tables<P> CT = cx::make_tables<P>();
-    struct CW { __device__ T operator[](int i) const { return (T)CT.w[i]; } } w;
-    struct C0 { __device__ T operator[](int i) const { return (T)CT.D[i][0]; } } D0;
-    struct CP { __device__ T operator[](int i) const { return (T)CT.D[i][P]; } } DP;
-
     const int tid = threadIdx.x;
     const int s = tid % NE;
     int j = tid / NE;
     if (NE % 64 == 0) j = __builtin_amdgcn_readfirstlane(j);
     const int ex = s % BX, ey = s / BX;
-    const T wj = w[j];
-    const T pen2 = pen + pen;
     const int sl = (ey + 1) * BX + ex;                  // own LDS slot
     const T *tile = lds + sl * TS;
     const T *tlL = lds + (ex > 0 ? sl - 1 : SL_COL + ey) * TS;
@@ -322,138 +560,8 @@ __device__ __forceinline__ double block_compute(T *lds, T *__restrict__ y, int d
     const bool Lb = (gx == 0), Rb = (gx == dimx - 1);
     const bool Ub = (gy == 0), Db = (gy == dimy - 1);   // y grows downwards
 
-    T u[N1], v[N1];
-#pragma unroll
-    for (int i = 0; i < N1; ++i) {
-        u[i] = tile[i * N1 + j];
-        v[i] = tile[j * N1 + i];
-    }
-
-    // even-odd form of both lines: K1 is centrosymmetric and the derivative
-    // table antisymmetric (DP[i] = -D0[p-i], exactly), so the contraction,
-    // the element's own face derivatives and the face terms all act on the
-    // pair sums ue = u[i] + u[p-i] and differences uo = u[i] - u[p-i]
-    constexpr cx::EvenOdd<P> EO = cx::make_even_odd<P>();
-    constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
-    T ue[ME], uo[H > 0 ? H : 1], ve[ME], vo[H > 0 ? H : 1];
-#pragma unroll
-    for (int i = 0; i < H; ++i) {
-        ue[i] = u[i] + u[P - i];
-        uo[i] = u[i] - u[P - i];
-        ve[i] = v[i] + v[P - i];
-        vo[i] = v[i] - v[P - i];
-    }
-    if (ME > H) { ue[ME - 1] = u[H]; ve[ME - 1] = v[H]; }
-
-    // own face derivatives: dL = sum D0[i] u[i] = S + B, dR = sum DP[i] u[i]
-    // = B - S with S = sum DA[i] ue[i] (+ D0[h] u[h]), B = sum DB[i] uo[i]
-    T dL, dR, dD, dU;
-    {
-        T su = 0, bu = 0, sv = 0, bv = 0;
-#pragma unroll
-        for (int i = 0; i < H; ++i) {
-            su += (T)EO.DA[i] * ue[i];
-            bu += (T)EO.DB[i] * uo[i];
-            sv += (T)EO.DA[i] * ve[i];
-            bv += (T)EO.DB[i] * vo[i];
-        }
-        if (ME > H) {
-            su += (T)EO.DM * ue[ME - 1];
-            sv += (T)EO.DM * ve[ME - 1];
-        }
-        dL = su + bu;
-        dR = bu - su;
-        dD = sv + bv;
-        dU = bv - sv;
-    }
-
-    T ev, fv, Ax, Bx, g, h, Cy, Dy;
-    {
-        T nL = 0, tL = 0, nR = 0, tR = 0;
-        if (!Lb) {
-#pragma unroll
-            for (int i = 0; i < N1; ++i) {
-                const T t = tlL[i * N1 + j];
-                nL += DP[i] * t;
-                if (i == P) tL = t;
-            }
-        }
-        if (!Rb) {
-#pragma unroll
-            for (int i = 0; i < N1; ++i) {
-                const T t = tlR[i * N1 + j];
-                nR += D0[i] * t;
-                if (i == 0) tR = t;
-            }
-        }
-        ev = wj * (Lb ? u[0] : T(0.5) * (u[0] - tL));
-        fv = wj * (Rb ? -u[P] : T(0.5) * (tR - u[P]));
-        Ax = Lb ? dL : T(0.5) * (dL + nL);
-        Bx = Rb ? -dR : T(-0.5) * (dR + nR);
-    }
-    {
-        T nD = 0, tD = 0, nU = 0, tU = 0;
-        if (!Db) {
-#pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                const T t = tlD[j * N1 + k];
-                nD += DP[k] * t;
-                if (k == P) tD = t;
-            }
-        }
-        if (!Ub) {
-#pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                const T t = tlU[j * N1 + k];
-                nU += D0[k] * t;
-                if (k == 0) tU = t;
-            }
-        }
-        g = wj * (Db ? v[0] : T(0.5) * (v[0] - tD));
-        h = wj * (Ub ? -v[P] : T(0.5) * (tU - v[P]));
-        Cy = Db ? dD : T(0.5) * (dD + nD);
-        Dy = Ub ? -dU : T(-0.5) * (dU + nU);
-    }
-
-    // Yx[a] = wj (K1 u)[a] + D0[a] ev + DP[a] fv, and the same for Yy with
-    // (v, g, h).  In even-odd form the face part of pair (a, p-a) is
-    // DA[a] (ev - fv) +/- DB[a] (ev + fv), and D0[h] (ev - fv) at the middle
-    // node; each constant of the E/O blocks serves both directions while live.
-    T Yx[N1], Yy[N1];
-    {
-        const T s0x = ev - fv, s1x = ev + fv, s0y = g - h, s1y = g + h;
-#pragma unroll
-        for (int a = 0; a < ME; ++a) {
-            T ex_ = 0, ey_ = 0, ox_ = 0, oy_ = 0;
-#pragma unroll
-            for (int i = 0; i < ME; ++i) {
-                const T c = (T)EO.E[a][i];
-                ex_ += c * ue[i];
-                ey_ += c * ve[i];
-            }
-            if (a < H) {
-#pragma unroll
-                for (int i = 0; i < H; ++i) {
-                    const T c = (T)EO.O[a][i];
-                    ox_ += c * uo[i];
-                    oy_ += c * vo[i];
-                }
-                const T e1x = wj * ex_ + (T)EO.DA[a] * s0x, o1x = wj * ox_ + (T)EO.DB[a] * s1x;
-                const T e1y = wj * ey_ + (T)EO.DA[a] * s0y, o1y = wj * oy_ + (T)EO.DB[a] * s1y;
-                Yx[a] = e1x + o1x;
-                Yx[P - a] = e1x - o1x;
-                Yy[a] = e1y + o1y;
-                Yy[P - a] = e1y - o1y;
-            } else {
-                Yx[a] = wj * ex_ + (T)EO.DM * s0x;
-                Yy[a] = wj * ey_ + (T)EO.DM * s0y;
-            }
-        }
-    }
-    Yx[0] += wj * Ax + (Lb ? pen : pen2) * ev;
-    Yx[P] += wj * Bx - (Rb ? pen : pen2) * fv;
-    Yy[0] += wj * Cy + (Db ? pen : pen2) * g;
-    Yy[P] += wj * Dy - (Ub ? pen : pen2) * h;
+    T v[N1], Yx[N1], Yy[N1];
+    line_operator<T, P>(tile, tlL, tlR, tlD, tlU, j, Lb, Rb, Db, Ub, pen, Yx, Yy, v);
 
     __syncthreads();                                   // all tile reads done
 #pragma unroll

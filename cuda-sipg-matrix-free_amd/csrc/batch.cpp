@@ -461,10 +461,12 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
     std::vector<int> nrep_req(ni, 0);
     // final checks: the smallest true r.r of the failed ones, how many failed
     // checks gained less than 10% on it, and whether one is in flight;
-    // kStallChecks (PCG: kStallChecksPcg) such checks mean the FP64 floor -- the instance ends
+    // kStallChecks (PCG: kStallChecksPcg) such checks, with the best one
+    // stall_window iterations old, mean the FP64 floor -- the instance ends
     // there, not converged
     std::vector<double> check_ref(ni, -1.0);
     std::vector<int> check_stale(ni, 0);
+    std::vector<int> best_it(ni, 0);      // iteration count at the best failed check
     std::vector<char> check_req(ni, 0);
     std::vector<int> rep_it(ni, 0);       // iteration count when the last true residual was requested
     bool stopped = false;
@@ -514,7 +516,9 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
                     check_req[i] = 0;
                     const bool gain = check_ref[i] < 0.0 || hs[i].rr_ref <= 0.81 * check_ref[i];
                     if (!gain) ++check_stale[i];
-                    if (check_stale[i] >= (pcg ? kStallChecksPcg : kStallChecks)) {
+                    else best_it[i] = c.it;
+                    if (check_stale[i] >= (pcg ? kStallChecksPcg : kStallChecks) &&
+                        c.it - best_it[i] >= stall_window(c.it)) {
                         phase[i] = FINISHED;
                         stopped = true;
                         --n_open;

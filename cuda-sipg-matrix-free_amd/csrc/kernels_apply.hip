@@ -219,13 +219,20 @@ using AllDegrees = Dispatch<T, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15
 }  // namespace
 
 int resolve_kernel(int dtype, int p, int kernel) {
-    // AUTO: the matrix-core kernel wherever one exists
-    if (kernel != SIPG_KERNEL_VALU && mfma_apply_available(dtype, p)) return SIPG_KERNEL_MFMA;
+    if (kernel == SIPG_KERNEL_WAVE) return wave_apply_available(dtype, p) ? SIPG_KERNEL_WAVE : SIPG_KERNEL_VALU;
+    if (kernel == SIPG_KERNEL_MFMA) return mfma_apply_available(dtype, p) ? SIPG_KERNEL_MFMA : SIPG_KERNEL_VALU;
+    if (kernel == SIPG_KERNEL_VALU) return SIPG_KERNEL_VALU;
+    // AUTO: the fastest measured -- the matrix cores at p = 15 in FP32, the
+    // wave-owned elements at even p <= 8 (1.2-1.6x the block kernel there)
+    if (mfma_apply_available(dtype, p)) return SIPG_KERNEL_MFMA;
+    if (wave_apply_available(dtype, p)) return SIPG_KERNEL_WAVE;
     return SIPG_KERNEL_VALU;
 }
 
 int apply_num_blocks(int dtype, int p, int dimx, int dimy, int kernel) {
-    if (resolve_kernel(dtype, p, kernel) == SIPG_KERNEL_MFMA) return mfma_apply_num_blocks(dimx, dimy);
+    const int k = resolve_kernel(dtype, p, kernel);
+    if (k == SIPG_KERNEL_MFMA) return mfma_apply_num_blocks(dimx, dimy);
+    if (k == SIPG_KERNEL_WAVE) return wave_apply_num_blocks(dtype, p, dimx, dimy);
     return dtype == SIPG_F64 ? AllDegrees<double>::nblocks(p, dimx, dimy)
                              : AllDegrees<float>::nblocks(p, dimx, dimy);
 }
@@ -233,8 +240,11 @@ int apply_num_blocks(int dtype, int p, int dimx, int dimy, int kernel) {
 int launch_apply(int dtype, int p, const void *x, void *y, const void *tab, int dimx, int dimy,
                  double pen, double *partials, const cg_state *st, int slot, hipStream_t s, int kernel)
 {
-    if (resolve_kernel(dtype, p, kernel) == SIPG_KERNEL_MFMA)
+    const int k = resolve_kernel(dtype, p, kernel);
+    if (k == SIPG_KERNEL_MFMA)
         return launch_apply_mfma(dtype, p, x, y, dimx, dimy, pen, partials, st, slot, s);
+    if (k == SIPG_KERNEL_WAVE)
+        return launch_apply_wave(dtype, p, x, y, dimx, dimy, pen, partials, st, slot, s);
     if (dtype == SIPG_F64)
         return AllDegrees<double>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, slot, s);
     return AllDegrees<float>::launch(p, x, y, tab, dimx, dimy, pen, partials, st, slot, s);

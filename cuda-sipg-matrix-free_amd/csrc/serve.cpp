@@ -63,6 +63,7 @@ struct Request {
     double check_ref = -1.0;         // smallest true r.r of the failed final checks
     int check_stale = 0;             // failed checks without a 10% gain on it
     int rep_it = 0;                  // iteration count when the last true residual was requested
+    int best_it = 0;                 // iteration count at the best failed final check
     bool cancel = false;             // cancelled while running: retired at the next progress
     sipg_request_times times{-1.0, -1.0, -1.0};
     sipg_cg_info info{};
@@ -662,12 +663,15 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
             }
             if (R.check_req) {
                 // kStallChecks (PCG: kStallChecksPcg) failed final checks
-                // that gain less than 10% on the best one mean the FP64
-                // floor: the request ends there
+                // that gain less than 10% on the best one, with the best one
+                // stall_window iterations old, mean the FP64 floor: the
+                // request ends there
                 R.check_req = false;
                 const bool gain = R.check_ref < 0.0 || st.rr_ref <= 0.81 * R.check_ref;
                 if (!gain) ++R.check_stale;
-                if (R.check_stale >= (S->method == SIPG_METHOD_PCG ? kStallChecksPcg : kStallChecks)) {
+                else R.best_it = c.it;
+                if (R.check_stale >= (S->method == SIPG_METHOD_PCG ? kStallChecksPcg : kStallChecks) &&
+                    c.it - R.best_it >= stall_window(c.it)) {
                     finish(*S, R, st);
                     lists_changed = true;
                     continue;

@@ -89,6 +89,12 @@ constexpr int kStallChecksPcg = 4;
 // of the solve's own average rate (ten decades in `it` iterations), so a
 // solve that is still converging gains more than 10% between checks.
 inline int force_check_interval(int it) { return it / 40 > 64 ? it / 40 : 64; }
+// The stall count ends a solve only once its best failed check is also
+// this many iterations old: a quarter of the solve so far.  A solve that is
+// converging sets a new best (10% below the last) well within that window,
+// even when single checks land on the peaks of its residual's swings; one
+// at the FP64 floor stops about a quarter of its iterations after arriving.
+inline int stall_window(int it) { return it / 4 > 128 ? it / 4 : 128; }
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b
@@ -132,8 +138,11 @@ extern const int kBatchShapeBX[kBatchShapes];
 extern const int kBatchShapeBY[kBatchShapes];
 bool batch_shape_supported(int p, int shape);
 
-// Reduction partial buffers hold at most this many block sums.
-constexpr int kMaxPartials = 2048;
+// Reduction partial buffers hold at most this many block sums; consumers
+// add up at most this many themselves (more are first folded into one by a
+// separate reduction).  The wave-owned apply has one block sum per 7 x 4
+// elements at p = 8: 2368 at 256 x 256.
+constexpr int kMaxPartials = 4096;
 constexpr int kVecThreads = 256;
 
 // ---- launchers (kernels_apply.hip, kernels_mfma.hip) ----------------------
@@ -147,6 +156,11 @@ int apply_num_blocks(int dtype, int p, int dimx, int dimy, int kernel = SIPG_KER
 int launch_apply(int dtype, int p, const void *x, void *y, const void *tab, int dimx, int dimy,
                  double pen, double *partials, const cg_state *st, int slot, hipStream_t s,
                  int kernel = SIPG_KERNEL_AUTO);
+// the vector-ALU apply with wave-owned elements (kernels_apply_wave.hip)
+bool wave_apply_available(int dtype, int p);
+int wave_apply_num_blocks(int dtype, int p, int dimx, int dimy);
+int launch_apply_wave(int dtype, int p, const void *x, void *y, int dimx, int dimy, double pen, double *partials,
+                      const cg_state *st, int slot, hipStream_t s);
 // the matrix-core apply (FP32, p = 15)
 bool mfma_apply_available(int dtype, int p);
 int mfma_apply_num_blocks(int dimx, int dimy);

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libsipg_hip.so")
 F64, F32 = 0, 1
 TOL_ABS, TOL_REL = 0, 1
 METHOD_CG, METHOD_PCG = 0, 1
-KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA = 0, 1, 2
+KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA, KERNEL_WAVE = 0, 1, 2, 3
 
 OK = 0
 ERR_INVALID = 1

@@ -95,13 +95,15 @@ int sipg_op_tables(const sipg_op *op, double *nodes, double *weights, double *D,
 
 /* Contraction kernel of the op's applies.  No counterpart in the
  * reference: SIPG_KERNEL_VALU runs the element contractions on the vector
- * ALUs, SIPG_KERNEL_MFMA on the matrix cores where an MFMA kernel exists
- * for the op's dtype and degree (FP32, p = 15) and on the vector ALUs
- * elsewhere, SIPG_KERNEL_AUTO (the default) picks the faster one.  A mixed-
- * precision solve passes the setting of the FP64 op to its FP32 inner
- * cycles.  sipg_op_kernel reports the kernel the op's own applies use
- * (SIPG_KERNEL_VALU or SIPG_KERNEL_MFMA); FP32 twin: pass its dtype. */
-enum { SIPG_KERNEL_AUTO = 0, SIPG_KERNEL_VALU = 1, SIPG_KERNEL_MFMA = 2 };
+ * ALUs with an element's lines spread over a workgroup, SIPG_KERNEL_WAVE
+ * on the vector ALUs with whole elements per wave (p = 8), SIPG_KERNEL_MFMA
+ * on the matrix cores (FP32, p = 15); a kernel that does not exist for the
+ * op's dtype and degree falls back to SIPG_KERNEL_VALU.  SIPG_KERNEL_AUTO
+ * (the default) picks the fastest.  A mixed-precision solve passes the
+ * setting of the FP64 op to its FP32 inner cycles.  sipg_op_kernel reports
+ * the kernel the op's applies in `dtype` use (the op's own dtype, or FP32
+ * for the inner cycles of an FP64 op). */
+enum { SIPG_KERNEL_AUTO = 0, SIPG_KERNEL_VALU = 1, SIPG_KERNEL_MFMA = 2, SIPG_KERNEL_WAVE = 3 };
 int sipg_op_set_kernel(sipg_op *op, int kernel);
 int sipg_op_kernel(const sipg_op *op, int dtype, int *kernel);
 

@@ -122,3 +122,48 @@ def test_layout_converters_round_trip(sipg, oracle, dtype):
     tol = TOL64 if dt == sipg.F64 else 2e-6 * (p + 1)
     assert np.max(np.abs(got - ref)) <= tol * np.max(np.abs(ref))
     assert op.to_device_layout(ref_buf, ref_buf) == sipg.ERR_ALIAS
+
+
+# The vector-ALU kernel with whole elements per wave (p = 8): same
+# per-line arithmetic as the workgroup-block kernel, so the two agree bit
+# for bit; both against the oracle on meshes that cut blocks at every edge.
+@pytest.mark.parametrize("p", [2, 4, 6, 8])
+@pytest.mark.parametrize("dtype", ["F64", "F32"])
+@pytest.mark.parametrize("dx,dy", [(1, 1), (5, 3), (7, 8), (8, 9), (15, 17), (33, 2), (50, 14), (128, 128)])
+def test_wave_kernel_matches_oracle_and_block_kernel(sipg, oracle, p, dtype, dx, dy):
+    pen = 5.0 * (p + 1) ** 2
+    dt = getattr(sipg, dtype)
+    op = sipg.Operator(dx, dy, p, pen, dt)
+    x = _x(op.n, 13 * dx + dy)
+    if dt == sipg.F32:
+        x = x.astype(np.float32).astype(np.float64)
+    op.set_kernel(sipg.KERNEL_WAVE)
+    assert op.kernel() == sipg.KERNEL_WAVE
+    yw = op.apply_host(x)
+    op.set_kernel(sipg.KERNEL_VALU)
+    yb = op.apply_host(x)
+    assert np.array_equal(yw, yb)
+    ref = oracle.apply(dx, dy, p, pen, x)
+    tol = TOL64 if dt == sipg.F64 else 2e-6 * (p + 1)
+    assert np.max(np.abs(yw - ref)) <= tol * np.max(np.abs(ref))
+
+
+def test_wave_kernel_in_cg(sipg, oracle):
+    # the fused p.Ap of the wave kernel drives CG like the block kernel's;
+    # its block sums cover other element sets, so p.Ap is summed in another
+    # order and the iterates agree to rounding, not bit for bit
+    dim, p = 24, 8
+    pen = 5.0 * (p + 1) ** 2
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    b = op.to_device(oracle.rhs(dim, p, pen, oracle.problem(0, sipg.forcing_draw(0, 4, 4, 4))))
+    res = {}
+    for k in (sipg.KERNEL_WAVE, sipg.KERNEL_VALU):
+        op.set_kernel(k)
+        x = op.vector()
+        x.zero()
+        st, it, rn = op.cg(x, b, 1e-10, sipg.TOL_REL)
+        assert st == sipg.OK
+        res[k] = (it, op.download(x))
+    assert abs(res[sipg.KERNEL_WAVE][0] - res[sipg.KERNEL_VALU][0]) <= 2
+    xw, xb = res[sipg.KERNEL_WAVE][1], res[sipg.KERNEL_VALU][1]
+    assert np.linalg.norm(xw - xb) <= 1e-9 * np.linalg.norm(xb)
