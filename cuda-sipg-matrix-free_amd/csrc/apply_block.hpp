@@ -212,7 +212,8 @@ __device__ __forceinline__ void dma_run(const unsigned *src, lds_dword *dst, int
 // runs and tiles outside the mesh are skipped (their slots are never read
 // by a valid element), and a run cut by the right mesh edge loads only the
 // elements that exist.
-template <typename T, int P, int BX, int BY>
+// NW: the number of (full) waves of the workgroup that issue.
+template <typename T, int P, int BX, int BY, int NW = BlockGeo<T, P, BX, BY>::NT / 64>
 __device__ __forceinline__ void block_issue(T *lds, const T *__restrict__ x, int dimx, int dimy, int x0, int y0)
 {
     using G = BlockGeo<T, P, BX, BY>;
@@ -220,7 +221,6 @@ __device__ __forceinline__ void block_issue(T *lds, const T *__restrict__ x, int
     constexpr int WD = sizeof(T) / 4;                   // dwords per value
     constexpr int NQD = G::NQ * WD, RUND = G::RUN * WD;
     constexpr int CHR = (RUND + 63) / 64;
-    constexpr int NW = G::NT / 64;
     constexpr int NITEM = BY + 2 + 2 * BY;
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -293,70 +293,23 @@ __device__ __forceinline__ void block_stage(T *lds, const BlockRegs<T, P, BX, BY
 // element's own face derivatives and the face terms all act on the pair
 // sums ue = u[i] + u[p-i] and differences uo = u[i] - u[p-i].
 //
-// The coefficients come from a policy class C: CxCoef has them as
-// compile-time constants (immediates built in scalar registers at each
-// use), TabCoef reads them from a __constant__ table (scalar loads, several
-// coefficients per load; the lane-dependent w[j] by one vector load).
+// The coefficients come from a policy object c; CxCoef has them as
+// compile-time constants (immediates built in scalar registers at each use).
 template <typename T, int P>
 struct CxCoef {
     static constexpr cx::Tables<P> CT = cx::make_tables<P>();
     static constexpr cx::EvenOdd<P> EO = cx::make_even_odd<P>();
-    __device__ static T E(int a, int i) { return (T)EO.E[a][i]; }
-    __device__ static T O(int a, int i) { return (T)EO.O[a][i]; }
-    __device__ static T DA(int i) { return (T)EO.DA[i]; }
-    __device__ static T DB(int i) { return (T)EO.DB[i]; }
-    __device__ static T DM() { return (T)EO.DM; }
-    __device__ static T D0(int i) { return (T)CT.D[i][0]; }
-    __device__ static T DP(int i) { return (T)CT.D[i][P]; }
-    __device__ static T w(int i) { return (T)CT.w[i]; }
-};
-
-template <typename T, int P>
-struct CoefTable {
-    static constexpr int N1 = P + 1, H = N1 / 2, ME = (N1 + 1) / 2;
-    T E[ME][ME];
-    T O[H > 0 ? H : 1][H > 0 ? H : 1];
-    T DA[H > 0 ? H : 1], DB[H > 0 ? H : 1];
-    T DM;
-    T D0[N1], DP[N1], w[N1];
-};
-
-template <typename T, int P>
-constexpr CoefTable<T, P> make_coef_table() {
-    const cx::Tables<P> t = cx::make_tables<P>();
-    const cx::EvenOdd<P> eo = cx::make_even_odd<P>();
-    CoefTable<T, P> c{};
-    constexpr int N1 = P + 1, H = N1 / 2, ME = (N1 + 1) / 2;
-    for (int a = 0; a < ME; ++a)
-        for (int i = 0; i < ME; ++i) c.E[a][i] = (T)eo.E[a][i];
-    for (int a = 0; a < H; ++a)
-        for (int i = 0; i < H; ++i) c.O[a][i] = (T)eo.O[a][i];
-    for (int i = 0; i < H; ++i) {
-        c.DA[i] = (T)eo.DA[i];
-        c.DB[i] = (T)eo.DB[i];
-    }
-    c.DM = (T)eo.DM;
-    for (int i = 0; i < N1; ++i) {
-        c.D0[i] = (T)t.D[i][0];
-        c.DP[i] = (T)t.D[i][P];
-        c.w[i] = (T)t.w[i];
-    }
-    return c;
-}
-
-template <typename T, int P>
-__constant__ CoefTable<T, P> kCoefTable = make_coef_table<T, P>();
-
-template <typename T, int P>
-struct TabCoef {
-    __device__ static T E(int a, int i) { return kCoefTable<T, P>.E[a][i]; }
-    __device__ static T O(int a, int i) { return kCoefTable<T, P>.O[a][i]; }
-    __device__ static T DA(int i) { return kCoefTable<T, P>.DA[i]; }
-    __device__ static T DB(int i) { return kCoefTable<T, P>.DB[i]; }
-    __device__ static T DM() { return kCoefTable<T, P>.DM; }
-    __device__ static T D0(int i) { return kCoefTable<T, P>.D0[i]; }
-    __device__ static T DP(int i) { return kCoefTable<T, P>.DP[i]; }
-    __device__ static T w(int i) { return kCoefTable<T, P>.w[i]; }
+    // E and O are symmetric (K1 symmetric and centrosymmetric); one copy of
+    // each pair is used, so the two index orders give the same value
+    __device__ T E(int a, int i) const { return (T)(a <= i ? EO.E[a][i] : EO.E[i][a]); }
+    __device__ T O(int a, int i) const { return (T)(a <= i ? EO.O[a][i] : EO.O[i][a]); }
+    __device__ T DA(int i) const { return (T)EO.DA[i]; }
+    __device__ T DB(int i) const { return (T)EO.DB[i]; }
+    __device__ T DM() const { return (T)EO.DM; }
+    __device__ T D0(int i) const { return (T)CT.D[i][0]; }
+    // DP[i] = -D0[p-i] exactly (the derivative table is antisymmetric)
+    __device__ T DP(int i) const { return -(T)CT.D[P - i][0]; }
+    __device__ T w(int i) const { return (T)CT.w[i]; }
 };
 
 template <typename T, int P>
@@ -374,7 +327,8 @@ struct LineNbr {
 };
 
 template <typename T, int P, class C = CxCoef<T, P>>
-__device__ __forceinline__ void line_load(const T *tile, int j, LineState<T, P> &S, T (&v)[P + 1])
+__device__ __forceinline__ void line_load(const T *tile, int j, LineState<T, P> &S, T (&v)[P + 1],
+                                          const C &c = C())
 {
     constexpr int N1 = P + 1;
     constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
@@ -401,14 +355,14 @@ __device__ __forceinline__ void line_load(const T *tile, int j, LineState<T, P> 
     T su = 0, bu = 0, sv = 0, bv = 0;
 #pragma unroll
     for (int i = 0; i < H; ++i) {
-        su += C::DA(i) * S.ue[i];
-        bu += C::DB(i) * S.uo[i];
-        sv += C::DA(i) * S.ve[i];
-        bv += C::DB(i) * S.vo[i];
+        su += c.DA(i) * S.ue[i];
+        bu += c.DB(i) * S.uo[i];
+        sv += c.DA(i) * S.ve[i];
+        bv += c.DB(i) * S.vo[i];
     }
     if (ME > H) {
-        su += C::DM() * S.ue[ME - 1];
-        sv += C::DM() * S.ve[ME - 1];
+        su += c.DM() * S.ue[ME - 1];
+        sv += c.DM() * S.ve[ME - 1];
     }
     S.dL = su + bu;
     S.dR = bu - su;
@@ -418,14 +372,15 @@ __device__ __forceinline__ void line_load(const T *tile, int j, LineState<T, P> 
 
 template <typename T, int P, class C = CxCoef<T, P>>
 __device__ __forceinline__ void line_neighbours(const T *tlL, const T *tlR, const T *tlD, const T *tlU, int j,
-                                                bool Lb, bool Rb, bool Db, bool Ub, LineNbr<T> &N)
+                                                bool Lb, bool Rb, bool Db, bool Ub, LineNbr<T> &N,
+                                                const C &c = C())
 {
     constexpr int N1 = P + 1;
     if (!Lb) {
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
             const T t = tlL[i * N1 + j];
-            N.nL += C::DP(i) * t;
+            N.nL += c.DP(i) * t;
             if (i == P) N.tL = t;
         }
     }
@@ -433,7 +388,7 @@ __device__ __forceinline__ void line_neighbours(const T *tlL, const T *tlR, cons
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
             const T t = tlR[i * N1 + j];
-            N.nR += C::D0(i) * t;
+            N.nR += c.D0(i) * t;
             if (i == 0) N.tR = t;
         }
     }
@@ -441,7 +396,7 @@ __device__ __forceinline__ void line_neighbours(const T *tlL, const T *tlR, cons
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
             const T t = tlD[j * N1 + k];
-            N.nD += C::DP(k) * t;
+            N.nD += c.DP(k) * t;
             if (k == P) N.tD = t;
         }
     }
@@ -449,7 +404,7 @@ __device__ __forceinline__ void line_neighbours(const T *tlL, const T *tlR, cons
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
             const T t = tlU[j * N1 + k];
-            N.nU += C::D0(k) * t;
+            N.nU += c.D0(k) * t;
             if (k == 0) N.tU = t;
         }
     }
@@ -457,10 +412,11 @@ __device__ __forceinline__ void line_neighbours(const T *tlL, const T *tlR, cons
 
 template <typename T, int P, class C = CxCoef<T, P>>
 __device__ __forceinline__ void line_finish(const LineState<T, P> &S, const LineNbr<T> &N, int j, bool Lb, bool Rb,
-                                            bool Db, bool Ub, T pen, T (&Yx)[P + 1], T (&Yy)[P + 1])
+                                            bool Db, bool Ub, T pen, T (&Yx)[P + 1], T (&Yy)[P + 1],
+                                            const C &c = C())
 {
     constexpr int H = cx::EvenOdd<P>::H, ME = cx::EvenOdd<P>::ME;
-    const T wj = C::w(j);
+    const T wj = c.w(j);
     const T pen2 = pen + pen;
     const T ev = wj * (Lb ? S.u0 : T(0.5) * (S.u0 - N.tL));
     const T fv = wj * (Rb ? -S.uP : T(0.5) * (N.tR - S.uP));
@@ -482,26 +438,26 @@ __device__ __forceinline__ void line_finish(const LineState<T, P> &S, const Line
             T ex_ = 0, ey_ = 0, ox_ = 0, oy_ = 0;
 #pragma unroll
             for (int i = 0; i < ME; ++i) {
-                const T c = C::E(a, i);
-                ex_ += c * S.ue[i];
-                ey_ += c * S.ve[i];
+                const T k = c.E(a, i);
+                ex_ += k * S.ue[i];
+                ey_ += k * S.ve[i];
             }
             if (a < H) {
 #pragma unroll
                 for (int i = 0; i < H; ++i) {
-                    const T c = C::O(a, i);
-                    ox_ += c * S.uo[i];
-                    oy_ += c * S.vo[i];
+                    const T k = c.O(a, i);
+                    ox_ += k * S.uo[i];
+                    oy_ += k * S.vo[i];
                 }
-                const T e1x = wj * ex_ + C::DA(a) * s0x, o1x = wj * ox_ + C::DB(a) * s1x;
-                const T e1y = wj * ey_ + C::DA(a) * s0y, o1y = wj * oy_ + C::DB(a) * s1y;
+                const T e1x = wj * ex_ + c.DA(a) * s0x, o1x = wj * ox_ + c.DB(a) * s1x;
+                const T e1y = wj * ey_ + c.DA(a) * s0y, o1y = wj * oy_ + c.DB(a) * s1y;
                 Yx[a] = e1x + o1x;
                 Yx[P - a] = e1x - o1x;
                 Yy[a] = e1y + o1y;
                 Yy[P - a] = e1y - o1y;
             } else {
-                Yx[a] = wj * ex_ + C::DM() * s0x;
-                Yy[a] = wj * ey_ + C::DM() * s0y;
+                Yx[a] = wj * ex_ + c.DM() * s0x;
+                Yy[a] = wj * ey_ + c.DM() * s0y;
             }
         }
     }
@@ -514,13 +470,13 @@ __device__ __forceinline__ void line_finish(const LineState<T, P> &S, const Line
 template <typename T, int P, class C = CxCoef<T, P>>
 __device__ __forceinline__ void line_operator(const T *tile, const T *tlL, const T *tlR, const T *tlD, const T *tlU,
                                               int j, bool Lb, bool Rb, bool Db, bool Ub, T pen, T (&Yx)[P + 1],
-                                              T (&Yy)[P + 1], T (&v)[P + 1])
+                                              T (&Yy)[P + 1], T (&v)[P + 1], const C &c = C())
 {
     LineState<T, P> S;
     LineNbr<T> N;
-    line_load<T, P, C>(tile, j, S, v);
-    line_neighbours<T, P, C>(tlL, tlR, tlD, tlU, j, Lb, Rb, Db, Ub, N);
-    line_finish<T, P, C>(S, N, j, Lb, Rb, Db, Ub, pen, Yx, Yy);
+    line_load<T, P, C>(tile, j, S, v, c);
+    line_neighbours<T, P, C>(tlL, tlR, tlD, tlU, j, Lb, Rb, Db, Ub, N, c);
+    line_finish<T, P, C>(S, N, j, Lb, Rb, Db, Ub, pen, Yx, Yy, c);
 }
 
 // The operator on the staged block (x0, y0), result written to y; returns

@@ -39,8 +39,9 @@ void apply_kernel_wave(const T *__restrict__ x, T *__restrict__ y, int dimx, int
     }
     const int b = xcd_remap(blockIdx.x, nblk);
     const int x0 = (b % nbx) * BX, y0 = (b / nbx) * BY;
-    block_issue<T, P, BX, BY>(lds, x, dimx, dimy, x0, y0);
+    block_issue<T, P, BX, BY, BY>(lds, x, dimx, dimy, x0, y0);      // every wave issues
 
+    const C coef{};
     const int lane = threadIdx.x & 63;
     const int ey = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s = lane / N1, j = lane - s * N1;
@@ -59,7 +60,7 @@ void apply_kernel_wave(const T *__restrict__ x, T *__restrict__ y, int dimx, int
     block_issue_wait();
     __syncthreads();                                    // the staged block is complete
     T v[N1], Yx[N1], Yy[N1];
-    line_operator<T, P, C>(tile, tlL, tlR, tlD, tlU, j, Lb, Rb, Db, Ub, pen, Yx, Yy, v);
+    line_operator<T, P, C>(tile, tlL, tlR, tlD, tlU, j, Lb, Rb, Db, Ub, pen, Yx, Yy, v, coef);
     __syncthreads();                                    // every wave is done reading the block
 
     // wave-local from here: the element's tile takes the column results,
