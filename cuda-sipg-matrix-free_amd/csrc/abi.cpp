@@ -350,11 +350,21 @@ int sipg_err_norms_fourier(sipg_op *op, const void *x, int nmodes, const double 
         SIPG_HIP_TRY(hipMemcpyAsync(dn, n, nmodes * sizeof(int), hipMemcpyHostToDevice, op->stream));
     }
     SIPG_HIP_TRY(hipMemcpyAsync(P.pAp, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice, op->stream));
-    const int64_t total = op->n;
-    const int nb = (int)std::min<int64_t>(kMaxPartials, (total + kVecThreads - 1) / kVecThreads);
+    const int nb = norms_num_blocks(op);
     int rc = launch_err_norms(op->dtype, p, op->dimx, (const double *)op->d_tab64, P.pAp, nmodes, da, dm, dn, x,
                               P.rr, nb, op->stream);
     if (rc) return rc;
+    return norms_finish(op, nb, out);
+}
+
+}  // extern "C"
+
+int sipg_internal::norms_num_blocks(sipg_op *op) {
+    return (int)std::min<int64_t>(kMaxPartials, (op->n + kVecThreads - 1) / kVecThreads);
+}
+
+int sipg_internal::norms_finish(sipg_op *op, int nb, double *out) {
+    const parts P = part_layout(op);
     std::vector<double> h((size_t)4 * nb);
     SIPG_HIP_TRY(hipMemcpyAsync(h.data(), P.rr, h.size() * sizeof(double), hipMemcpyDeviceToHost, op->stream));
     SIPG_HIP_TRY(hipStreamSynchronize(op->stream));
@@ -370,6 +380,8 @@ int sipg_err_norms_fourier(sipg_op *op, const void *x, int nmodes, const double 
     out[2] = std::sqrt(l2 + sx + sy);
     return SIPG_OK;
 }
+
+extern "C" {
 
 int sipg_time_apply(sipg_op *op, const void *x, void *y, int reps, double *ms_per_apply) {
     if (!op || !x || !y || !ms_per_apply || reps < 1) return SIPG_ERR_INVALID;
@@ -442,24 +454,17 @@ int sipg_write_result(sipg_op *op, const void *x, const char *path) {
     for (int i = 0; i < n1; ++i) nodes[i] = (z[i] * .5 + .5) / dim;
     FILE *f = (path && path[0]) ? std::fopen(path, "w") : stderr;
     if (!f) return SIPG_ERR_INVALID;
-    // element corners as the mesh builder places them: x summed along each
-    // row from 0, y stepped down from 1 once per row (build_square_mesh.hpp:
-    // 251-284), so the printed coordinates carry the same rounding
-    const double delta = 1. / (double)dim;
-    double yy = 1;
+    // element corners as the mesh builder places them, so the printed
+    // coordinates carry the same rounding
+    std::vector<double> xe, ye;
+    mesh_corners(dim, xe, ye);
     bool ok = true;
-    for (int row = 0; row < dim && ok; ++row) {
-        double xx = 0;
-        yy -= delta;
-        for (int col = 0; col < dim && ok; ++col) {
-            const int64_t e = (int64_t)row * dim + col;
-            for (int i = 0; i < n1; ++i)
-                for (int j = 0; j < n1; ++j)
-                    ok = ok && std::fprintf(f, "%10g\t%10g\t%10g\n", xx + nodes[i], yy + nodes[j],
-                                            u[((int64_t)i * n1 + j) * noe + e]) > 0;
-            ok = ok && std::fputc('\n', f) != EOF;
-            xx += delta;
-        }
+    for (int64_t e = 0; e < noe && ok; ++e) {
+        for (int i = 0; i < n1; ++i)
+            for (int j = 0; j < n1; ++j)
+                ok = ok && std::fprintf(f, "%10g\t%10g\t%10g\n", xe[e] + nodes[i], ye[e] + nodes[j],
+                                        u[((int64_t)i * n1 + j) * noe + e]) > 0;
+        ok = ok && std::fputc('\n', f) != EOF;
     }
     if (f != stderr) ok = (std::fclose(f) == 0) && ok;
     return ok ? SIPG_OK : SIPG_ERR_INVALID;

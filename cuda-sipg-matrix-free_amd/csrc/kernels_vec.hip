@@ -329,19 +329,44 @@ __global__ __launch_bounds__(NT) void rhs_fourier_kernel(int p, int dim, const d
     }
 }
 
-// Error norms of u against the Fourier solution u_ex = sum_k a_k sin(m_k pi x)
-// sin(n_k pi y) (sipg_sem_2d::err_norms, sipg_sem_2d.hpp:311-370), one thread
-// per GLL node: the nodal error, and the x and y derivatives of u at the node
-// from the element's line through it, u_x = 2 dim sum_i u(i,q) D_i(p) (the
-// basis is nodal, so phi_j(z_q) = [j = q]), weighted by the GLL quadrature
-// weights (w_p/2dim)(w_q/2dim).  Block partials: max |e|, sum w e^2,
-// sum w (u_x - u_ex,x)^2, sum w (u_y - u_ex,y)^2.  D[i*(p+1)+k] = phi_i'(z_k).
-template <typename T>
+// The exact solution at a node for the error norms: the Fourier solution
+// u_ex = sum_k a_k sin(m_k pi x) sin(n_k pi y) evaluated in place, or nodal
+// samples of u_ex and its two derivatives (element-major, one per node).
+struct FourierExact {
+    int nmodes;
+    const double *am;
+    const int *mm, *nn;
+    __device__ void at(int64_t, double X, double Y, double &u, double &dx, double &dy) const {
+        u = dx = dy = 0.0;
+        for (int k = 0; k < nmodes; ++k) {
+            const double m = mm[k] * M_PI, q = nn[k] * M_PI;
+            const double sxm = sin(m * X), cxm = cos(m * X), syq = sin(q * Y), cyq = cos(q * Y);
+            u += am[k] * sxm * syq;
+            dx += am[k] * m * cxm * syq;
+            dy += am[k] * q * sxm * cyq;
+        }
+    }
+};
+struct NodalExact {
+    const double *u, *ux, *uy;
+    __device__ void at(int64_t idx, double, double, double &a, double &dx, double &dy) const {
+        a = u[idx];
+        dx = ux[idx];
+        dy = uy[idx];
+    }
+};
+
+// Error norms of u against an exact solution (sipg_sem_2d::err_norms,
+// sipg_sem_2d.hpp:311-370), one thread per GLL node: the nodal error, and
+// the x and y derivatives of u at the node from the element's line through
+// it, u_x = 2 dim sum_i u(i,q) D_i(p) (the basis is nodal, so phi_j(z_q) =
+// [j = q]), weighted by the GLL quadrature weights (w_p/2dim)(w_q/2dim).
+// Block partials: max |e|, sum w e^2, sum w (u_x - u_ex,x)^2, sum w (u_y -
+// u_ex,y)^2.  D[i*(p+1)+k] = phi_i'(z_k).
+template <typename T, class Exact>
 __global__ __launch_bounds__(NT) void err_norms_kernel(int p, int dim, const double *__restrict__ tab,
-                                                       const double *__restrict__ D, int nmodes,
-                                                       const double *__restrict__ am, const int *__restrict__ mm,
-                                                       const int *__restrict__ nn, const T *__restrict__ u,
-                                                       double *__restrict__ part) {
+                                                       const double *__restrict__ D, Exact exact,
+                                                       const T *__restrict__ u, double *__restrict__ part) {
     const int n1 = p + 1, nq = n1 * n1;
     const int64_t noe = (int64_t)dim * dim, n = noe * nq;
     const double *w = tab;
@@ -361,14 +386,8 @@ __global__ __launch_bounds__(NT) void err_norms_kernel(int p, int dim, const dou
         const int col = (int)(e % dim), row = (int)(e / dim);
         const double X = col * h + (z[i1] * .5 + .5) / dim;
         const double Y = 1.0 - (row + 1) * h + (z[i2] * .5 + .5) / dim;
-        double ux = 0.0, dx = 0.0, dy = 0.0;
-        for (int k = 0; k < nmodes; ++k) {
-            const double m = mm[k] * M_PI, q = nn[k] * M_PI;
-            const double sxm = sin(m * X), cxm = cos(m * X), syq = sin(q * Y), cyq = cos(q * Y);
-            ux += am[k] * sxm * syq;
-            dx += am[k] * m * cxm * syq;
-            dy += am[k] * q * sxm * cyq;
-        }
+        double ux, dx, dy;
+        exact.at(idx, X, Y, ux, dx, dy);
         const double wgt = (w[i1] * .5 / dim) * (w[i2] * .5 / dim);
         const double err = fabs((double)ue[node] - ux);
         emax = fmax(emax, err);
@@ -572,9 +591,21 @@ int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t 
 int launch_err_norms(int dtype, int p, int dim, const double *tab_f64, const double *D_dev, int nmodes,
                      const double *a_dev, const int *m_dev, const int *n_dev, const void *u, double *part, int nb,
                      hipStream_t s) {
+    const FourierExact ex{nmodes, a_dev, m_dev, n_dev};
     DISPATCH_T(dtype,
-        hipLaunchKernelGGL(err_norms_kernel<double>, dim3(nb), dim3(NT), 0, s, p, dim, tab_f64, D_dev, nmodes, a_dev, m_dev, n_dev, (const double *)u, part),
-        hipLaunchKernelGGL(err_norms_kernel<float>, dim3(nb), dim3(NT), 0, s, p, dim, tab_f64, D_dev, nmodes, a_dev, m_dev, n_dev, (const float *)u, part));
+        hipLaunchKernelGGL((err_norms_kernel<double, FourierExact>), dim3(nb), dim3(NT), 0, s, p, dim, tab_f64, D_dev, ex, (const double *)u, part),
+        hipLaunchKernelGGL((err_norms_kernel<float, FourierExact>), dim3(nb), dim3(NT), 0, s, p, dim, tab_f64, D_dev, ex, (const float *)u, part));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_err_norms_nodal(int dtype, int p, int dim, const double *tab_f64, const double *D_dev,
+                           const double *ue, const double *uxe, const double *uye, const void *u, double *part,
+                           int nb, hipStream_t s) {
+    const NodalExact ex{ue, uxe, uye};
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL((err_norms_kernel<double, NodalExact>), dim3(nb), dim3(NT), 0, s, p, dim, tab_f64, D_dev, ex, (const double *)u, part),
+        hipLaunchKernelGGL((err_norms_kernel<float, NodalExact>), dim3(nb), dim3(NT), 0, s, p, dim, tab_f64, D_dev, ex, (const float *)u, part));
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

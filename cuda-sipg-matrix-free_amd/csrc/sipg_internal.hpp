@@ -308,6 +308,11 @@ int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t 
 int launch_err_norms(int dtype, int p, int dim, const double *tab_f64, const double *D_dev, int nmodes,
                      const double *a_dev, const int *m_dev, const int *n_dev, const void *u, double *part, int nb,
                      hipStream_t s);
+// as launch_err_norms against nodal samples of u_ex, d/dx u_ex, d/dy u_ex
+// (device arrays, element-major, one value per node)
+int launch_err_norms_nodal(int dtype, int p, int dim, const double *tab_f64, const double *D_dev,
+                           const double *ue, const double *uxe, const double *uye, const void *u, double *part,
+                           int nb, hipStream_t s);
 int launch_rhs_fourier(int dtype, int p, int dim, const void *tab_f64, int nmodes,
                        const double *a_dev, const int *m_dev, const int *n_dev, void *b,
                        hipStream_t s);
@@ -354,4 +359,13 @@ struct parts {
     int napply;
 };
 parts part_layout(sipg_op *op);
+// Error norms: the block count of the norm kernels, and the host reduction
+// of their 4 x nb partials (left in the r.r region of op->d_part) into
+// out = {max, L2, H1}
+int norms_num_blocks(sipg_op *op);
+int norms_finish(sipg_op *op, int nb, double *out);
+// Element corners of the square mesh as the mesh builder places them (x
+// summed along each row from 0, y stepped down from 1 once per row,
+// build_square_mesh.hpp:250-288), element e = row * dim + col.
+void mesh_corners(int dim, std::vector<double> &xe, std::vector<double> &ye);
 }  // namespace sipg_internal

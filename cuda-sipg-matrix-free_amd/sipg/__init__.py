@@ -65,6 +65,8 @@ _lib = None
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+# double f(double x, double y, void *ctx)
+FN2 = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
 
 
 def _declare(L):
@@ -148,6 +150,11 @@ def _declare(L):
         "sipg_server_peek": (i, [_vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "sipg_server_set_timeout": (i, [_vp, d]),
         "sipg_vec_dot_noalloc": (i, [i, i64, _vp, _vp, _dp, _vp]),
+        "sipg_quad_points": (i, [_vp, _dp, _dp]),
+        "sipg_rhs_nodal": (i, [_vp, _dp, _dp, _vp]),
+        "sipg_rhs": (i, [_vp, FN2, FN2, _vp, _vp]),
+        "sipg_err_norms_nodal": (i, [_vp, _vp, _dp, _dp, _dp, _dp]),
+        "sipg_err_norms": (i, [_vp, _vp, FN2, FN2, FN2, _vp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -386,6 +393,63 @@ class Operator:
         out = np.zeros(3)
         check(lib().sipg_err_norms_fourier(self.h, x.ptr, len(a), a.ctypes.data_as(_dp), m.ctypes.data_as(_ip),
                                            n.ctypes.data_as(_ip), out.ctypes.data_as(_dp)), "err_norms_fourier")
+        return {"max": out[0], "L2": out[1], "H1": out[2]}
+
+    def quad_points(self):
+        """Physical (x, y) of every GLL node, element-major like device
+        vectors (sipg_quad_points)."""
+        x, y = np.zeros(self.n), np.zeros(self.n)
+        check(lib().sipg_quad_points(self.h, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp)), "quad_points")
+        return x, y
+
+    def _nodal(self, v):
+        """A function of (x, y) (vectorised over numpy arrays) or an array
+        of nodal values, as a contiguous float64 array of the op's length."""
+        if callable(v):
+            x, y = self.quad_points()
+            v = v(x, y)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (self.n,)))
+        return a
+
+    def rhs(self, f, g=None, b=None):
+        """Load vector of -lap u = f with Dirichlet data g (sipg_rhs_nodal;
+        sipg_sem_2d::compute_rhs).  f, g: numpy-vectorised functions of
+        (x, y) or nodal arrays; g None: g = 0.  Returns b (a new device
+        vector unless one is given)."""
+        fa = self._nodal(f)
+        ga = None if g is None else self._nodal(g)
+        if b is None:
+            b = self.vector()
+        check(lib().sipg_rhs_nodal(self.h, fa.ctypes.data_as(_dp),
+                                   None if ga is None else ga.ctypes.data_as(_dp), b.ptr), "rhs_nodal")
+        return b
+
+    def rhs_fn(self, f, g=None, b=None):
+        """As rhs with f and g as scalar Python functions called back from
+        the library at each node (sipg_rhs)."""
+        cf = FN2(lambda x, y, _c: f(x, y))
+        cg = FN2(lambda x, y, _c: g(x, y)) if g is not None else FN2()
+        if b is None:
+            b = self.vector()
+        check(lib().sipg_rhs(self.h, cf, cg, None, b.ptr), "rhs")
+        return b
+
+    def err_norms(self, x, u, ux, uy):
+        """Max nodal, L2 and H1 errors of x against an exact solution u with
+        derivatives ux, uy (numpy-vectorised functions or nodal arrays;
+        sipg_err_norms_nodal)."""
+        a, ax, ay = self._nodal(u), self._nodal(ux), self._nodal(uy)
+        out = np.zeros(3)
+        check(lib().sipg_err_norms_nodal(self.h, x.ptr, a.ctypes.data_as(_dp), ax.ctypes.data_as(_dp),
+                                         ay.ctypes.data_as(_dp), out.ctypes.data_as(_dp)), "err_norms_nodal")
+        return {"max": out[0], "L2": out[1], "H1": out[2]}
+
+    def err_norms_fn(self, x, u, ux, uy):
+        """As err_norms with scalar Python functions called back from the
+        library (sipg_err_norms)."""
+        cu, cx, cy = (FN2(lambda X, Y, _c, fn=fn: fn(X, Y)) for fn in (u, ux, uy))
+        out = np.zeros(3)
+        check(lib().sipg_err_norms(self.h, x.ptr, cu, cx, cy, None, out.ctypes.data_as(_dp)), "err_norms")
         return {"max": out[0], "L2": out[1], "H1": out[2]}
 
     def write_result(self, x, path=None):

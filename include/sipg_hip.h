@@ -389,6 +389,43 @@ int sipg_rhs_fourier(sipg_op *op, int nmodes, const double *a, const int *m, con
  * H1_err, sipg_sem_2d.hpp:73-81).  Square meshes. */
 int sipg_err_norms_fourier(sipg_op *op, const void *x, int nmodes, const double *a, const int *m, const int *n,
                            double *out);
+
+/* ---- general problem set-up: any f, Dirichlet data g, any exact solution */
+/* A function of the point (x, y) with a caller context, evaluated on the
+ * host (the reference's FLOAT_TYPE (*)(FLOAT_TYPE, FLOAT_TYPE) arguments,
+ * sipg_sem_2d.hpp:83-92, with a context pointer added). */
+typedef double (*sipg_fn2)(double x, double y, void *ctx);
+/* Physical coordinates of every GLL node of the op's square mesh of the
+ * unit square, element-major like device vectors (x[k], y[k] for k = e
+ * (p+1)^2 + i1 (p+1) + i2): element corners as build_square_mesh.hpp:
+ * 250-288 places them (element e = row * dim + col, row 0 at the top), plus
+ * (z_i + 1) / (2 dim) per direction (sipg_sem_2d.hpp:250-255).  x, y: host
+ * arrays of the op's length. */
+int sipg_quad_points(sipg_op *op, double *x, double *y);
+/* b = load vector of -lap u = f with Dirichlet data g, as
+ * sipg_sem_2d::compute_rhs (sipg_sem_2d.hpp:232-305): (w_i1 h/2)(w_i2 h/2)
+ * f at every node, plus on the four sides of the square pen w_i g and the
+ * consistency terms -/+ w_i phi_j'(side) g.  f, g: host arrays of the op's
+ * length holding f and g at the points of sipg_quad_points (g is read on
+ * the boundary nodes of boundary elements only; g NULL: g = 0).  b: device
+ * vector of the op (its dtype).  Assembled on the host in double in the
+ * reference's order of terms.  Square meshes. */
+int sipg_rhs_nodal(sipg_op *op, const double *f, const double *g, void *b);
+/* As sipg_rhs_nodal with f and g as functions evaluated at the node points
+ * (compute_rhs(f, u_ex), sipg_sem_2d.hpp:232-235).  g NULL: g = 0. */
+int sipg_rhs(sipg_op *op, sipg_fn2 f, sipg_fn2 g, void *ctx, void *b);
+/* Error norms of the device vector x against an exact solution given by
+ * nodal samples u, ux = du/dx, uy = du/dy at the points of sipg_quad_points
+ * (host arrays of the op's length): out = {max nodal error, L2 error, H1
+ * error} as sipg_err_norms_fourier (sipg_sem_2d.hpp:311-370), reduced on
+ * the device.  Square meshes. */
+int sipg_err_norms_nodal(sipg_op *op, const void *x, const double *u, const double *ux, const double *uy,
+                         double *out);
+/* As sipg_err_norms_nodal with the exact solution and its derivatives as
+ * functions (err_norms(u, f, u_ex, dx_u_ex, dy_u_ex), sipg_sem_2d.hpp:
+ * 311-316). */
+int sipg_err_norms(sipg_op *op, const void *x, sipg_fn2 u, sipg_fn2 ux, sipg_fn2 uy, void *ctx, double *out);
+
 /* The solution as text, as sipg_sem_2d::print_result (sipg_sem_2d.hpp:
  * 375-403): per element (row-major from the top-left), per node (x index
  * outer, y inner) one line "x<TAB>y<TAB>u", each field printf "%10g" (the

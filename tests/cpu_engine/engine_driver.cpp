@@ -7,9 +7,9 @@
 #include <cstdio>
 #include <vector>
 
-extern "C" int orc_cg(int, int, int, double, const double *, double *, double, int, int, double *);
-extern "C" void orc_apply(int, int, int, double, const double *, double *);
-extern "C" void orc_gll(int, double *, double *);
+extern "C" {
+#include "../../oracle/sipg_oracle.h"
+}
 
 namespace sipg_internal {
 void coarse_dst(int dim, std::vector<double> &Q, std::vector<double> &lamK, std::vector<double> &lamM, int p);
@@ -140,6 +140,50 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
     sipg_op_destroy(op);
 }
 
+// The general set-up's host path against the oracle: sipg_rhs_nodal fed
+// with orc_f / orc_u_ex at the library's node points vs orc_rhs, and
+// sipg_err_norms_nodal vs orc_err_norms on a perturbed interpolant.
+static void problem_check(int kind, int dim, int p) {
+    const double pen = 100.0 * p * p;
+    sipg_op *op = nullptr;
+    if (sipg_op_create(&op, dim, dim, p, pen, SIPG_F64, nullptr)) return;
+    int64_t n;
+    sipg_op_size(op, &n);
+    orc_problem pb{kind, 0, nullptr, nullptr, nullptr};
+    std::vector<double> X(n), Y(n), f(n), g(n), u(n), ux(n), uy(n);
+    int rc = sipg_quad_points(op, X.data(), Y.data());
+    for (int64_t k = 0; k < n; ++k) {
+        f[k] = orc_f(&pb, X[k], Y[k]);
+        g[k] = u[k] = orc_u_ex(&pb, X[k], Y[k]);
+        ux[k] = orc_dx_u_ex(&pb, X[k], Y[k]);
+        uy[k] = orc_dy_u_ex(&pb, X[k], Y[k]);
+    }
+    void *b, *x;
+    sipg_malloc(&b, n * 8);
+    sipg_malloc(&x, n * 8);
+    if (!rc) rc = sipg_rhs_nodal(op, f.data(), g.data(), b);
+    std::vector<double> bh(n), bo(n);
+    sipg_download(op, b, bh.data());
+    orc_rhs(dim, p, pen, &pb, bo.data());
+    double db = 0, sb = 0;
+    for (int64_t k = 0; k < n; ++k) { db = std::fmax(db, std::fabs(bh[k] - bo[k])); sb = std::fmax(sb, std::fabs(bo[k])); }
+    // x = the interpolant of u_ex plus a smooth perturbation (device order)
+    std::vector<double> xd(n), xr(n);
+    for (int64_t k = 0; k < n; ++k) xd[k] = u[k] + 1e-3 * std::sin(3.0 * X[k] + 2.0 * Y[k]);
+    sipg_memcpy_h2d(x, xd.data(), n * 8);
+    double ours[3] = {0, 0, 0}, want[3];
+    if (!rc) rc = sipg_err_norms_nodal(op, x, u.data(), ux.data(), uy.data(), ours);
+    sipg_download(op, x, xr.data());
+    orc_err_norms(dim, p, xr.data(), &pb, want);
+    double dn = 0;
+    for (int k = 0; k < 3; ++k) dn = std::fmax(dn, std::fabs(ours[k] - want[k]) / want[k]);
+    printf("{\"case\":\"problem_%d_%d_p%d\",\"rc\":%d,\"rhs_rel\":%.3e,\"norms_rel\":%.3e}\n", kind, dim, p,
+           rc, db / sb, dn);
+    sipg_free(b);
+    sipg_free(x);
+    sipg_op_destroy(op);
+}
+
 int main() {
     run("plain", 6, 3, 80.0, 0, 1e-10, 0, 0.0);
     run("plain_cap", 6, 3, 80.0, 0, 1e-12, 5, 0.0);
@@ -162,5 +206,11 @@ int main() {
     coarse_check(5, 5, 3);
     coarse_check(6, 4, 2);
     coarse_check(4, 5, 1);
+    problem_check(1, 4, 2);
+    problem_check(2, 5, 3);
+    problem_check(3, 3, 4);
+    problem_check(4, 6, 2);
+    problem_check(5, 8, 3);
+    problem_check(5, 1, 5);
     return 0;
 }

@@ -309,6 +309,41 @@ int launch_err_norms(int dtype, int p, int dim, const double *, const double *, 
     return SIPG_OK;
 }
 
+// error norms against nodal samples, as one block partial: the norms'
+// definition written out per node (FP64 only)
+int launch_err_norms_nodal(int dtype, int p, int dim, const double *, const double *D, const double *ue,
+                           const double *uxe, const double *uye, const void *u, double *part, int nb, hipStream_t) {
+    if (dtype != SIPG_F64 || nb < 1) return SIPG_ERR_UNSUPPORTED;
+    const int n1 = p + 1, nq = n1 * n1;
+    const int64_t noe = (int64_t)dim * dim;
+    std::vector<double> z(n1), w(n1);
+    orc_gll(p, z.data(), w.data());
+    const double *U = tp<const double>(u);
+    double emax = -1.0, l2 = 0.0, sx = 0.0, sy = 0.0;
+    for (int64_t e = 0; e < noe; ++e)
+        for (int i1 = 0; i1 < n1; ++i1)
+            for (int i2 = 0; i2 < n1; ++i2) {
+                const int64_t k = e * nq + i1 * n1 + i2;
+                double dxu = 0.0, dyu = 0.0;
+                for (int i = 0; i < n1; ++i) {
+                    dxu += U[e * nq + i * n1 + i2] * 2.0 * dim * D[i * n1 + i1];
+                    dyu += U[e * nq + i1 * n1 + i] * 2.0 * dim * D[i * n1 + i2];
+                }
+                const double wgt = (w[i1] * .5 / dim) * (w[i2] * .5 / dim), err = std::fabs(U[k] - ue[k]);
+                emax = std::fmax(emax, err);
+                l2 += wgt * err * err;
+                sx += wgt * (dxu - uxe[k]) * (dxu - uxe[k]);
+                sy += wgt * (dyu - uye[k]) * (dyu - uye[k]);
+            }
+    for (int k = 0; k < 4; ++k)
+        for (int b = 0; b < nb; ++b) part[(size_t)k * nb + b] = k == 0 ? -1.0 : 0.0;
+    part[0] = emax;
+    part[nb] = l2;
+    part[2 * nb] = sx;
+    part[3 * nb] = sy;
+    return SIPG_OK;
+}
+
 // ---- PCG (kernels_pcg.hip) ----------------------------------------------
 // z = M^-1 r per element with the op's tables [S, Linv]:
 // Z = S [(S^T R S) .* Linv] S^T on each (p+1) x (p+1) tile.

@@ -94,6 +94,18 @@ def test_coarse_tables_reproduce_the_galerkin_coarse_operator(results, case):
     assert abs(r["center"] - (4.0 if case.endswith("p1") else 8.0 / 3.0)) < 1e-6, r      # printed to 6 places
 
 
+@pytest.mark.parametrize("case", ["problem_1_4_p2", "problem_2_5_p3", "problem_3_3_p4", "problem_4_6_p2",
+                                  "problem_5_8_p3", "problem_5_1_p5"])
+def test_general_rhs_assembly_is_the_reference_loop(results, case):
+    # sipg_rhs_nodal fed with f and g = u_ex at sipg_quad_points: the host
+    # assembly (problem.cpp) reproduces orc_rhs, which follows the
+    # reference's compute_rhs term for term, to the bit; the nodal norms'
+    # host side (partials, reduction) against orc_err_norms
+    r = results[case]
+    assert r["rc"] == 0 and r["rhs_rel"] == 0.0, r
+    assert r["norms_rel"] < 1e-13, r
+
+
 @pytest.mark.parametrize("case", ["pcg2_replace", "pcg2_true_stop", "pcg2_plain", "pcg2_one_element"])
 def test_pcg2_converges_to_the_oracle_solution(results, case):
     r = results[case]

@@ -81,3 +81,18 @@ def test_cg_absolute_vs_relative(oracle):
     assert np.sqrt(rr2) <= 1e-6 * np.linalg.norm(b)
     # ||b|| > 1 here, so the relative threshold is the looser one
     assert np.linalg.norm(b) > 1 and it2 <= it1
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3, 4, 5])
+def test_analytic_solutions_match_the_oracle(oracle, kind):
+    # the numpy forms of solutions A-E (tests/analytic.py, what the library's
+    # general set-up is fed with in the GPU tests) against the oracle's
+    from analytic import SOLUTIONS
+    prob = oracle.problem(kind)
+    L = oracle.lib()
+    rng = np.random.default_rng(kind)
+    pts = rng.uniform(0, 1, (64, 2))
+    for fn, name in zip(SOLUTIONS[kind], ("orc_u_ex", "orc_f", "orc_dx_u_ex", "orc_dy_u_ex")):
+        ours = fn(pts[:, 0], pts[:, 1])
+        want = np.array([getattr(L, name)(prob.ref(), x, y) for x, y in pts])
+        assert np.allclose(ours, want, rtol=1e-13, atol=1e-13 * np.abs(want).max()), name
