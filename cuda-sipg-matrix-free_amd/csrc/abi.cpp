@@ -64,7 +64,8 @@ const char *sipg_strerror(int s) {
     }
 }
 
-int sipg_max_degree(void) { return kMaxDegree; }
+int sipg_max_degree(void) { return kMaxGenericDegree; }
+int sipg_max_fused_degree(void) { return kMaxDegree; }
 
 int sipg_basis_tables(int p, double *nodes, double *weights, double *D, double *K1) {
     if (p < 1 || p > 63) return SIPG_ERR_INVALID;
@@ -131,12 +132,12 @@ int sipg_op_create(sipg_op **out, int dimx, int dimy, int p, double pen, int dty
     *out = nullptr;
     if (dimx < 1 || dimy < 1 || p < 1 || !std::isfinite(pen)) return SIPG_ERR_INVALID;
     if (dtype != SIPG_F64 && dtype != SIPG_F32) return SIPG_ERR_INVALID;
-    if (p > kMaxDegree) return SIPG_ERR_UNSUPPORTED;
+    if (p > kMaxGenericDegree) return SIPG_ERR_UNSUPPORTED;
     const int64_t noe = (int64_t)dimx * dimy;
     if (noe > (int64_t)INT_MAX) return SIPG_ERR_INVALID;
-    // the operator kernel addresses a block's footprint (up to 6 element
-    // rows) with 32-bit offsets
-    if ((int64_t)6 * dimx * (p + 1) * (p + 1) > (int64_t)INT_MAX) return SIPG_ERR_UNSUPPORTED;
+    // the fused operator kernels address a block's footprint (up to 6
+    // element rows) with 32-bit offsets
+    if (p <= kMaxDegree && (int64_t)6 * dimx * (p + 1) * (p + 1) > (int64_t)INT_MAX) return SIPG_ERR_UNSUPPORTED;
     sipg_op *op = new sipg_op();
     op->dimx = dimx;
     op->dimy = dimy;
@@ -147,16 +148,21 @@ int sipg_op_create(sipg_op **out, int dimx, int dimy, int p, double pen, int dty
     op->stream = (hipStream_t)stream;
     build_tables(p, op->h_tab);
     const int tl = table_len(p);
+    // above the compiled degrees the operator's face buffer follows the
+    // tables in the op's dtype allocation
+    const bool generic = p > kMaxDegree;
+    const size_t tl64 = generic && dtype == SIPG_F64 ? generic_table_len(SIPG_F64, p, dimx, dimy) : (size_t)tl;
+    const size_t tl32 = generic ? generic_table_len(SIPG_F32, p, dimx, dimy) : (size_t)tl;
     int rc = SIPG_OK;
     auto fail = [&](int code) { sipg_op_destroy(op); return code; };
     hipError_t e;
-    if ((e = hipMalloc(&op->d_tab64, tl * sizeof(double))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
+    if ((e = hipMalloc(&op->d_tab64, tl64 * sizeof(double))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
     if ((e = hipMemcpy(op->d_tab64, op->h_tab.data(), tl * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_HIP); }
     if (dtype == SIPG_F64) {
         op->d_tab = op->d_tab64;
     } else {
         std::vector<float> tf(op->h_tab.begin(), op->h_tab.end());
-        if ((e = hipMalloc(&op->d_tab, tl * sizeof(float))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
+        if ((e = hipMalloc(&op->d_tab, tl32 * sizeof(float))) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_NOMEM); }
         if ((e = hipMemcpy(op->d_tab, tf.data(), tl * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) { set_last_hip_error(e); return fail(SIPG_ERR_HIP); }
     }
     // room for the block sums of any contraction kernel (sipg_op_set_kernel)

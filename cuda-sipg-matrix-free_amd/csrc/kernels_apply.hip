@@ -230,6 +230,7 @@ int resolve_kernel(int dtype, int p, int kernel) {
 }
 
 int apply_num_blocks(int dtype, int p, int dimx, int dimy, int kernel) {
+    if (p > kMaxDegree) return p <= kMaxGenericDegree ? generic_apply_num_blocks(dimx, dimy) : -1;
     const int k = resolve_kernel(dtype, p, kernel);
     if (k == SIPG_KERNEL_MFMA) return mfma_apply_num_blocks(dimx, dimy);
     if (k == SIPG_KERNEL_WAVE) return wave_apply_num_blocks(dtype, p, dimx, dimy);
@@ -240,6 +241,10 @@ int apply_num_blocks(int dtype, int p, int dimx, int dimy, int kernel) {
 int launch_apply(int dtype, int p, const void *x, void *y, const void *tab, int dimx, int dimy,
                  double pen, double *partials, const cg_state *st, int slot, hipStream_t s, int kernel)
 {
+    // above the compiled degrees: the run-time-table operator, whatever the
+    // setting (its only implementation)
+    if (p > kMaxDegree)
+        return launch_apply_generic(dtype, p, x, y, tab, dimx, dimy, pen, partials, st, slot, s);
     const int k = resolve_kernel(dtype, p, kernel);
     if (k == SIPG_KERNEL_MFMA)
         return launch_apply_mfma(dtype, p, x, y, dimx, dimy, pen, partials, st, slot, s);

@@ -70,7 +70,8 @@ void jacobi_eigen(int n, std::vector<double> &A, std::vector<double> &V, std::ve
 }  // namespace
 
 int build_block_jacobi(int p, double pen, std::vector<double> &tab) {
-    if (p < 1 || p > kMaxDegree || !(pen > 0.0)) return SIPG_ERR_INVALID;
+    if (p < 1 || !(pen > 0.0)) return SIPG_ERR_INVALID;
+    if (p > kMaxDegree) return SIPG_ERR_UNSUPPORTED;     // no compiled preconditioner kernel
     const int n = p + 1, N = p;
     std::vector<double> z(n), w(n), D((size_t)n * n), K1((size_t)n * n);
     gll(p, z.data(), w.data());

@@ -22,6 +22,9 @@ namespace sipg_internal {
 
 // Highest degree with a compiled fused kernel.
 constexpr int kMaxDegree = 16;
+// Highest degree of the run-time-table operator above it (64 LGL points,
+// the reference's table size).
+constexpr int kMaxGenericDegree = 63;
 
 void set_last_hip_error(hipError_t e);
 
@@ -166,6 +169,16 @@ bool mfma_apply_available(int dtype, int p);
 int mfma_apply_num_blocks(int dimx, int dimy);
 int launch_apply_mfma(int dtype, int p, const void *x, void *y, int dimx, int dimy, double pen, double *partials,
                       const cg_state *st, int slot, hipStream_t s);
+// the run-time-table apply for kMaxDegree < p <= kMaxGenericDegree
+// (kernels_generic.hip).  Its face buffer (4 (p+1) values per element)
+// lives in the op's table allocation after the tables: such ops allocate
+// generic_table_len elements of their dtype for d_tab, of which the first
+// generic_face_offset hold the tables.
+int generic_apply_num_blocks(int dimx, int dimy);
+size_t generic_face_offset(int p, size_t elem);
+size_t generic_table_len(int dtype, int p, int dimx, int dimy);
+int launch_apply_generic(int dtype, int p, const void *x, void *y, const void *tab, int dimx, int dimy, double pen,
+                         double *partials, const cg_state *st, int slot, hipStream_t s);
 
 // ---- launchers (kernels_vec.hip) ------------------------------------------
 int vec_num_blocks(int64_t n);

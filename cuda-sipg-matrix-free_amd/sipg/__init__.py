@@ -77,6 +77,7 @@ def _declare(L):
     sig = {
         "sipg_strerror": (ctypes.c_char_p, [i]),
         "sipg_max_degree": (i, []),
+        "sipg_max_fused_degree": (i, []),
         "sipg_basis_tables": (i, [i, _dp, _dp, _dp, _dp]),
         "sipg_device_count": (i, [_ip]),
         "sipg_set_device": (i, [i]),
@@ -185,7 +186,13 @@ def check(status, what=""):
 
 
 def max_degree():
+    """Highest degree an operator accepts (63)."""
     return lib().sipg_max_degree()
+
+
+def max_fused_degree():
+    """Highest degree of the compiled fused operator kernels (16)."""
+    return lib().sipg_max_fused_degree()
 
 
 def device_count():

@@ -55,8 +55,14 @@ typedef struct sipg_op sipg_op;   /* operator: mesh, degree, penalty, tables */
 
 /* ---- library ----------------------------------------------------------- */
 const char *sipg_strerror(int status);
-/* Highest polynomial degree the fused kernels are built for. */
+/* Highest polynomial degree an operator accepts: 63, the reference's
+ * 64-point LGL tables (device_const.hpp:10-18). */
 int sipg_max_degree(void);
+/* Highest degree of the compiled fused operator kernels (16).  Above it the
+ * operator runs on run-time tables (two launches per apply), CG and the
+ * mixed CG work as below it, and the preconditioned solvers, batches and
+ * the server return SIPG_ERR_UNSUPPORTED. */
+int sipg_max_fused_degree(void);
 /* Device count / selection helpers (thin HIP wrappers, no torch). */
 int sipg_device_count(int *count);
 int sipg_set_device(int device);

@@ -24,6 +24,9 @@ static const int kMockApplyBlocks = 3;
 static const int kMockVecBlocks = 2;
 
 int apply_num_blocks(int, int p, int, int, int) { return p <= kMaxDegree ? kMockApplyBlocks : -1; }
+// the CPU stand-in builds no run-time-table operator (op creation above the
+// compiled degrees then fails on the block count above)
+size_t generic_table_len(int, int p, int, int) { return (size_t)table_len(p); }
 // the CPU stand-in has one contraction path; the setting is only recorded
 int resolve_kernel(int, int, int) { return SIPG_KERNEL_VALU; }
 int vec_num_blocks(int64_t) { return kMockVecBlocks; }

@@ -3,7 +3,8 @@
     python tests/golden/make_golden.py
 
 * basis_ref.json   -- LGL nodes/weights, derivative table D and 1-D stiffness
-                      K1 for p = 1..16, produced by the REFERENCE's own host
+                      K1 for p = 1..16, 20, 24, 32, 40, 48 and 63 (the
+                      reference's largest table), produced by the REFERENCE's own host
                       code (LGL_quadrature_table.hpp, sem_function.hpp,
                       laplacian_operator.hpp) compiled from its sources into
                       oracle/_ref by `make -C oracle ref`; stored as hex floats.
@@ -34,12 +35,14 @@ import oracle as O   # noqa: E402
 import dense         # noqa: E402
 from mt19937_64 import forcing_draw   # noqa: E402
 
+BASIS_DEGREES = list(range(1, 17)) + [20, 24, 32, 40, 48, 63]
+
 
 def main():
     # basis from the reference's own host code
     if O.ref_basis_available():
         out = {"source": "reference host code via oracle/_ref/libref_basis.so", "p": {}}
-        for p in range(1, 17):
+        for p in BASIS_DEGREES:
             z, w, D, K = O.ref_basis(p)
             out["p"][str(p)] = {k: [float(v).hex() for v in arr.ravel()]
                                 for k, arr in (("nodes", z), ("weights", w), ("D", D), ("K1", K))}

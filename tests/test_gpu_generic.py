@@ -1,0 +1,137 @@
+"""Degrees above the compiled fused kernels (17..63, the reference's 64-point
+LGL tables, device_const.hpp:10-18): the run-time-table operator
+(kernels_generic.hip) against the oracle, and the solvers and set-up that
+run on it.  FP64 bar as for the fused kernels: max|y - y_oracle| <= 1e-12
+max|y_oracle|."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from analytic import SOLUTIONS
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _x(n, seed):
+    return np.random.default_rng(seed).uniform(-1, 1, n)
+
+
+def test_degree_limits(sipg):
+    assert sipg.max_fused_degree() == 16 and sipg.max_degree() == 63
+
+
+@pytest.mark.parametrize("p", [17, 20, 24, 31, 32, 40, 48, 63])
+@pytest.mark.parametrize("dx,dy", [(1, 1), (3, 2), (5, 4)])
+def test_generic_apply_f64_matches_oracle(sipg, oracle, p, dx, dy):
+    pen = 5.0 * (p + 1) ** 2
+    op = sipg.Operator(dx, dy, p, pen, sipg.F64)
+    x = _x(op.n, 31 * p + dx)
+    y = op.apply_host(x)
+    ref = oracle.apply(dx, dy, p, pen, x)
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("p", [17, 32, 63])
+def test_generic_apply_f32_close_to_oracle(sipg, oracle, p):
+    dx, dy = 4, 3
+    pen = 5.0 * (p + 1) ** 2
+    op = sipg.Operator(dx, dy, p, pen, sipg.F32)
+    x = _x(op.n, p).astype(np.float32).astype(np.float64)
+    y = op.apply_host(x)
+    ref = oracle.apply(dx, dy, p, pen, x)
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    assert err <= 2e-6 * (p + 1), err
+
+
+def test_generic_tables_match_the_reference_golden(sipg):
+    with open(os.path.join(GOLD, "basis_ref.json")) as f:
+        ref = json.load(f)["p"]
+    for p in (20, 24, 32, 40, 48, 63):
+        op = sipg.Operator(1, 1, p, 1.0, sipg.F64)
+        z, w, D, K = op.tables()
+        for name, mine in (("nodes", z), ("weights", w), ("D", D), ("K1", K)):
+            theirs = np.array([float.fromhex(s) for s in ref[str(p)][name]])
+            scale = max(1.0, np.max(np.abs(theirs)))
+            assert np.max(np.abs(mine.ravel() - theirs)) <= 4e-15 * scale * (p + 1), (p, name)
+
+
+def test_generic_apply_repeats_bitwise(sipg):
+    # the face buffer is rewritten by each apply: two applies of different
+    # vectors, then the first again, give the first result to the bit
+    op = sipg.Operator(6, 5, 21, 5.0 * 22 ** 2, sipg.F64)
+    x1, x2 = _x(op.n, 1), _x(op.n, 2)
+    y1 = op.apply_host(x1)
+    op.apply_host(x2)
+    assert np.array_equal(op.apply_host(x1), y1)
+
+
+@pytest.mark.parametrize("p", [18, 24])
+def test_generic_cg_tracks_the_oracle(sipg, oracle, p):
+    dim = 4
+    pen = sipg.pen_sigma(p)
+    modes = sipg.forcing_draw(5, 4, 4, 4)
+    bh = oracle.rhs(dim, p, pen, oracle.problem(0, modes))
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    b = op.to_device(bh)
+    x = op.vector()
+    x.zero()
+    st, it, res = op.cg(x, b, 1e-10, sipg.TOL_REL)
+    assert st == sipg.OK, sipg.strerror(st)
+    xo, ito, _ = oracle.cg(dim, dim, p, pen, bh, tol=1e-10, tol_mode=1)
+    assert abs(it - ito) <= max(2, 0.02 * ito), (it, ito)
+    assert np.linalg.norm(op.download(x) - xo) <= 1e-8 * np.linalg.norm(xo)
+    assert op.true_relres(x, b) <= 1.5e-10
+
+
+def test_generic_mixed_cg_reaches_fp64_accuracy(sipg, oracle):
+    dim, p = 4, 20
+    pen = sipg.pen_sigma(p)
+    modes = sipg.forcing_draw(6, 4, 4, 4)
+    bh = oracle.rhs(dim, p, pen, oracle.problem(0, modes))
+    op = sipg.Operator(dim, dim, p, pen, sipg.F64)
+    b = op.to_device(bh)
+    x = op.vector()
+    x.zero()
+    st, info = op.cg_mixed(x, b, 1e-10)
+    assert st == sipg.OK, sipg.strerror(st)
+    assert op.true_relres(x, b) <= 1e-10
+
+
+def test_generic_p_convergence_through_the_library(sipg, oracle):
+    # p-refinement on solution E at dim = 2 (the reference's
+    # sipg_2d_p_adaptivity driver): RHS, solve and norms by the library;
+    # the H1 error keeps falling spectrally past p = 16 and matches the
+    # oracle's solve at every degree
+    u, f, ux, uy = SOLUTIONS[5]
+    errs = {}
+    for p in (12, 16, 17, 20, 24):
+        pen = 100.0 * p * p
+        op = sipg.Operator(2, 2, p, pen, sipg.F64)
+        b = op.rhs(f, u)
+        x = op.vector()
+        x.zero()
+        st, it, res = op.cg(x, b, 1e-13, sipg.TOL_REL)
+        assert st == sipg.OK, (p, sipg.strerror(st))
+        errs[p] = op.err_norms(x, u, ux, uy)
+        _, _, want = oracle.solve(2, p, pen, oracle.problem(5), tol=1e-13, tol_mode=1)
+        # below ~1e-6 the two solves' stopping points (relres 1e-13 on a
+        # system of condition ~1e9) set the last digits
+        tol = 1e-4 if want["H1"] > 1e-6 else 2e-2
+        assert abs(errs[p]["H1"] - want["H1"]) <= tol * want["H1"], (p, errs[p], want)
+    assert errs[24]["H1"] < errs[20]["H1"] < errs[17]["H1"] < errs[16]["H1"] < errs[12]["H1"]
+    assert errs[24]["H1"] < 1e-8
+
+
+def test_generic_degree_preconditioners_are_unsupported(sipg):
+    op = sipg.Operator(2, 2, 17, sipg.pen_sigma(17), sipg.F64)
+    b = op.to_device(np.ones(op.n))
+    x = op.vector()
+    x.zero()
+    st, info = op.pcg(x, b, 1e-10)
+    assert st == sipg.ERR_UNSUPPORTED
+    st, info = op.pcg2(x, b, 1e-10)
+    assert st == sipg.ERR_UNSUPPORTED
