@@ -283,13 +283,8 @@ def profile_apply_c3(sipg, reps):
     log(f"profile-apply-c3: {reps} MFMA applies, {reps} VALU applies and {reps} copies of {op.n} floats")
 
 
-def batched_leg(sipg, log_every=True):
-    """2048 independent FP64 solves in one batch: mesh side ~ U{4..64},
-    degree ~ U{2..10} (numpy default_rng(4)), instance i forced by
-    forcing_draw(i, 1..4 modes, m,n <= 4), pen = 5 (p+1)^2, 1e-10 true
-    relative residual each, by batched CG and by batched block-Jacobi PCG.
-    Each is compared with solving a sample of the same instances (every
-    64th) one after another with the single-problem solver of that method."""
+def c4_batch(sipg):
+    """The 2048-instance batch of the batched workload, forced and tuned."""
     N = 2048
     rng = np.random.default_rng(4)
     dims = rng.integers(4, 65, N)
@@ -300,6 +295,35 @@ def batched_leg(sipg, log_every=True):
     for i in range(N):
         B.rhs_fourier(i, sipg.forcing_draw(i, 1, 4, 4), b)
     B.tune()
+    return N, dims, ps, pens, B, b
+
+
+def profile_batch(sipg):
+    """The batched workload's CG and PCG solves once each with graphs off:
+    the same kernels launched directly, for a kernel trace (no graph is
+    replayed in this mode)."""
+    N, dims, ps, pens, B, b = c4_batch(sipg)
+    x = B.vector()
+    B.set_graphs(False)
+    for method in ("cg", "pcg"):
+        solve = B.cg if method == "cg" else B.pcg
+        x.zero()
+        sipg.synchronize()
+        t0 = time.perf_counter()
+        st, infos = solve(x, b, TOL, sipg.TOL_REL, maxit=20000, replace_factor=3e-3)
+        t = time.perf_counter() - t0
+        log(f"batched {method} (graphs off): {sipg.strerror(st)}, {1e3 * t:.0f} ms, "
+            f"max iterations {max(k.iters for k in infos)}")
+
+
+def batched_leg(sipg, log_every=True):
+    """2048 independent FP64 solves in one batch: mesh side ~ U{4..64},
+    degree ~ U{2..10} (numpy default_rng(4)), instance i forced by
+    forcing_draw(i, 1..4 modes, m,n <= 4), pen = 5 (p+1)^2, 1e-10 true
+    relative residual each, by batched CG and by batched block-Jacobi PCG.
+    Each is compared with solving a sample of the same instances (every
+    64th) one after another with the single-problem solver of that method."""
+    N, dims, ps, pens, B, b = c4_batch(sipg)
     shapes = {int(p): B.tuning(int(p))[:2] for p in sorted(set(ps.tolist()))}
     x = B.vector()
 
@@ -350,6 +374,17 @@ def batched_leg(sipg, log_every=True):
     out.update(one("cg"))
     out["pcg"] = one("pcg")
     out["pcg_speedup_vs_cg"] = out["pcg"]["dof_per_s"] / out["dof_per_s"]
+    # what the captured graphs save: the same CG solve with direct launches
+    B.set_graphs(False)
+    x.zero()
+    sipg.synchronize()
+    t0 = time.perf_counter()
+    st, _ = B.cg(x, b, TOL, sipg.TOL_REL, maxit=20000, replace_factor=3e-3)
+    t = time.perf_counter() - t0
+    B.set_graphs(True)
+    out["cg_graphs_off"] = {"ms_per_batch_solve": 1e3 * t, "status": sipg.strerror(st),
+                            "graphs_speedup": t / (1e-3 * out["ms_per_batch_solve"])}
+    log(f"batched cg without graphs: {1e3 * t:.0f} ms ({out['cg_graphs_off']['graphs_speedup']:.2f}x with graphs)")
     return out
 
 
@@ -535,6 +570,8 @@ def main():
     ap.add_argument("--profile-apply-c3", type=int, default=0, metavar="N",
                     help="only N FP32 p=15 applies with each contraction kernel and N copies, then exit")
     ap.add_argument("--no-c3", action="store_true", help="skip the 128x128 p=15 (C3) leg")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="only the batched workload's CG and PCG solves, with graphs off (for a kernel trace)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -545,6 +582,9 @@ def main():
     sipg.check(sipg.lib().sipg_set_device(0), "set_device")
     if args.profile_apply_c3:
         profile_apply_c3(sipg, args.profile_apply_c3)
+        return
+    if args.profile_batch:
+        profile_batch(sipg)
         return
     pen = sipg.pen_sigma(P)
     modes = sipg.forcing_draw(0, 4, 4, 4)

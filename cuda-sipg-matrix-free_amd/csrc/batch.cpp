@@ -57,6 +57,7 @@ struct sipg_batch {
     std::vector<int64_t> off;            // ninst + 1
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    bool graphs = true;                  // replay captured iterations (sipg_batch_set_graphs)
 
     struct Group {
         int p = 0, shape = -1;
@@ -432,6 +433,8 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
     if (n_open > 0) {
         SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
         SIPG_HIP_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    }
+    if (n_open > 0 && B.graphs) {
         SIPG_HIP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeRelaxed));
         for (int it = 0; it < kGraphIters && !rc; ++it) rc = enqueue_iteration(B, X, Bv, it & 1, it == 0, pcg);
         const hipError_t ec = hipStreamEndCapture(B.stream, &graph);
@@ -471,8 +474,17 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
     std::vector<int> rep_it(ni, 0);       // iteration count when the last true residual was requested
     bool stopped = false;
     const double rf2 = opts->replace_factor * opts->replace_factor;
+    // one replay of the captured iterations, or the same launches made
+    // directly when graphs are off
     auto launch_replay = [&]() -> int {
-        SIPG_HIP_TRY(hipGraphLaunch(exec, B.stream));
+        if (exec) {
+            SIPG_HIP_TRY(hipGraphLaunch(exec, B.stream));
+        } else {
+            for (int it = 0; it < kGraphIters; ++it) {
+                const int r = enqueue_iteration(B, X, Bv, it & 1, it == 0, pcg);
+                if (r) return r;
+            }
+        }
         ++replays;
         return SIPG_OK;
     };
@@ -754,6 +766,12 @@ int sipg_batch_set_shape(sipg_batch *b, int p, int bx, int by) {
         return alloc_lists(*b);
     }
     return SIPG_ERR_NOT_FOUND;
+}
+
+int sipg_batch_set_graphs(sipg_batch *b, int use_graphs) {
+    if (!b || (use_graphs != 0 && use_graphs != 1)) return SIPG_ERR_INVALID;
+    b->graphs = use_graphs != 0;
+    return SIPG_OK;
 }
 
 int sipg_batch_apply(sipg_batch *b, const void *x, void *y) {

@@ -131,6 +131,7 @@ def _declare(L):
         "sipg_batch_tune": (i, [_vp]),
         "sipg_batch_tuning": (i, [_vp, i, _ip, _ip, _dp]),
         "sipg_batch_set_shape": (i, [_vp, i, i, i]),
+        "sipg_batch_set_graphs": (i, [_vp, i]),
         "sipg_batch_apply": (i, [_vp, _vp, _vp]),
         "sipg_batch_rhs_fourier": (i, [_vp, i, i, _dp, _ip, _ip, _vp]),
         "sipg_batch_upload": (i, [_vp, i, _dp, _vp]),
@@ -562,6 +563,11 @@ class Batch:
 
     def set_shape(self, p, bx, by):
         return lib().sipg_batch_set_shape(self.h, p, bx, by)
+
+    def set_graphs(self, use_graphs):
+        """True (default): replay captured CG iterations; False: launch them
+        directly (sipg_batch_set_graphs)."""
+        check(lib().sipg_batch_set_graphs(self.h, 1 if use_graphs else 0), "batch_set_graphs")
 
     def apply(self, x, y):
         return lib().sipg_batch_apply(self.h, x.ptr, y.ptr)

@@ -4,7 +4,7 @@
     python -m sipg.rocprof_summary pmc DIR KERNEL_RE    # --pmc runs
 
 ``stats`` prints the per-kernel table of every ``*_kernel_stats.csv``
-under DIR.  ``pmc`` averages each counter over the dispatches of the
+under DIR (or, without CSV output, of the ``*_results.db`` databases).  ``pmc`` averages each counter over the dispatches of the
 kernels whose name matches KERNEL_RE, read from every
 ``*_counter_collection.csv`` under DIR.
 """
@@ -29,10 +29,37 @@ def _col(row, *names):
     raise KeyError(f"none of {names} in {sorted(row)}")
 
 
+def _db_stats(root):
+    """Per-kernel rows (name, calls, avg/min/max ns, total ns) from the
+    rocpd databases (``*_results.db``, rocprofv3's default output) under
+    root, most total time first."""
+    import sqlite3
+    agg = {}
+    for f in sorted(glob.glob(os.path.join(root, "**", "*_results.db"), recursive=True)):
+        con = sqlite3.connect(f)
+        for name, calls, mn, mx, tot in con.execute(
+                "select name, count(*), min(duration), max(duration), sum(duration) from kernels group by name"):
+            a = agg.setdefault(name, [0, float("inf"), 0.0, 0.0])
+            a[0] += calls
+            a[1] = min(a[1], mn)
+            a[2] = max(a[2], mx)
+            a[3] += tot
+        con.close()
+    return sorted(((n, c, t / c, mn, mx, t) for n, (c, mn, mx, t) in agg.items()), key=lambda r: -r[5])
+
+
 def stats(root):
     rows = list(_rows("*kernel_stats.csv", root))
     if not rows:
-        sys.exit(f"no kernel_stats.csv under {root}")
+        db = _db_stats(root)
+        if not db:
+            sys.exit(f"no kernel_stats.csv or *_results.db under {root}")
+        total = sum(r[5] for r in db)
+        print(f"{'calls':>7} {'avg_us':>10} {'min_us':>10} {'max_us':>10} {'total_ms':>10} {'pct':>6}  kernel")
+        for name, calls, avg, mn, mx, tot in db:
+            print(f"{calls:7d} {avg / 1e3:10.2f} {mn / 1e3:10.2f} {mx / 1e3:10.2f} {tot / 1e6:10.3f} "
+                  f"{100.0 * tot / total:6.2f}  {name[:150]}")
+        return
     print(f"{'calls':>7} {'avg_us':>10} {'min_us':>10} {'max_us':>10} {'total_ms':>10} {'pct':>6}  kernel")
     for r in rows:
         calls = int(_col(r, "Calls"))

@@ -284,6 +284,12 @@ int sipg_batch_tuning(const sipg_batch *b, int p, int *bx, int *by, double *us_p
  * instances of degree p, in place of tuning (for callers that tune offline,
  * and for tests of every shape). */
 int sipg_batch_set_shape(sipg_batch *b, int p, int bx, int by);
+/* use_graphs 1 (the default): the solvers capture 8 CG iterations in a
+ * hipGraph once and replay it; 0: they launch the same kernels in the same
+ * order directly on the batch stream each time (identical results; for
+ * callers or tools that cannot use graphs).  No counterpart in the
+ * reference. */
+int sipg_batch_set_graphs(sipg_batch *b, int use_graphs);
 /* y = A_i x_i for every instance (x, y: batch vectors, not overlapping). */
 int sipg_batch_apply(sipg_batch *b, const void *x, void *y);
 /* Fourier forcing of instance `inst` into its part of the batch vector rhs

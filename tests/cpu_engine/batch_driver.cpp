@@ -3,6 +3,8 @@
 // line per instance and case for tests/test_cpu_engine.py.
 #include "../../include/sipg_hip.h"
 
+#include <hip/hip_runtime.h>   // the mock runtime (its execution model)
+
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -81,6 +83,9 @@ static void run(const char *name, const std::vector<int> &dim, const std::vector
 }
 
 int main() {
+    // asynchronous work runs only when the engine synchronises (the
+    // runtime's ordering, with host buffers and graphs read late)
+    mock_deferred() = true;
     run("batch", {6, 4, 5, 3, 7, 4, 2}, {3, 2, 4, 5, 2, 13, 16}, {}, 0);
     run("batch_cap", {6, 4, 5}, {3, 2, 4}, {}, 5);
     // maxit = INT_MAX as "no cap": the replay bound must not overflow
@@ -88,5 +93,7 @@ int main() {
     run("batch_breakdown", {6, 6, 5}, {3, 3, 4}, {80.0, -500.0, 125.0}, 0);
     run("batch_pcg", {6, 4, 5, 3, 7, 4, 2}, {3, 2, 4, 5, 2, 13, 16}, {}, 0, true);
     run("batch_pcg_other_pen", {6, 4}, {3, 2}, {81.0, 45.0}, 0, true);
+    // nothing may be left queued once every call has returned
+    printf("{\"case\":\"deferred_queue\",\"pending\":%zu}\n", mock_pending());
     return 0;
 }

@@ -284,6 +284,7 @@ int launch_layout(int dtype, int to_device, const void *src, void *dst, int64_t 
 int launch_rhs_fourier(int dtype, int p, int dim, const void *, int nmodes, const double *a, const int *m,
                        const int *n, void *b, hipStream_t) {
     if (dtype != SIPG_F64) return SIPG_ERR_UNSUPPORTED;
+    mock_drain();      // runs now: the work queued before it (the modes' copies) first
     orc_problem pb{0, nmodes, a, m, n};
     const int64_t noe = (int64_t)dim * dim, nq = (int64_t)(p + 1) * (p + 1);
     std::vector<double> br(noe * nq);

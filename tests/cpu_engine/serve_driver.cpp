@@ -3,6 +3,8 @@
 // tests/test_cpu_engine.py.
 #include "../../include/sipg_hip.h"
 
+#include <hip/hip_runtime.h>   // the mock runtime (its execution model)
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -79,6 +81,9 @@ static std::map<int, std::vector<double>> serve(const std::vector<Req> &reqs, do
 }
 
 int main() {
+    // asynchronous work runs only when the server synchronises (the
+    // runtime's ordering, with host buffers and graphs read late)
+    mock_deferred() = true;
     const std::vector<Req> reqs = make_requests(24);
     int64_t demand = 0;
     for (auto &r : reqs) demand += (int64_t)r.dim * r.dim * (r.p + 1) * (r.p + 1);
@@ -204,5 +209,7 @@ int main() {
         }
         sipg_server_destroy(S);
     }
+    // nothing may be left queued once every call has returned
+    printf("{\"case\":\"deferred_queue\",\"pending\":%zu}\n", mock_pending());
     return 0;
 }

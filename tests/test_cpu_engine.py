@@ -194,6 +194,16 @@ def serve_results(results):
     return [json.loads(l) for l in out.stdout.strip().splitlines()]
 
 
+def test_batch_and_server_under_deferred_execution(batch_results, serve_results):
+    # both drivers run with the mock's deferred model: copies, launches and
+    # graph replays execute only at the engine's synchronisation points and
+    # read host buffers and graphs then, under ASan/UBSan (the fixtures
+    # fail on any report); every queued item has run once the calls return
+    assert batch_results["deferred_queue"][0]["pending"] == 0
+    q = [r for r in serve_results if r["case"] == "deferred_queue"]
+    assert q and q[0]["pending"] == 0
+
+
 def test_server_admits_within_budget_and_finishes_all(serve_results):
     mix = serve_results[0]
     # the budget holds about a third of the demand: requests had to wait

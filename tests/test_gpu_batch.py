@@ -138,6 +138,23 @@ def test_batch_cg_deterministic(sipg):
     assert np.array_equal(x1.download(np.float64), x2.download(np.float64))
 
 
+@pytest.mark.parametrize("method", ["cg", "pcg"])
+def test_batch_without_graphs_is_the_same_solve(sipg, method):
+    # direct launches of the captured kernels, in the same order: the same
+    # iterates to the bit
+    meshes = [(d, d, p) for d, p in SQUARE[:6]]
+    B, b = _fourier_batch(sipg, meshes)
+    x1, x2 = B.vector(), B.vector()
+    x1.zero()
+    x2.zero()
+    st1, i1 = getattr(B, method)(x1, b, 1e-10, sipg.TOL_REL)
+    B.set_graphs(False)
+    st2, i2 = getattr(B, method)(x2, b, 1e-10, sipg.TOL_REL)
+    assert st1 == st2 == sipg.OK
+    assert [k.iters for k in i1] == [k.iters for k in i2]
+    assert np.array_equal(x1.download(np.float64), x2.download(np.float64))
+
+
 def test_batch_iteration_cap(sipg):
     meshes = [(d, d, p) for d, p in SQUARE[:5]]
     B, b = _fourier_batch(sipg, meshes)
