@@ -283,6 +283,46 @@ def profile_apply_c3(sipg, reps):
     log(f"profile-apply-c3: {reps} MFMA applies, {reps} VALU applies and {reps} copies of {op.n} floats")
 
 
+def large_leg(sipg):
+    """1024 x 1024 elements, p = 8 (84.9M DoF, 680 MB per FP64 vector: x and
+    y of an apply cannot stay in the 256 MB last-level cache): the apply's
+    HBM roofline fraction there, and one two-level PCG solve to 1e-10 (the
+    coarse level's dense sine transforms are O(dim^3) per iteration)."""
+    dim = 1024
+    pen = sipg.pen_sigma(P)
+    op = sipg.Operator(dim, dim, P, pen, sipg.F64)
+    n = op.n
+    x, y = op.vector(), op.vector()
+    x.upload(np.random.default_rng(2).uniform(-1, 1, n))
+    op.time_apply(x, y, 5)
+    ms = op.time_apply(x, y, 50)
+    gbs = 2 * 8 * n / (ms * 1e-3) / 1e9
+    log(f"1024^2 apply: {ms * 1e3:.1f} us/launch, {gbs:.0f} GB/s algorithmic ({gbs / HBM_PEAK_GBS:.3f} of HBM)")
+    out = {"dim": dim, "p": P, "dof": n, "apply_us": ms * 1e3, "achieved": gbs, "frac": gbs / HBM_PEAK_GBS}
+    del y
+    b = op.vector()
+    op.rhs_fourier(sipg.forcing_draw(0, 4, 4, 4), b)
+    op.prec2_apply(b, x)               # builds the preconditioner's tables outside the timing
+    # at this size 1e-10 lies below what FP64 attains (the floor grows with
+    # the condition number, ~dim^2): the solve ends at the floor, not
+    # converged; 1e-7 is reached
+    for tol in (TOL, 1e-7):
+        x.zero()
+        sipg.synchronize()
+        t0 = time.perf_counter()
+        st, info = op.pcg2(x, b, tol, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=3e-3)
+        t = time.perf_counter() - t0
+        rel = op.true_relres(x, b)
+        log(f"1024^2 pcg2 to {tol:.0e}: {sipg.strerror(st)}, {1e3 * t:.0f} ms, {info.iters} iterations "
+            f"({1e6 * t / max(info.iters, 1):.0f} us/iter), true relres {rel:.2e}")
+        out[f"pcg2_tol_{tol:.0e}"] = {
+            "status": "ok" if st == sipg.OK else ("stopped at the FP64 floor" if st == sipg.NOT_CONVERGED
+                                                 else sipg.strerror(st)),
+            "ms_per_solve": 1e3 * t, "iterations": info.iters,
+            "us_per_iteration": 1e6 * t / max(info.iters, 1), "true_relres": rel, "dof_per_s": n / t}
+    return out
+
+
 def c4_batch(sipg):
     """The 2048-instance batch of the batched workload, forced and tuned."""
     N = 2048
@@ -570,6 +610,8 @@ def main():
     ap.add_argument("--profile-apply-c3", type=int, default=0, metavar="N",
                     help="only N FP32 p=15 applies with each contraction kernel and N copies, then exit")
     ap.add_argument("--no-c3", action="store_true", help="skip the 128x128 p=15 (C3) leg")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 1024x1024 p=8 leg (vectors of 680 MB, past the 256 MB last-level cache)")
     ap.add_argument("--profile-batch", action="store_true",
                     help="only the batched workload's CG and PCG solves, with graphs off (for a kernel trace)")
     args = ap.parse_args()
@@ -735,6 +777,7 @@ def main():
     log(f"error vs exact solution: max {err['max']:.3e}, L2 {err['L2']:.3e}, H1 {err['H1']:.3e}")
 
     c1 = c1_leg(sipg)
+    large = None if args.no_large else large_leg(sipg)
     c3 = None if args.no_c3 else c3_leg(sipg, args.steps)
     batched = None if args.no_batch else batched_leg(sipg)
     serving = None
@@ -787,8 +830,11 @@ def main():
             "frac": apply_gbs / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_note": traffic_note,
-            "kernel": "apply_kernel_lds<double,8,16,4,false> (fused volume + flux_term6a + flux_term6b; blocks staged into LDS by global_load_lds, double-buffered)",
+            "kernel": "apply_kernel_wave<double,8,4,false,CxCoef<double,8>> (fused volume + flux_term6a + "
+                      "flux_term6b; a wave owns 7 whole elements of a 7x4-element block, the block and its "
+                      "halo staged into LDS by direct global->LDS loads, two workgroup barriers per block)",
             "algorithmic_bytes_per_launch": apply_bytes,
+            "cache_busting": large,
         },
     }
     if not args.no_cpu:

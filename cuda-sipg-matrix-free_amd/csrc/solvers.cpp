@@ -205,6 +205,21 @@ struct Engine {
         bool restarting = restart == 1;
         int seen_rep = 0;
         double last_ref = cur.rr_ref;
+        // the FP64 floor: the true residual no longer follows the
+        // recursion.  A true residual is requested only once the recursion
+        // has fallen 16x (a final check) or 1/replace_factor^2 (a periodic
+        // replacement) in r.r below the last one; while the solve converges
+        // the true residual falls with it.  kStallChecks (PCG:
+        // kStallChecksPcg) consecutive ones that gain less than 4x in r.r on
+        // the previous one, the first of them stall_window iterations ago,
+        // end the solve there, not converged.  (Not in the mixed-precision
+        // solver, whose refinement cycles each restart the FP32 Krylov
+        // space from the true residual.)
+        const bool floor_watch = opI == opO;
+        bool check_pending = false;
+        double ref_at_rep = 0.0;           // the true r.r before the pending one
+        int lag_count = 0, lag_it = 0, rep_it = 0;
+        double rec_min = -1.0;             // lowest recursive r.r since the last true residual
         // Each pass: decide on the newest polled state `cur`, enqueue one
         // chunk of iterations (4 growing to 32; 8 near the tolerance) and a
         // state copy, and block on the older of two outstanding copies.  Iterations past the finish are no-ops on
@@ -220,14 +235,34 @@ struct Engine {
                 last_ref = cur.rr_ref;
                 seen_rep = cur.nrep;
             }
+            if (rep_visible && check_pending) {
+                // the requested true residual is in and the solve is not
+                // done; the recursion fell 16x or more (in r.r) since the
+                // previous one
+                check_pending = false;
+                if (floor_watch && cur.rr_ref > 0.25 * ref_at_rep) {
+                    if (lag_count++ == 0) lag_it = cs.it;
+                    if (lag_count >= (pcg ? kStallChecksPcg : kStallChecks) && cs.it - lag_it >= stall_window(cs.it))
+                        break;
+                } else {
+                    lag_count = 0;
+                }
+            }
+            // a recursion that does not fall 4x (in r.r) below the last true
+            // residual within stall_window iterations has stalled at it
+            // (preconditioned solvers: the residual of plain CG on this
+            // operator can swing above its start for that long)
+            if (floor_watch && pcg && seen_rep > 0 && !check_pending) {
+                rec_min = rec_min < 0.0 ? cs.rr : std::min(rec_min, cs.rr);
+                if (rec_min > 0.25 * cur.rr_ref && cs.it - rep_it >= stall_window(cs.it)) break;
+            }
             bool want_rep = false, restart_p = restarting;
             if (rep_visible) {
                 // the recursion met the tolerance: check it on a true
                 // residual, but not before it has fallen 4x below the last
                 // one -- near the FP64 floor the true residual lags the
                 // recursion, and replacing at every small drop stalls it
-                if (true_stop && (cs.flags & kSlotWantCheck) && cs.rr <= 0.0625 * cur.rr_ref)
-                    want_rep = true;
+                if (true_stop && (cs.flags & kSlotWantCheck) && cs.rr <= 0.0625 * cur.rr_ref) want_rep = true;
                 if (replace_factor > 0.0 && cs.rr <= replace_factor * replace_factor * cur.rr_ref)
                     want_rep = true;
                 // divergence guard: a recursion that grew 100x past the last
@@ -243,7 +278,13 @@ struct Engine {
             for (int c = 0; c < n_it; ++c)
                 if ((rc = iteration(want_rep && c == 0, restart_p))) break;
             if (rc) break;
-            if (want_rep) ++nrep_enq;
+            if (want_rep) {
+                ++nrep_enq;
+                check_pending = true;
+                ref_at_rep = cur.rr_ref;
+                rep_it = cs.it;
+                rec_min = -1.0;
+            }
             if (chunk < 32) chunk *= 2;
             SIPG_HIP_TRY(hipMemcpyAsync(&hst[buf], st, sizeof(cg_state), hipMemcpyDeviceToHost, s));
             SIPG_HIP_TRY(hipEventRecord(ev[buf], s));

@@ -203,6 +203,10 @@ int main() {
     run("pcg2_plain", 12, 3, 80.0, 5, 1e-10, 0, 0.0, 0, 0);
     run("pcg2_vs_pcg", 12, 3, 80.0, 3, 1e-10, 0, 0.0, 0, 0);
     run("pcg2_one_element", 1, 3, 80.0, 5, 1e-10, 0, 0.0, 0, 0);
+    // a tolerance below the FP64 floor: the solvers end at the floor
+    run("ex_floor", 8, 4, 125.0, 1, 1e-17, 0, 0.1, 0);
+    run("pcg_floor", 8, 4, 125.0, 3, 1e-17, 0, 0.1, 0);
+    run("pcg2_floor", 12, 3, 80.0, 5, 1e-17, 0, 0.1, 0);
     coarse_check(5, 5, 3);
     coarse_check(6, 4, 2);
     coarse_check(4, 5, 1);

@@ -94,6 +94,23 @@ def test_coarse_tables_reproduce_the_galerkin_coarse_operator(results, case):
     assert abs(r["center"] - (4.0 if case.endswith("p1") else 8.0 / 3.0)) < 1e-6, r      # printed to 6 places
 
 
+@pytest.mark.parametrize("case,n", [("pcg_floor", 1600), ("pcg2_floor", 2304)])
+def test_preconditioned_solvers_stop_at_the_fp64_floor(results, case, n):
+    # tolerance 1e-17, below what FP64 attains: the recursion stalls at the
+    # last true residual and the solve ends there, not converged, well
+    # before its iteration cap (n) -- with the true residual at the floor
+    r = results[case]
+    assert r["rc"] == 6 and r["iters"] < 0.7 * n, r
+    assert r["relres"] < 1e-13, r
+
+
+def test_plain_cg_at_the_floor_runs_to_its_cap(results):
+    # plain CG's residual swings too far for the stall test: it reports the
+    # cap, with the true residual at the floor
+    r = results["ex_floor"]
+    assert r["rc"] == 6 and r["iters"] == 1600 and r["relres"] < 1e-13, r
+
+
 @pytest.mark.parametrize("case", ["problem_1_4_p2", "problem_2_5_p3", "problem_3_3_p4", "problem_4_6_p2",
                                   "problem_5_8_p3", "problem_5_1_p5"])
 def test_general_rhs_assembly_is_the_reference_loop(results, case):
