@@ -472,6 +472,7 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
     std::vector<int> best_it(ni, 0);      // iteration count at the best failed check
     std::vector<char> check_req(ni, 0);
     std::vector<int> rep_it(ni, 0);       // iteration count when the last true residual was requested
+    std::vector<double> rec_min(ni, -1.0);  // lowest polled recursive r.r since then
     bool stopped = false;
     const double rf2 = opts->replace_factor * opts->replace_factor;
     // one replay of the captured iterations, or the same launches made
@@ -540,6 +541,20 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
                     }
                     if (gain) check_ref[i] = hs[i].rr_ref;
                 }
+                // PCG: a recursion that does not fall 4x (in r.r) below the
+                // last true residual within stall_window iterations has
+                // stalled at the FP64 floor without meeting the tolerance
+                if (pcg && true_stop && !(c.flags & kSlotWantCheck)) {
+                    rec_min[i] = rec_min[i] < 0.0 ? c.rr : std::min(rec_min[i], c.rr);
+                    if (rec_min[i] > 0.25 * hs[i].rr_ref && c.it - rep_it[i] >= stall_window(c.it)) {
+                        phase[i] = FINISHED;
+                        stopped = true;
+                        --n_open;
+                        ++finished_since;
+                        done_work += B.off[i + 1] - B.off[i];
+                        continue;
+                    }
+                }
                 const bool check = true_stop && (c.flags & kSlotWantCheck) &&
                                    (c.rr <= 0.0625 * hs[i].rr_ref || c.it - rep_it[i] >= force_check_interval(c.it));
                 const bool periodic = rf2 > 0.0 && c.rr <= rf2 * hs[i].rr_ref;
@@ -549,6 +564,7 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
                     ++nrep_req[i];
                     check_req[i] = check ? 1 : 0;
                     rep_it[i] = c.it;
+                    rec_min[i] = -1.0;
                 }
             }
             // the replacement set of replay j+2 (empty most of the time)

@@ -64,7 +64,8 @@ struct Request {
     int check_stale = 0;             // failed checks without a 10% gain on it
     int rep_it = 0;                  // iteration count when the last true residual was requested
     int best_it = 0;                 // iteration count at the best failed final check
-    bool cancel = false;             // cancelled while running: retired at the next progress
+    double rec_min = -1.0;           // lowest polled recursive r.r since the last true residual
+    bool cancel = false;            // cancelled while running: retired at the next progress
     sipg_request_times times{-1.0, -1.0, -1.0};
     sipg_cg_info info{};
 };
@@ -678,6 +679,18 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
                 }
                 if (gain) R.check_ref = st.rr_ref;
             }
+            // PCG: a recursion that does not fall 4x (in r.r) below the last
+            // true residual within stall_window iterations has stalled at
+            // the FP64 floor without meeting the tolerance: the request
+            // ends there
+            if (S->method == SIPG_METHOD_PCG && !(c.flags & kSlotWantCheck)) {
+                R.rec_min = R.rec_min < 0.0 ? c.rr : std::min(R.rec_min, c.rr);
+                if (R.rec_min > 0.25 * st.rr_ref && c.it - R.rep_it >= stall_window(c.it)) {
+                    finish(*S, R, st);
+                    lists_changed = true;
+                    continue;
+                }
+            }
             still.push_back(id);
             const bool check = (c.flags & kSlotWantCheck) &&
                                (c.rr <= 0.0625 * st.rr_ref || c.it - R.rep_it >= force_check_interval(c.it));
@@ -687,6 +700,7 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
                 ++R.nrep_req;
                 R.check_req = check;
                 R.rep_it = c.it;
+                R.rec_min = -1.0;
             }
         }
         S->active.swap(still);
