@@ -31,6 +31,9 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 #include <vector>
 
 using namespace sipg_internal;
@@ -299,8 +302,58 @@ int alloc_lists(sipg_batch &b) {
 
 // Times each candidate block shape on all instances of each degree and
 // keeps the fastest (2 warm-up + 5 timed applies per candidate).
-int tune(sipg_batch &b) {
+// Tuning results kept for the life of the process, keyed by the device's
+// architecture, the degree and the size class of that degree's work (the
+// binary order of its element count; batches are FP64): a later batch of
+// the same kind takes the cached shape without timing.  An explicit
+// sipg_batch_tune always times, and refreshes the entry.
+struct TuneKey {
+    std::string arch;
+    int p, size_class;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(arch, p, size_class) < std::tie(o.arch, o.p, o.size_class);
+    }
+};
+struct TuneVal {
+    int shape;
+    double us[kBatchShapes];
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+std::string device_arch() {
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return "?";
+    return std::string(prop.gcnArchName);
+}
+
+TuneKey tune_key(const sipg_batch &b, const sipg_batch::Group &G, const std::string &arch) {
+    int64_t noe = 0;
+    for (int i : G.inst) noe += (int64_t)b.dimx[i] * b.dimy[i];
+    int cls = 0;
+    while ((noe >> cls) > 1) ++cls;
+    return TuneKey{arch, G.p, cls};
+}
+
+int tune(sipg_batch &b, bool force) {
     int rc;
+    const std::string arch = device_arch();
+    if (!force) {
+        // every degree in the cache: no timing at all
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        bool all = true;
+        for (auto &G : b.groups) all = all && g_tune.count(tune_key(b, G, arch));
+        if (all) {
+            for (auto &G : b.groups) {
+                const TuneVal &v = g_tune.at(tune_key(b, G, arch));
+                G.shape = v.shape;
+                std::copy(v.us, v.us + kBatchShapes, G.us);
+            }
+            b.tuned = true;
+            return alloc_lists(b);
+        }
+    }
     hipEvent_t e0, e1;
     SIPG_HIP_TRY(hipEventCreate(&e0));
     SIPG_HIP_TRY(hipEventCreate(&e1));
@@ -345,6 +398,10 @@ int tune(sipg_batch &b) {
         }
         if (rc) break;
         G.shape = best_shape;
+        TuneVal v{best_shape, {}};
+        std::copy(G.us, G.us + kBatchShapes, v.us);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[tune_key(b, G, arch)] = v;
     }
     if (d_tmp) (void)hipFree(d_tmp);
     (void)hipFree(d_cnt);
@@ -355,7 +412,7 @@ int tune(sipg_batch &b) {
     return alloc_lists(b);
 }
 
-int ensure_tuned(sipg_batch &b) { return b.tuned ? SIPG_OK : tune(b); }
+int ensure_tuned(sipg_batch &b) { return b.tuned ? SIPG_OK : tune(b, false); }
 
 int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opts, sipg_cg_info *info, bool pcg) {
     if (!b || !x || !rhs || !opts) return SIPG_ERR_INVALID;
@@ -753,7 +810,7 @@ int sipg_batch_offset(const sipg_batch *b, int inst, int64_t *offset, int64_t *n
 
 int sipg_batch_tune(sipg_batch *b) {
     if (!b) return SIPG_ERR_INVALID;
-    return tune(*b);
+    return tune(*b, true);
 }
 
 int sipg_batch_tuning(const sipg_batch *b, int p, int *bx, int *by, double *us_per_apply) {

@@ -275,9 +275,14 @@ int sipg_batch_size(const sipg_batch *b, int64_t *total_n);
 int sipg_batch_offset(const sipg_batch *b, int inst, int64_t *offset, int64_t *n);
 /* Chooses the operator kernel's block shape for every degree of the batch
  * by timing the candidates (16x4, 8x4, 8x2 elements) on the batch's own
- * instances; done on first use if not called.  sipg_batch_tuning reports
- * the choice for degree p and its time per apply of all instances of that
- * degree (-1 before tuning). */
+ * instances, and records the choice in a process-wide cache keyed by the
+ * device architecture, the degree and the size class of that degree's
+ * work (the binary order of its element count).  On first use without a
+ * call, a batch whose every degree is in the cache takes those shapes
+ * without timing; otherwise it is timed as here.  sipg_batch_tuning
+ * reports the choice for degree p and its time per apply of all instances
+ * of that degree (as timed when the cache entry was made; -1 before
+ * tuning). */
 int sipg_batch_tune(sipg_batch *b);
 int sipg_batch_tuning(const sipg_batch *b, int p, int *bx, int *by, double *us_per_apply);
 /* Fixes the block shape (bx x by elements, one of the candidates) for the

@@ -77,6 +77,12 @@ inline const char *hipGetErrorString(hipError_t) { return "mock hip error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int *c) { *c = 1; return hipSuccess; }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+struct hipDeviceProp_t { char gcnArchName[256]; };
+inline hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
+    std::strcpy(p->gcnArchName, "mock");
+    return hipSuccess;
+}
 inline hipError_t hipDeviceSynchronize() { mock_drain(); return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { mock_drain(); return hipSuccess; }
 template <typename T> inline hipError_t hipMalloc(T **p, size_t n) {

@@ -48,6 +48,21 @@ def test_batch_tuning_picks_a_candidate(sipg):
         assert not (p >= 13 and bx == 16)
 
 
+def test_batch_tuning_is_cached_for_the_same_kind_of_batch(sipg):
+    # an explicit tune records its choice; a second batch of the same
+    # degrees and sizes tuned on first use takes it without timing (the
+    # reported times are the recorded ones, to the bit)
+    meshes = [(d, d, p) for d, p in [(12, 3), (20, 5), (16, 8), (10, 3)]]
+    B1 = _batch(sipg, meshes)
+    B1.tune()
+    B2 = _batch(sipg, meshes)
+    x, y = B2.vector(), B2.vector()
+    x.zero()
+    assert B2.apply(x, y) == sipg.OK            # first use: tunes
+    for p in (3, 5, 8):
+        assert B2.tuning(p) == B1.tuning(p), p
+
+
 def _fourier_batch(sipg, meshes):
     B = _batch(sipg, meshes)
     b = B.vector()
