@@ -588,7 +588,7 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
                     if (!gain) ++check_stale[i];
                     else best_it[i] = c.it;
                     if (check_stale[i] >= (pcg ? kStallChecksPcg : kStallChecks) &&
-                        c.it - best_it[i] >= stall_window(c.it)) {
+                        c.it - best_it[i] >= stall_window(c.it) && floor_in_reach(hs[i].rr_ref, hs[i].bb)) {
                         phase[i] = FINISHED;
                         stopped = true;
                         --n_open;
@@ -603,7 +603,8 @@ int batch_solve(sipg_batch *b, void *x, const void *rhs, const sipg_cg_opts *opt
                 // stalled at the FP64 floor without meeting the tolerance
                 if (pcg && true_stop && !(c.flags & kSlotWantCheck)) {
                     rec_min[i] = rec_min[i] < 0.0 ? c.rr : std::min(rec_min[i], c.rr);
-                    if (rec_min[i] > 0.25 * hs[i].rr_ref && c.it - rep_it[i] >= stall_window(c.it)) {
+                    if (floor_in_reach(hs[i].rr_ref, hs[i].bb) && rec_min[i] > 0.25 * hs[i].rr_ref &&
+                        c.it - rep_it[i] >= stall_window(c.it)) {
                         phase[i] = FINISHED;
                         stopped = true;
                         --n_open;

@@ -672,7 +672,7 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
                 if (!gain) ++R.check_stale;
                 else R.best_it = c.it;
                 if (R.check_stale >= (S->method == SIPG_METHOD_PCG ? kStallChecksPcg : kStallChecks) &&
-                    c.it - R.best_it >= stall_window(c.it)) {
+                    c.it - R.best_it >= stall_window(c.it) && floor_in_reach(st.rr_ref, st.bb)) {
                     finish(*S, R, st);
                     lists_changed = true;
                     continue;
@@ -685,7 +685,8 @@ int sipg_server_progress(sipg_server *S, int *n_done) {
             // ends there
             if (S->method == SIPG_METHOD_PCG && !(c.flags & kSlotWantCheck)) {
                 R.rec_min = R.rec_min < 0.0 ? c.rr : std::min(R.rec_min, c.rr);
-                if (R.rec_min > 0.25 * st.rr_ref && c.it - R.rep_it >= stall_window(c.it)) {
+                if (floor_in_reach(st.rr_ref, st.bb) && R.rec_min > 0.25 * st.rr_ref &&
+                    c.it - R.rep_it >= stall_window(c.it)) {
                     finish(*S, R, st);
                     lists_changed = true;
                     continue;

@@ -98,6 +98,18 @@ inline int force_check_interval(int it) { return it / 40 > 64 ? it / 40 : 64; }
 // even when single checks land on the peaks of its residual's swings; one
 // at the FP64 floor stops about a quarter of its iterations after arriving.
 inline int stall_window(int it) { return it / 4 > 128 ? it / 4 : 128; }
+// The floor rules act only once the last true residual (r.r = rr_ref) is
+// below kFloorWatchRel of |b| in norm.  The FP64 floor of this operator
+// lies far below that for any mesh that fits in memory (about 8e-11 at
+// 256 x 256 elements of degree 8, 7e-10 at 1024 x 1024, growing about as
+// the mesh side to the power 1.5); above it the residual 2-norm of PCG
+// swings up to ~100x over a hundred iterations early in a solve (it
+// minimises the energy norm of the error, not the residual), which the
+// rules would read as a stall.
+constexpr double kFloorWatchRel = 1e-6;
+inline bool floor_in_reach(double rr_ref, double bb) {
+    return rr_ref <= kFloorWatchRel * kFloorWatchRel * bb;
+}
 struct cg_state {
     cg_slot s[2];
     double bb;       // b.b

@@ -240,7 +240,7 @@ struct Engine {
                 // done; the recursion fell 16x or more (in r.r) since the
                 // previous one
                 check_pending = false;
-                if (floor_watch && cur.rr_ref > 0.25 * ref_at_rep) {
+                if (floor_watch && floor_in_reach(cur.rr_ref, cur.bb) && cur.rr_ref > 0.25 * ref_at_rep) {
                     if (lag_count++ == 0) lag_it = cs.it;
                     if (lag_count >= (pcg ? kStallChecksPcg : kStallChecks) && cs.it - lag_it >= stall_window(cs.it))
                         break;
@@ -254,7 +254,9 @@ struct Engine {
             // operator can swing above its start for that long)
             if (floor_watch && pcg && seen_rep > 0 && !check_pending) {
                 rec_min = rec_min < 0.0 ? cs.rr : std::min(rec_min, cs.rr);
-                if (rec_min > 0.25 * cur.rr_ref && cs.it - rep_it >= stall_window(cs.it)) break;
+                if (floor_in_reach(cur.rr_ref, cur.bb) && rec_min > 0.25 * cur.rr_ref &&
+                    cs.it - rep_it >= stall_window(cs.it))
+                    break;
             }
             bool want_rep = false, restart_p = restarting;
             if (rep_visible) {
@@ -267,7 +269,10 @@ struct Engine {
                     want_rep = true;
                 // divergence guard: a recursion that grew 100x past the last
                 // true residual is recomputed and restarted from p = r
-                if ((replace_factor > 0.0 || true_stop) && cs.rr > 1e4 * cur.rr_ref) {
+                // (FP64 PCG: 10^6x; its residual norm swings up to ~100x
+                // early in a solve while the error's energy norm falls)
+                const double guard = pcg && floor_watch ? 1e12 : 1e4;
+                if ((replace_factor > 0.0 || true_stop) && cs.rr > guard * cur.rr_ref) {
                     want_rep = true;
                     restart_p = true;
                 }

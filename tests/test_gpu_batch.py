@@ -233,3 +233,17 @@ def test_batch_pcg_needs_the_standard_penalty(sipg):
     x.zero()
     st, _ = B.pcg(x, b, 1e-10, sipg.TOL_REL)
     assert st == 3          # SIPG_ERR_UNSUPPORTED
+
+
+def test_batch_pcg_residual_swings_are_not_a_floor(sipg):
+    # forcings whose PCG residual norm swings up to ~90x above its start
+    # early on: the batched floor rules must leave them running
+    meshes = [(256, 256, 8), (256, 256, 8)]
+    B = _batch(sipg, meshes)
+    b = B.vector()
+    for i, seed in enumerate((1, 5)):
+        B.rhs_fourier(i, sipg.forcing_draw(seed, 4, 4, 4), b)
+    x = B.vector()
+    x.zero()
+    st, infos = B.pcg(x, b, 1e-8, sipg.TOL_REL, maxit=20000, replace_factor=1e-1)
+    assert st == sipg.OK, [(k.status, k.iters) for k in infos]
