@@ -227,6 +227,52 @@ constexpr PrecTables<P> make_prec() {
     return r;
 }
 
+// The interior block commutes with the reflection i -> p - i (centro-
+// symmetric K, symmetric nodes and weights), so each eigenvector is even or
+// odd: S[p - i][a] = sig[a] S[i][a].  The Jacobi sweeps keep that to
+// rounding; this table has it exactly (each mirror pair replaced by its
+// signed mean), so a contraction with S can work on the pair sums and
+// differences of its operand, half the multiplies.  asym: the largest
+// departure of the Jacobi table S0 from the symmetry, smax its largest entry.
+// A degenerate eigenvalue lets the sweeps return a basis without parities
+// (p = 1: K is a multiple of the identity and S0 stays the identity); ok()
+// is false there and S0 is the table to use.
+template <int P>
+struct PrecEvenOdd {
+    double S[P + 1][P + 1] = {};
+    double S0[P + 1][P + 1] = {};
+    double Linv[P + 1][P + 1] = {};
+    int sig[P + 1] = {};
+    double asym = 0.0, smax = 0.0;
+    constexpr bool ok() const { return asym <= 1e-10 * smax; }
+};
+
+template <int P>
+constexpr PrecEvenOdd<P> make_prec_even_odd() {
+    const PrecTables<P> t = make_prec<P>();
+    constexpr int n = P + 1, N = P;
+    PrecEvenOdd<P> r{};
+    for (int a = 0; a < n; ++a) {
+        double d = 0.0;
+        for (int i = 0; i < n; ++i) d += t.S[i][a] * t.S[N - i][a];
+        const int sg = d >= 0.0 ? 1 : -1;
+        r.sig[a] = sg;
+        for (int i = 0; 2 * i <= N; ++i) {
+            const double m = 0.5 * (t.S[i][a] + sg * t.S[N - i][a]);
+            r.S[i][a] = m;
+            r.S[N - i][a] = sg * m;             // the middle row of an odd vector: 0
+            const double dev = cabs(t.S[N - i][a] - sg * t.S[i][a]);
+            if (dev > r.asym) r.asym = dev;
+        }
+        for (int i = 0; i < n; ++i) {
+            r.S0[i][a] = t.S[i][a];
+            if (cabs(t.S[i][a]) > r.smax) r.smax = cabs(t.S[i][a]);
+        }
+        for (int c = 0; c < n; ++c) r.Linv[a][c] = t.Linv[a][c];
+    }
+    return r;
+}
+
 // One instance per degree, usable inside device code as constants.
 template <int P>
 struct Basis {

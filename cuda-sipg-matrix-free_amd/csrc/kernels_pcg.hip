@@ -254,6 +254,292 @@ void pcg_rz_kernel(T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ 
     }
 }
 
+// ---- z = M^-1 r with wave-owned elements ----------------------------------
+// The same preconditioner and pass structure with another work split: a
+// wave holds EPW = 64 / (p+1) whole elements (7 at p = 8), lane (s, j) =
+// s (p+1) + j on line j of element s, and streams runs of EPW consecutive
+// elements on its own.  The transposes between the three contraction
+// passes stay inside the wave, so the element loop has no workgroup barrier;
+// with the standard penalty the contractions take the even-odd form of S
+// (cx::PrecEvenOdd), half the multiplies of the plain one.
+template <int P>
+struct RzWave {
+    static constexpr int N1 = P + 1, NQ = N1 * N1;
+    static constexpr int EPW = 64 / N1;
+    static constexpr int TS = tile_stride(NQ);
+    static constexpr int NV = EPW * NQ;                 // values per run
+    static constexpr int KR = (NV + 63) / 64;
+    static constexpr int W = 4;                         // waves per workgroup
+    static constexpr int NT = 64 * W;
+};
+
+template <int P>
+struct PrecEO {
+    static constexpr cx::PrecEvenOdd<P> V = cx::make_prec_even_odd<P>();
+    static constexpr bool EO = V.ok();
+    static_assert(EO || P == 1, "block-Jacobi eigenvectors without a parity");
+};
+
+// LDS writes of one pass are visible to every lane of the wave in the next
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// out[a] = sum_i S[i][a] in[i] (S^T in) and out[d] = sum_c S[d][c] in[c]
+// (S in).  The standard penalty's table in even-odd form works on the pair
+// sums e[i] = in[i] + in[p-i] and differences o[i] = in[i] - in[p-i];
+// otherwise the plain sums run over the compile-time Jacobi table (CX, a
+// degree without parities) or a row-major table in memory.
+template <typename T, int P, bool CX>
+__device__ __forceinline__ T s_coef(const T *__restrict__ S, int i, int a) {
+    if constexpr (CX) return (T)PrecEO<P>::V.S0[i][a];
+    else return S[i * (P + 1) + a];
+}
+
+template <typename T, int P, bool CX>
+__device__ __forceinline__ void contract_st(const T *__restrict__ S, const T (&in)[P + 1], T (&out)[P + 1]) {
+    constexpr int N1 = P + 1, H = N1 / 2;
+    if constexpr (CX && PrecEO<P>::EO) {
+        using E = PrecEO<P>;
+        T e[H], o[H];
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            e[i] = in[i] + in[P - i];
+            o[i] = in[i] - in[P - i];
+        }
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            T acc = T(0);
+            if (E::V.sig[a] > 0) {
+#pragma unroll
+                for (int i = 0; i < H; ++i) acc += (T)E::V.S[i][a] * e[i];
+                if (P % 2 == 0) acc += (T)E::V.S[H][a] * in[H];
+            } else {
+#pragma unroll
+                for (int i = 0; i < H; ++i) acc += (T)E::V.S[i][a] * o[i];
+            }
+            out[a] = acc;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+            T acc = T(0);
+#pragma unroll
+            for (int i = 0; i < N1; ++i) acc += s_coef<T, P, CX>(S, i, a) * in[i];
+            out[a] = acc;
+        }
+    }
+}
+
+template <typename T, int P, bool CX>
+__device__ __forceinline__ void contract_s(const T *__restrict__ S, const T (&in)[P + 1], T (&out)[P + 1]) {
+    constexpr int N1 = P + 1, H = N1 / 2;
+    if constexpr (CX && PrecEO<P>::EO) {
+        using E = PrecEO<P>;
+#pragma unroll
+        for (int d = 0; d < H; ++d) {
+            T ev = T(0), od = T(0);
+#pragma unroll
+            for (int c = 0; c < N1; ++c) {
+                if (E::V.sig[c] > 0) ev += (T)E::V.S[d][c] * in[c];
+                else od += (T)E::V.S[d][c] * in[c];
+            }
+            out[d] = ev + od;
+            out[P - d] = ev - od;
+        }
+        if (P % 2 == 0) {
+            T ev = T(0);
+#pragma unroll
+            for (int c = 0; c < N1; ++c)
+                if (E::V.sig[c] > 0) ev += (T)E::V.S[H][c] * in[c];
+            out[H] = ev;
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < N1; ++d) {
+            T acc = T(0);
+#pragma unroll
+            for (int c = 0; c < N1; ++c) acc += s_coef<T, P, CX>(S, d, c) * in[c];
+            out[d] = acc;
+        }
+    }
+}
+
+// Arguments and flags as pcg_rz_kernel; RESTR: with E4, the restriction's
+// corner sums (for every degree: they are summed in an LDS array of their
+// own).
+template <typename T, int P, bool UPDATE, bool COPYP, bool CX, bool RESTR = false>
+__global__ __launch_bounds__(RzWave<P>::NT)
+void pcg_rz_wave_kernel(T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ z,
+                        T *__restrict__ pcopy, int64_t noe, const T *__restrict__ prec,
+                        const double *__restrict__ ppAp, int np, double *__restrict__ prr,
+                        double *__restrict__ prz, cg_state *st, int slot, double *__restrict__ E4)
+{
+    using G = RzWave<P>;
+    constexpr int N1 = G::N1, NQ = G::NQ, TS = G::TS, EPW = G::EPW, NV = G::NV, KR = G::KR, NT = G::NT;
+    __shared__ __attribute__((aligned(16))) T lds[G::W * EPW * TS];
+    __shared__ T linv[NQ];
+    __shared__ double csum[RESTR ? G::W * 4 * N1 * EPW : 1];
+    __shared__ double red[G::W];
+    __shared__ double total;
+    using TB = cx::Basis<P>;                           // TB::T.z: the nodes (a lane's own j: a load)
+
+    T alpha = T(0);
+    if (UPDATE) {
+        if (st->s[slot].done | st->err) return;
+        double s = 0.0;
+        for (int i = threadIdx.x; i < np; i += NT) s += ppAp[i];
+        const double pAp = group_sum_all<NT>(s, red, &total);
+        const double a = st->s[slot].rz / pAp;
+        if (!(pAp > 0.0) || !isfinite(a)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) st->err = isfinite(pAp) ? 2 : 3;
+            return;
+        }
+        alpha = (T)a;
+    }
+    // 1 / (lambda_a + lambda_c), read per lane (row j) from LDS
+    for (int q = threadIdx.x; q < NQ; q += NT) {
+        if constexpr (CX) linv[q] = (T)PrecEO<P>::V.Linv[q / N1][q % N1];
+        else linv[q] = prec[NQ + q];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = lane / N1, j = lane - s * N1;
+    const bool act = s < EPW;                          // lanes past the last element: copies, no writes
+    T *wl = lds + w * (EPW * TS);
+    T *tile = wl + (act ? s : EPW - 1) * TS;
+    const T *lrow = linv + j * N1;
+    const int64_t nval = noe * NQ;
+    const int64_t nrun = (noe + EPW - 1) / EPW;
+    double srr = 0.0, srz = 0.0;
+
+    for (int64_t run = (int64_t)blockIdx.x * G::W + w; run < nrun; run += (int64_t)gridDim.x * G::W) {
+        const int64_t base = run * NV;
+        T rv[KR];
+        double cv[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int idx = lane + 64 * k;
+            const int64_t gi = base + idx;
+            T v = T(0);
+            if (idx < NV && gi < nval) {
+                v = r[gi];
+                if (UPDATE) {
+                    v = v - alpha * Ap[gi];
+                    r[gi] = v;
+                }
+                srr += (double)v * (double)v;
+            }
+            rv[k] = v;
+            if (idx < NV) wl[lds_index<NQ, TS>(idx)] = v;
+        }
+        wave_sync();
+        // 1: column j of S^T R
+        {
+            T c[N1], t[N1];
+#pragma unroll
+            for (int i = 0; i < N1; ++i) c[i] = tile[i * N1 + j];
+            if (RESTR && E4) {
+                // column j is the x line r(., j) at y node j: x weights here,
+                // the y weight of line j, summed over the lines below
+                double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+                for (int i = 0; i < N1; ++i) {
+                    const double xi = 0.5 * (TB::T.z[i] + 1.0);
+                    a0 += (1.0 - xi) * (double)c[i];
+                    a1 += xi * (double)c[i];
+                }
+                const double eta = 0.5 * (TB::T.z[j] + 1.0);
+                cv[0] = (1.0 - eta) * a0;
+                cv[1] = (1.0 - eta) * a1;
+                cv[2] = eta * a0;
+                cv[3] = eta * a1;
+            }
+            contract_st<T, P, CX>(prec, c, t);
+            if (act) {
+#pragma unroll
+                for (int a = 0; a < N1; ++a) tile[a * N1 + j] = t[a];
+            }
+        }
+        wave_sync();
+        // 2: row j of ((S^T R S) ./ L) S^T
+        {
+            T wv[N1], v[N1];
+#pragma unroll
+            for (int b = 0; b < N1; ++b) wv[b] = tile[j * N1 + b];
+            contract_st<T, P, CX>(prec, wv, v);
+#pragma unroll
+            for (int c = 0; c < N1; ++c) v[c] *= lrow[c];
+            contract_s<T, P, CX>(prec, v, wv);
+            if (act) {
+#pragma unroll
+                for (int d = 0; d < N1; ++d) tile[j * N1 + d] = wv[d];
+            }
+        }
+        wave_sync();
+        // 3: column j of S (...)
+        {
+            T c[N1], t[N1];
+#pragma unroll
+            for (int i = 0; i < N1; ++i) c[i] = tile[i * N1 + j];
+            contract_s<T, P, CX>(prec, c, t);
+            if (act) {
+#pragma unroll
+                for (int a = 0; a < N1; ++a) tile[a * N1 + j] = t[a];
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int idx = lane + 64 * k;
+            const int64_t gi = base + idx;
+            if (idx < NV && gi < nval) {
+                const T zv = wl[lds_index<NQ, TS>(idx)];
+                z[gi] = zv;
+                if (COPYP) pcopy[gi] = zv;
+                srz += (double)rv[k] * (double)zv;
+            }
+        }
+        if (RESTR && E4) {
+            // E4[c * noe + e]: the lines' shares summed in a fixed order
+            double *cs = csum + w * (4 * N1 * EPW);
+            if (act) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) cs[(c * N1 + j) * EPW + s] = cv[c];
+            }
+            wave_sync();
+            for (int q = lane; q < 4 * EPW; q += 64) {
+                const int c = q / EPW, el = q - c * EPW;
+                const int64_t e = run * EPW + el;
+                double sum = 0.0;
+#pragma unroll
+                for (int jj = 0; jj < N1; ++jj) sum += cs[(c * N1 + jj) * EPW + el];
+                if (e < noe) E4[(int64_t)c * noe + e] = sum;
+            }
+        }
+        wave_sync();                                   // the wave's LDS is free for its next run
+    }
+    const double a = block_sum<NT>(srr, red);
+    __syncthreads();
+    const double b = block_sum<NT>(srz, red);
+    if (threadIdx.x == 0) {
+        prr[blockIdx.x] = a;
+        prz[blockIdx.x] = b;
+    }
+}
+
+// Which split z = M^-1 r takes at degree p with es bytes per value: the
+// wave-owned one where it measured faster (z = M^-1 r over ~5.3M values on
+// MI355X, FP64: 23.9 against 28.5 us at p = 8, 31.3 against 68.5 at
+// p = 16; FP32 likewise at p = 4, 6 and from 8), the workgroup-wide one
+// below (up to 15 % faster there: p <= 3, and FP32 at p = 5, 7).
+__host__ __device__ constexpr bool rz_by_wave(int p, int es) {
+    return es == 8 ? p >= 4 : (p == 4 || p == 6 || p >= 8);
+}
+
 // Workgroups of one pcg_rz kernel that the device holds at once, cached per
 // device; the grid is min(groups, that, kMaxPartials).
 template <typename T, int P, bool UPDATE, bool COPYP, bool CX>
@@ -266,20 +552,39 @@ int pcg_resident() {
     if (g > 0) return g;
     int cus = 0, per = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, reinterpret_cast<const void *>(&pcg_rz_kernel<T, P, UPDATE, COPYP, CX>), PcgShape<P>::NT, 0) !=
-        hipSuccess)
-        per = 0;
+    hipError_t e;
+    if constexpr (rz_by_wave(P, (int)sizeof(T)))
+        // the instantiation that also restricts (more registers) where it exists
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, reinterpret_cast<const void *>(&pcg_rz_wave_kernel<T, P, UPDATE, COPYP, CX, CX>), RzWave<P>::NT, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, reinterpret_cast<const void *>(&pcg_rz_kernel<T, P, UPDATE, COPYP, CX>), PcgShape<P>::NT, 0);
+    if (e != hipSuccess) per = 0;
     g = (cus > 0 ? cus : 1) * (per > 0 ? per : 1);
     cache[dev].store(g, std::memory_order_relaxed);
     return g;
 }
 
+// The wave-owned split: W runs of EPW elements per workgroup and pass, in
+// as many passes as it takes for the grid to fit on the device at once
+// (at most kMaxPartials workgroups), with the runs spread evenly over the
+// passes.  Fewer workgroups also means fewer block sums for the kernels
+// that reduce them afterwards (each of their workgroups reads all of them).
 template <typename T, int P, bool UPDATE, bool COPYP, bool CX>
 int pcg_grid(int64_t noe) {
-    const int64_t groups = (noe + PcgShape<P>::NE - 1) / PcgShape<P>::NE;
-    return (int)std::min<int64_t>(std::min<int64_t>(groups, pcg_resident<T, P, UPDATE, COPYP, CX>()),
-                                  kMaxPartials);
+    if constexpr (rz_by_wave(P, (int)sizeof(T))) {
+        using G = RzWave<P>;
+        const int64_t runs = (noe + G::EPW - 1) / G::EPW;
+        const int64_t wgs = std::max<int64_t>(1, (runs + G::W - 1) / G::W);
+        const int64_t cap = std::min<int64_t>(pcg_resident<T, P, UPDATE, COPYP, CX>(), kMaxPartials);
+        const int64_t rounds = (wgs + cap - 1) / cap;
+        return (int)((wgs + rounds - 1) / rounds);
+    } else {
+        const int64_t groups = (noe + PcgShape<P>::NE - 1) / PcgShape<P>::NE;
+        return (int)std::min<int64_t>(std::min<int64_t>(groups, pcg_resident<T, P, UPDATE, COPYP, CX>()),
+                                      kMaxPartials);
+    }
 }
 
 // grid: that of the plain kernel (the block-sum count the solver expects);
@@ -288,6 +593,17 @@ template <typename T, int P, bool UPDATE, bool COPYP, bool CX>
 void rz_launch(dim3 grid, void *r, const void *Ap, void *z, void *pcopy, int64_t noe, const void *prec,
                const double *ppAp, int np, double *prr, double *prz, cg_state *st, int slot, double *E4,
                hipStream_t s) {
+    if constexpr (rz_by_wave(P, (int)sizeof(T))) {
+        if (CX && E4)
+            hipLaunchKernelGGL((pcg_rz_wave_kernel<T, P, UPDATE, COPYP, CX, CX>), grid, dim3(RzWave<P>::NT), 0, s,
+                               (T *)r, (const T *)Ap, (T *)z, (T *)pcopy, noe, (const T *)prec, ppAp, np, prr, prz,
+                               st, slot, E4);
+        else
+            hipLaunchKernelGGL((pcg_rz_wave_kernel<T, P, UPDATE, COPYP, CX>), grid, dim3(RzWave<P>::NT), 0, s,
+                               (T *)r, (const T *)Ap, (T *)z, (T *)pcopy, noe, (const T *)prec, ppAp, np, prr, prz,
+                               st, slot, nullptr);
+        return;
+    }
     if (CX && E4 && rz_fuses_restrict(P, (int)sizeof(T)))
         hipLaunchKernelGGL((pcg_rz_kernel<T, P, UPDATE, COPYP, CX, CX>), grid, dim3(PcgShape<P>::NT), 0, s, (T *)r,
                            (const T *)Ap, (T *)z, (T *)pcopy, noe, (const T *)prec, ppAp, np, prr, prz, st, slot, E4);
@@ -493,7 +809,8 @@ int pcg_num_blocks(int dtype, int p, double pen, int mode, int64_t noe) {
 }
 
 bool pcg_rz_fuses_restrict(int dtype, int p, double pen) {
-    return p >= 1 && p <= kMaxDegree && standard_pen(p, pen) && rz_fuses_restrict(p, dtype == SIPG_F64 ? 8 : 4);
+    const int es = dtype == SIPG_F64 ? 8 : 4;
+    return p >= 1 && p <= kMaxDegree && standard_pen(p, pen) && (rz_by_wave(p, es) || rz_fuses_restrict(p, es));
 }
 
 int launch_pcg_rz(int dtype, int p, double pen, int mode, void *r, const void *Ap, void *z, void *pcopy,
