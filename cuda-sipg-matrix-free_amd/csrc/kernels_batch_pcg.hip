@@ -5,6 +5,7 @@
 // the standard penalty 5 (p+1)^2 (kernels_pcg.hip), so every coefficient is
 // a compile-time constant of the degree; batched PCG requires that penalty.
 #include "apply_block.hpp"
+#include "prec_block.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -29,7 +30,6 @@ void batch_prec_kernel(const double *__restrict__ R, double *__restrict__ Zq, do
     constexpr int RUN = BX * NQ, KR = N1;               // NE*NQ values = KR per thread
     __shared__ double lds[NE * TS];
     __shared__ double red[(NT + 63) / 64];
-    constexpr cx::PrecTables<P> PT = cx::make_prec<P>();
     const int tid = threadIdx.x;
     const int s_el = tid % NE;
     int j = tid / NE;
@@ -68,13 +68,7 @@ void batch_prec_kernel(const double *__restrict__ R, double *__restrict__ Zq, do
             double c[N1], t[N1];
 #pragma unroll
             for (int i = 0; i < N1; ++i) c[i] = tile[i * N1 + j];
-#pragma unroll
-            for (int a = 0; a < N1; ++a) {
-                double acc = 0.0;
-#pragma unroll
-                for (int i = 0; i < N1; ++i) acc += PT.S[i][a] * c[i];
-                t[a] = acc;
-            }
+            contract_st<double, P, true>(nullptr, c, t);
 #pragma unroll
             for (int a = 0; a < N1; ++a) tile[a * N1 + j] = t[a];
         }
@@ -83,20 +77,10 @@ void batch_prec_kernel(const double *__restrict__ R, double *__restrict__ Zq, do
             double w[N1], v[N1];
 #pragma unroll
             for (int b = 0; b < N1; ++b) w[b] = tile[j * N1 + b];
+            contract_st<double, P, true>(nullptr, w, v);
 #pragma unroll
-            for (int c = 0; c < N1; ++c) {
-                double acc = 0.0;
-#pragma unroll
-                for (int b = 0; b < N1; ++b) acc += w[b] * PT.S[b][c];
-                v[c] = acc * PT.Linv[j][c];
-            }
-#pragma unroll
-            for (int d = 0; d < N1; ++d) {
-                double acc = 0.0;
-#pragma unroll
-                for (int c = 0; c < N1; ++c) acc += v[c] * PT.S[d][c];
-                w[d] = acc;
-            }
+            for (int c = 0; c < N1; ++c) v[c] *= PrecEO<P>::V.Linv[j][c];
+            contract_s<double, P, true>(nullptr, v, w);
 #pragma unroll
             for (int d = 0; d < N1; ++d) tile[j * N1 + d] = w[d];
         }
@@ -105,13 +89,7 @@ void batch_prec_kernel(const double *__restrict__ R, double *__restrict__ Zq, do
             double c[N1], t[N1];
 #pragma unroll
             for (int i = 0; i < N1; ++i) c[i] = tile[i * N1 + j];
-#pragma unroll
-            for (int a = 0; a < N1; ++a) {
-                double acc = 0.0;
-#pragma unroll
-                for (int i = 0; i < N1; ++i) acc += PT.S[a][i] * c[i];
-                t[a] = acc;
-            }
+            contract_s<double, P, true>(nullptr, c, t);
 #pragma unroll
             for (int a = 0; a < N1; ++a) tile[a * N1 + j] = t[a];
         }

@@ -22,6 +22,7 @@
 // too (still SPD; about 11 % more iterations than their own blocks, in
 // exchange for coefficients that are uniform across every wave).
 #include "apply_block.hpp"
+#include "prec_block.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -273,97 +274,10 @@ struct RzWave {
     static constexpr int NT = 64 * W;
 };
 
-template <int P>
-struct PrecEO {
-    static constexpr cx::PrecEvenOdd<P> V = cx::make_prec_even_odd<P>();
-    static constexpr bool EO = V.ok();
-    static_assert(EO || P == 1, "block-Jacobi eigenvectors without a parity");
-};
-
 // LDS writes of one pass are visible to every lane of the wave in the next
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
-}
-
-// out[a] = sum_i S[i][a] in[i] (S^T in) and out[d] = sum_c S[d][c] in[c]
-// (S in).  The standard penalty's table in even-odd form works on the pair
-// sums e[i] = in[i] + in[p-i] and differences o[i] = in[i] - in[p-i];
-// otherwise the plain sums run over the compile-time Jacobi table (CX, a
-// degree without parities) or a row-major table in memory.
-template <typename T, int P, bool CX>
-__device__ __forceinline__ T s_coef(const T *__restrict__ S, int i, int a) {
-    if constexpr (CX) return (T)PrecEO<P>::V.S0[i][a];
-    else return S[i * (P + 1) + a];
-}
-
-template <typename T, int P, bool CX>
-__device__ __forceinline__ void contract_st(const T *__restrict__ S, const T (&in)[P + 1], T (&out)[P + 1]) {
-    constexpr int N1 = P + 1, H = N1 / 2;
-    if constexpr (CX && PrecEO<P>::EO) {
-        using E = PrecEO<P>;
-        T e[H], o[H];
-#pragma unroll
-        for (int i = 0; i < H; ++i) {
-            e[i] = in[i] + in[P - i];
-            o[i] = in[i] - in[P - i];
-        }
-#pragma unroll
-        for (int a = 0; a < N1; ++a) {
-            T acc = T(0);
-            if (E::V.sig[a] > 0) {
-#pragma unroll
-                for (int i = 0; i < H; ++i) acc += (T)E::V.S[i][a] * e[i];
-                if (P % 2 == 0) acc += (T)E::V.S[H][a] * in[H];
-            } else {
-#pragma unroll
-                for (int i = 0; i < H; ++i) acc += (T)E::V.S[i][a] * o[i];
-            }
-            out[a] = acc;
-        }
-    } else {
-#pragma unroll
-        for (int a = 0; a < N1; ++a) {
-            T acc = T(0);
-#pragma unroll
-            for (int i = 0; i < N1; ++i) acc += s_coef<T, P, CX>(S, i, a) * in[i];
-            out[a] = acc;
-        }
-    }
-}
-
-template <typename T, int P, bool CX>
-__device__ __forceinline__ void contract_s(const T *__restrict__ S, const T (&in)[P + 1], T (&out)[P + 1]) {
-    constexpr int N1 = P + 1, H = N1 / 2;
-    if constexpr (CX && PrecEO<P>::EO) {
-        using E = PrecEO<P>;
-#pragma unroll
-        for (int d = 0; d < H; ++d) {
-            T ev = T(0), od = T(0);
-#pragma unroll
-            for (int c = 0; c < N1; ++c) {
-                if (E::V.sig[c] > 0) ev += (T)E::V.S[d][c] * in[c];
-                else od += (T)E::V.S[d][c] * in[c];
-            }
-            out[d] = ev + od;
-            out[P - d] = ev - od;
-        }
-        if (P % 2 == 0) {
-            T ev = T(0);
-#pragma unroll
-            for (int c = 0; c < N1; ++c)
-                if (E::V.sig[c] > 0) ev += (T)E::V.S[H][c] * in[c];
-            out[H] = ev;
-        }
-    } else {
-#pragma unroll
-        for (int d = 0; d < N1; ++d) {
-            T acc = T(0);
-#pragma unroll
-            for (int c = 0; c < N1; ++c) acc += s_coef<T, P, CX>(S, d, c) * in[c];
-            out[d] = acc;
-        }
-    }
 }
 
 // Arguments and flags as pcg_rz_kernel; RESTR: with E4, the restriction's
