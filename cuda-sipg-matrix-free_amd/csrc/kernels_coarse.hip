@@ -30,9 +30,23 @@
 //                Cpad and the block partials of S.C
 // and the p update interpolates Cpad into every node (p = z + P0^T C + beta p)
 // with r.z = r.z_BJ + S.C.
+//
+// Large vertex grids (kBlasVertices and up: 1024 x 1024 elements has 1023^2
+// vertices) take the four products as rocBLAS FP64 GEMMs instead, with the
+// gather, the 1/Lambda scaling and the S.C partials as small kernels of
+// their own: the dense products grow as the cube of the mesh side, and the
+// library's matrix-core GEMM runs them several times faster than the
+// folded VALU kernel, which is latency-bound and wins below that size.
+// rocBLAS is opened on first use (dlopen); without it the folded kernel
+// runs at every size.
 #include "sipg_internal.hpp"
+#include "partials.hpp"
+
+#include <dlfcn.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <map>
 
 namespace sipg_internal {
 
@@ -73,9 +87,7 @@ __device__ __forceinline__ double vsum_all(double v) {
 }
 
 __device__ __forceinline__ double vreduce(const double *__restrict__ part, int np) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += VT) s += part[i];
-    return vsum_all(s);
+    return vsum_all(strided_sum<VT>(part, 0, np));
 }
 
 // E4[c * noe + e] = sum_nodes r(e, i1, i2) wx_c(i1) wy_c(i2); corner c = 0
@@ -383,6 +395,126 @@ int dispatch_p(int p, int dtype, int what, const coarse_args &a) {
 
 dim3 gemm_grid(int M, int N) { return dim3((N + GT - 1) / GT, (M + GT - 1) / GT); }
 
+// ---- the rocBLAS route for large vertex grids -------------------------------
+constexpr int64_t kBlasVertices = 511 * 511;
+
+// S(a, b): the sum of the four elements' corner sums meeting at interior
+// vertex (a, b) (the gather of dst_gemm_kernel MODE 3), into S (nvy x nvx)
+__global__ __launch_bounds__(VT) void gather_vertex_kernel(const double *__restrict__ E4, int nvy, int nvx,
+                                                           double *__restrict__ S) {
+    const int64_t nv = (int64_t)nvy * nvx;
+    const int dimx = nvx + 1;
+    const int64_t noe = (int64_t)dimx * (nvy + 1);
+    for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < nv; i += (int64_t)gridDim.x * VT) {
+        const int a = (int)(i / nvx) + 1, b = (int)(i - (i / nvx) * nvx) + 1;
+        S[i] = (E4[1 * noe + (int64_t)(a - 1) * dimx + (b - 1)] + E4[0 * noe + (int64_t)(a - 1) * dimx + b]) +
+               (E4[3 * noe + (int64_t)a * dimx + (b - 1)] + E4[2 * noe + (int64_t)a * dimx + b]);
+    }
+}
+
+// T .*= W (the 1/Lambda table), nv values
+__global__ __launch_bounds__(VT) void scale_kernel(double *__restrict__ T, const double *__restrict__ W, int64_t nv) {
+    for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < nv; i += (int64_t)gridDim.x * VT) T[i] *= W[i];
+}
+
+// per-tile sums of S .* C (C the interior of the zero-bordered grid Cpad),
+// on the tiles and in the partial order of dst_gemm_kernel MODE 2
+__global__ __launch_bounds__(GT * GT) void sc_partials_kernel(const double *__restrict__ S,
+                                                              const double *__restrict__ Cpad, int nvy, int nvx,
+                                                              double *__restrict__ part) {
+    __shared__ double red[GT * GT / 64];
+    const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
+    const int gr = blockIdx.y * GT + ty, gc = blockIdx.x * GT + tx;
+    double dot = 0.0;
+    if (gr < nvy && gc < nvx) dot = S[(int64_t)gr * nvx + gc] * Cpad[(int64_t)(gr + 1) * (nvx + 2) + gc + 1];
+    dot = wsum(dot);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// The three rocBLAS entry points used, resolved once from the installed
+// library.
+struct BlasFns {
+    using create_t = rocblas_status (*)(rocblas_handle *);
+    using stream_t = rocblas_status (*)(rocblas_handle, hipStream_t);
+    using dgemm_t = rocblas_status (*)(rocblas_handle, rocblas_operation, rocblas_operation, rocblas_int,
+                                       rocblas_int, rocblas_int, const double *, const double *, rocblas_int,
+                                       const double *, rocblas_int, const double *, double *, rocblas_int);
+    create_t create = nullptr;
+    stream_t set_stream = nullptr;
+    dgemm_t dgemm = nullptr;
+    bool ok() const { return create && set_stream && dgemm; }
+};
+
+const BlasFns &blas() {
+    static const BlasFns fns = [] {
+        BlasFns f;
+        void *so = nullptr;
+        for (const char *name : {"librocblas.so.5", "librocblas.so", "/opt/rocm/lib/librocblas.so.5"})
+            if ((so = dlopen(name, RTLD_NOW | RTLD_LOCAL))) break;
+        if (so) {
+            f.create = reinterpret_cast<BlasFns::create_t>(dlsym(so, "rocblas_create_handle"));
+            f.set_stream = reinterpret_cast<BlasFns::stream_t>(dlsym(so, "rocblas_set_stream"));
+            f.dgemm = reinterpret_cast<BlasFns::dgemm_t>(dlsym(so, "rocblas_dgemm"));
+        }
+        return f;
+    }();
+    return fns;
+}
+
+// One handle per host thread and device, kept for the life of the process
+// (a rocBLAS handle must not be used by two threads at once; the solver
+// sets its stream before each use).
+rocblas_handle blas_handle() {
+    thread_local std::map<int, rocblas_handle> handles;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto it = handles.find(dev);
+    if (it != handles.end()) return it->second;
+    rocblas_handle h = nullptr;
+    if (blas().create(&h) != rocblas_status_success) return nullptr;
+    handles[dev] = h;
+    return h;
+}
+
+// row-major C (m x n, ldc) = A (m x k, lda) B (k x n, ldb): the column-major
+// product C^T = B^T A^T on the same storage
+bool rm_gemm(rocblas_handle h, int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,
+             int ldc) {
+    const double one = 1.0, zero = 0.0;
+    return blas().dgemm(h, rocblas_operation_none, rocblas_operation_none, n, m, k, &one, B, ldb, A, lda, &zero, C,
+                        ldc) == rocblas_status_success;
+}
+
+int coarse_solve_blas(int nvy, int nvx, const coarse_tables &c, hipStream_t s) {
+    rocblas_handle h = blas_handle();
+    if (!h || blas().set_stream(h, s) != rocblas_status_success) {
+        set_last_hip_error(hipErrorNotInitialized);
+        return SIPG_ERR_HIP;
+    }
+    const int64_t nv = (int64_t)nvy * nvx;
+    const int nb = (int)std::min<int64_t>((nv + VT - 1) / VT, 4096);
+    hipLaunchKernelGGL(gather_vertex_kernel, dim3(nb), dim3(VT), 0, s, c.E4, nvy, nvx, c.S);
+    SIPG_HIP_TRY(hipGetLastError());
+    // T1 = S Qx; T2 = (Qy T1) ./ Lambda; T1 = T2 Qx; C = Qy T1 into Cpad
+    bool ok = rm_gemm(h, nvy, nvx, nvx, c.S, nvx, c.Qx, nvx, c.T1, nvx) &&
+              rm_gemm(h, nvy, nvx, nvy, c.Qy, nvy, c.T1, nvx, c.T2, nvx);
+    if (ok) {
+        hipLaunchKernelGGL(scale_kernel, dim3(nb), dim3(VT), 0, s, c.T2, c.Linv, nv);
+        SIPG_HIP_TRY(hipGetLastError());
+        ok = rm_gemm(h, nvy, nvx, nvx, c.T2, nvx, c.Qx, nvx, c.T1, nvx) &&
+             rm_gemm(h, nvy, nvx, nvy, c.Qy, nvy, c.T1, nvx, c.Cpad + (nvx + 2) + 1, nvx + 2);
+    }
+    if (!ok) {
+        set_last_hip_error(hipErrorLaunchFailure);
+        return SIPG_ERR_HIP;
+    }
+    hipLaunchKernelGGL(sc_partials_kernel, gemm_grid(nvy, nvx), dim3(GT * GT), 0, s, c.S, c.Cpad, nvy, nvx, c.part);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
 }  // namespace
 
 int coarse_num_partials(int dimx, int dimy) {
@@ -405,6 +537,7 @@ int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, con
     a.s = s;
     int rc = have_e4 ? SIPG_OK : dispatch_p<1>(p, dtype, 0, a);
     if (rc) return rc;
+    if ((int64_t)nvx * nvy >= kBlasVertices && blas().ok()) return coarse_solve_blas(nvy, nvx, c, s);
     // T1 = S Qx (S gathered from E4 on the fly, and stored); T2 = (Qy T1) ./
     // Lambda; T1 = T2 Qx; C = Qy T1 (into Cpad), partials of S.C
     const dim3 g = gemm_grid(nvy, nvx), b(GEMM_T);

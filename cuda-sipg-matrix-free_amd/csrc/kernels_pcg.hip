@@ -23,6 +23,7 @@
 // exchange for coefficients that are uniform across every wave).
 #include "apply_block.hpp"
 #include "prec_block.hpp"
+#include "partials.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -95,9 +96,7 @@ void pcg_rz_kernel(T *__restrict__ r, const T *__restrict__ Ap, T *__restrict__ 
     T alpha = T(0);
     if (UPDATE) {
         if (st->s[slot].done | st->err) return;
-        double s = 0.0;
-        for (int i = threadIdx.x; i < np; i += NT) s += ppAp[i];
-        const double pAp = group_sum_all<NT>(s, red, &total);
+        const double pAp = group_sum_all<NT>(strided_sum<NT>(ppAp, 0, np), red, &total);
         const double a = st->s[slot].rz / pAp;
         if (!(pAp > 0.0) || !isfinite(a)) {
             // every workgroup takes this branch (same p.Ap); the first
@@ -302,9 +301,7 @@ void pcg_rz_wave_kernel(T *__restrict__ r, const T *__restrict__ Ap, T *__restri
     T alpha = T(0);
     if (UPDATE) {
         if (st->s[slot].done | st->err) return;
-        double s = 0.0;
-        for (int i = threadIdx.x; i < np; i += NT) s += ppAp[i];
-        const double pAp = group_sum_all<NT>(s, red, &total);
+        const double pAp = group_sum_all<NT>(strided_sum<NT>(ppAp, 0, np), red, &total);
         const double a = st->s[slot].rz / pAp;
         if (!(pAp > 0.0) || !isfinite(a)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) st->err = isfinite(pAp) ? 2 : 3;
@@ -612,9 +609,7 @@ __device__ __forceinline__ double vsum_all(double v) {
 }
 
 __device__ __forceinline__ double vreduce(const double *__restrict__ part, int np) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += VT) s += part[i];
-    return vsum_all(s);
+    return vsum_all(strided_sum<VT>(part, 0, np));
 }
 
 // The step after z = M^-1 r: beta = r.z(new) / r.z(old) and p = z + beta p;

@@ -14,6 +14,7 @@
 //     device: each consumer kernel re-reduces the previous kernel's block
 //     sums itself, and block 0 records them in a cg_state.
 #include "sipg_internal.hpp"
+#include "partials.hpp"
 
 #include <cmath>
 
@@ -50,9 +51,7 @@ __device__ __forceinline__ double block_sum_all(double v) {
 
 // Fixed-order sum of np block partials, identical in every block.
 __device__ __forceinline__ double reduce_partials_dev(const double *__restrict__ part, int np) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += NT) s += part[i];
-    return block_sum_all(s);
+    return block_sum_all(strided_sum<NT>(part, 0, np));
 }
 
 template <typename T>

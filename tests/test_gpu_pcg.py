@@ -220,6 +220,61 @@ def test_prec2_adds_the_galerkin_coarse_solve(sipg, oracle, dx, dy, p):
     assert np.max(np.abs(coarse - want)) <= 1e-11 * np.max(np.abs(want)), np.max(np.abs(coarse - want))
 
 
+def _prolong(v, dx, dy, p, z):
+    """P0^T v without the matrix: v holds the interior vertex values
+    ((dy-1) x (dx-1), vertex (a, b) at x = b/dx, y = 1 - a/dy), interpolated
+    bilinearly at the GLL nodes z of every element, in the reference order
+    (the same map as _interp_matrix)."""
+    n1 = p + 1
+    xi = (z + 1) / 2
+    V = np.zeros((dy + 1, dx + 1))
+    V[1:dy, 1:dx] = v
+    tl, tr, bl, br = V[:-1, :-1], V[:-1, 1:], V[1:, :-1], V[1:, 1:]   # per element (row, col)
+    out = np.empty((n1, n1, dy * dx))
+    for i1 in range(n1):
+        X = xi[i1]
+        for i2 in range(n1):
+            Y = xi[i2]
+            out[i1, i2] = ((1 - X) * Y * tl + X * Y * tr + (1 - X) * (1 - Y) * bl + X * (1 - Y) * br).reshape(-1)
+    return out.reshape(-1)
+
+
+def test_prolong_is_the_interpolation_matrix(oracle):
+    dx, dy, p = 5, 4, 3
+    v = np.random.default_rng(2).uniform(-1, 1, (dy - 1, dx - 1))
+    z, _ = oracle.gll(p)
+    assert np.allclose(_prolong(v, dx, dy, p, z), _interp_matrix(oracle, dx, dy, p) @ v.reshape(-1), rtol=0,
+                       atol=1e-15)
+
+
+@pytest.mark.parametrize("dx,dy,p", [(7, 5, 3), (40, 33, 2), (520, 513, 1), (600, 600, 2)])
+def test_prec2_returns_a_coarse_function(sipg, oracle, dx, dy, p):
+    # r = A P0^T v for a random vertex vector v: then P0 r = A0 v, and the
+    # coarse part of M2^-1 r, P0^T A0^-1 P0 r, is P0^T v itself.  Vertex
+    # grids of 511^2 and more (the last two) take the rocBLAS GEMM route,
+    # the others the folded product kernel; the large square one also solves
+    # (the Fourier forcing is defined on square meshes).
+    pen = sipg.pen_sigma(p)
+    op = sipg.Operator(dx, dy, p, pen, sipg.F64)
+    z, _ = oracle.gll(p)
+    v = np.random.default_rng(dx + 7 * dy + p).uniform(-1, 1, (dy - 1, dx - 1))
+    u = _prolong(v, dx, dy, p, z)
+    ud, rd, z1, z2 = op.vector(), op.vector(), op.vector(), op.vector()
+    op.upload(u, ud)
+    op.apply(ud, rd)
+    op.prec_apply(rd, z1)
+    op.prec2_apply(rd, z2)
+    coarse = op.download(z2) - op.download(z1)
+    assert np.max(np.abs(coarse - u)) <= 1e-8 * np.max(np.abs(u)), np.max(np.abs(coarse - u))
+    if dx >= 500 and dx == dy:
+        b, x = op.vector(), op.vector()
+        op.rhs_fourier(sipg.forcing_draw(0, 4, 4, 4), b)
+        x.zero()
+        st, info = op.pcg2(x, b, 1e-8, sipg.TOL_REL, maxit=2000, replace_factor=0.1, true_stop=True)
+        assert st == sipg.OK and info.iters < 150, (st, info.iters)
+        assert op.true_relres(x, b) <= 1e-8
+
+
 def test_prec2_fp32_matches_fp64_and_is_spd(sipg):
     dim, p = 12, 5
     ops = {t: sipg.Operator(dim, dim, p, sipg.pen_sigma(p), t) for t in (sipg.F64, sipg.F32)}
