@@ -31,8 +31,8 @@
 // and the p update interpolates Cpad into every node (p = z + P0^T C + beta p)
 // with r.z = r.z_BJ + S.C.
 //
-// Large vertex grids (kBlasVertices and up: 1024 x 1024 elements has 1023^2
-// vertices) take the four products as rocBLAS FP64 GEMMs instead, with the
+// Large vertex grids (kBlasVertices and up: from about 271 x 271 elements)
+// take the four products as rocBLAS FP64 GEMMs instead, with the
 // gather, the 1/Lambda scaling and the S.C partials as small kernels of
 // their own: the dense products grow as the cube of the mesh side, and the
 // library's matrix-core GEMM runs them several times faster than the
@@ -396,7 +396,9 @@ int dispatch_p(int p, int dtype, int what, const coarse_args &a) {
 dim3 gemm_grid(int M, int N) { return dim3((N + GT - 1) / GT, (M + GT - 1) / GT); }
 
 // ---- the rocBLAS route for large vertex grids -------------------------------
-constexpr int64_t kBlasVertices = 511 * 511;
+// Two-level PCG at p = 8, us per iteration, folded kernel / rocBLAS:
+// 256^2 elements 135.5 / 198.8, 288^2 217.3 / 191.9, 384^2 377.4 / 324.9.
+constexpr int64_t kBlasVertices = 270 * 270;
 
 // S(a, b): the sum of the four elements' corner sums meeting at interior
 // vertex (a, b) (the gather of dst_gemm_kernel MODE 3), into S (nvy x nvx)

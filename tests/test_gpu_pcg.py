@@ -251,7 +251,7 @@ def test_prolong_is_the_interpolation_matrix(oracle):
 def test_prec2_returns_a_coarse_function(sipg, oracle, dx, dy, p):
     # r = A P0^T v for a random vertex vector v: then P0 r = A0 v, and the
     # coarse part of M2^-1 r, P0^T A0^-1 P0 r, is P0^T v itself.  Vertex
-    # grids of 511^2 and more (the last two) take the rocBLAS GEMM route,
+    # grids of 270^2 and more (the last two) take the rocBLAS GEMM route,
     # the others the folded product kernel; the large square one also solves
     # (the Fourier forcing is defined on square meshes).
     pen = sipg.pen_sigma(p)
