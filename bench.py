@@ -149,13 +149,21 @@ def profile_apply(sipg, op, reps):
         f"({8 * op.n} bytes per vector)")
 
 
-def profile_pcg2(sipg, op, b, reps):
-    """N two-level PCG solves of the 256x256 p=8 FP64 problem (for counter
-    runs over its kernels) and N copies of one vector of the same length,
-    which calibrate FETCH_SIZE / WRITE_SIZE as in profile_apply."""
+def profile_pcg2(sipg, op, b, reps, side=DIM):
+    """N two-level PCG solves of the side x side p=8 FP64 problem (256: to
+    1e-10 in the bench's settings; larger sides, whose FP64 floor lies above
+    1e-10: to 1e-7) for kernel traces and counter runs over its kernels, and
+    N copies of one vector of the same length, which calibrate FETCH_SIZE /
+    WRITE_SIZE as in profile_apply."""
     x, z = op.vector(), op.vector()
     for _ in range(reps):
-        gpu_solve(sipg, op, b, x, "pcg2")
+        if side == DIM:
+            gpu_solve(sipg, op, b, x, "pcg2")
+        else:
+            x.zero()
+            st, info = op.pcg2(x, b, 1e-7, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=3e-3)
+            if st != sipg.OK:
+                raise RuntimeError(f"pcg2 at {side}^2 failed: {sipg.strerror(st)}")
     for _ in range(reps):
         sipg.check(sipg.lib().sipg_vec_copy(sipg.F64, op.n, x.ptr, z.ptr, None), "vec_copy")
     sipg.synchronize()
@@ -604,6 +612,8 @@ def main():
                     help="skip the rocprofv3 counter runs that measure roofline.traffic")
     ap.add_argument("--profile-pcg2", type=int, default=0, metavar="N",
                     help="only N two-level PCG solves and N vector copies (for counter runs)")
+    ap.add_argument("--profile-side", type=int, default=DIM, metavar="S",
+                    help="mesh side of the --profile-pcg2 problem (default 256)")
     ap.add_argument("--profile-apply", type=int, default=0, metavar="N",
                     help="only launch N operator applies and N copies of one vector "
                          "(for hardware-counter runs), then exit")
@@ -630,7 +640,8 @@ def main():
         return
     pen = sipg.pen_sigma(P)
     modes = sipg.forcing_draw(0, 4, 4, 4)
-    op = sipg.Operator(DIM, DIM, P, pen, sipg.F64)
+    side = args.profile_side if args.profile_pcg2 else DIM
+    op = sipg.Operator(side, side, P, pen, sipg.F64)
     n = op.n
     b = op.vector()
     op.rhs_fourier(modes, b)
@@ -640,7 +651,7 @@ def main():
         profile_apply(sipg, op, args.profile_apply)
         return
     if args.profile_pcg2:
-        profile_pcg2(sipg, op, b, args.profile_pcg2)
+        profile_pcg2(sipg, op, b, args.profile_pcg2, side)
         return
 
     # ---- operator-apply roofline (dominant kernel) -----------------------
