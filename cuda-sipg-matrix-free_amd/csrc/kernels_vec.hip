@@ -210,6 +210,34 @@ __global__ __launch_bounds__(NT) void cg_update_p_kernel(const T *__restrict__ r
         p[i] = r[i] + beta * p[i];
 }
 
+// out = x + xl: the iterate that folding the pending increments xl into x
+// would give (the same rounded sums as accumulate_kernel), without folding.
+template <typename T>
+__global__ __launch_bounds__(NT) void sum_kernel(const T *__restrict__ x, const T *__restrict__ xl,
+                                                 T *__restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        out[i] = x[i] + xl[i];
+}
+
+// A true residual that leaves r alone (prr: the block sums of its r.r),
+// between two iterations: when it meets the tolerance, the slot the next
+// iteration reads is finished on it (true residual, checked); otherwise
+// nothing changes.  The single-problem solvers' threshold (stop_true = stop).
+__global__ __launch_bounds__(NT) void peek_finalize_kernel(const double *__restrict__ prr, int np, cg_state *st,
+                                                           int slot) {
+    const double rr = reduce_partials_dev(prr, np);
+    if (threadIdx.x != 0) return;
+    cg_slot c = st->s[slot];
+    if (c.done | st->err) return;
+    if (!isfinite(rr) || sqrt(rr) > st->stop_true) return;
+    c.rr = rr;
+    c.done = 1;
+    c.checked = 1;
+    c.flags = 0;
+    st->s[slot] = c;
+    st->rr_ref = rr;
+}
+
 // Runs between an iteration's x/r update and its p update, after r has
 // been overwritten with b - A x and the rr block sums with its own.
 __global__ void mark_replaced_kernel(cg_state *st, int slot, int restart) {
@@ -235,7 +263,7 @@ __global__ __launch_bounds__(NT) void true_residual_kernel(const TI *__restrict_
         const double bi = (double)bi_;
         const double ri = (double)ri_;
         const TO ro = (TO)ri;
-        out[i] = ro;
+        if (out) out[i] = ro;
         if (pc) pc[i] = ro;
         srr += ri * ri;
         sbb += bi * bi;
@@ -553,6 +581,23 @@ int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, h
 
 int launch_mark_replaced(cg_state *st, int slot, int restart, hipStream_t s) {
     hipLaunchKernelGGL(mark_replaced_kernel, dim3(1), dim3(64), 0, s, st, slot, restart);
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_sum(int dtype, const void *x, const void *xl, void *out, int64_t n, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(sum_kernel<double>, dim3(nb), dim3(NT), 0, s, (const double *)x, (const double *)xl,
+                           (double *)out, n),
+        hipLaunchKernelGGL(sum_kernel<float>, dim3(nb), dim3(NT), 0, s, (const float *)x, (const float *)xl,
+                           (float *)out, n));
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_peek_finalize(const double *prr, int np, cg_state *st, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(peek_finalize_kernel, dim3(1), dim3(NT), 0, s, prr, np, st, slot);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

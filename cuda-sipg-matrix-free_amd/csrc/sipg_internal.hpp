@@ -217,6 +217,10 @@ int launch_true_residual(int in_dtype, int out_dtype, const void *b, const void 
                          void *p_copy, int64_t n, double *prr, double *pbb, int nblk, hipStream_t s);
 // x += xl; xl = 0   (x in x_dtype, xl in xl_dtype; f64+=f64, f64+=f32, f32+=f32)
 int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t s);
+// out = x + xl, and the peek: with prr the block sums of a true residual's
+// r.r that did not replace r, finish slot `slot` on it if it meets st->stop
+int launch_sum(int dtype, const void *x, const void *xl, void *out, int64_t n, hipStream_t s);
+int launch_peek_finalize(const double *prr, int np, cg_state *st, int slot, hipStream_t s);
 // Marks the slot an iteration reads as replaced (r is now b - A x and the
 // rr block sums are its own), optionally with a restart p = r; a no-op
 // for a finished slot.

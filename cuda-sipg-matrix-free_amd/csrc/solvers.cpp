@@ -35,6 +35,7 @@ struct Engine {
     void *xO = nullptr;              // solution vector in outer precision
     const void *b = nullptr;         // right-hand side in outer precision
     void *AxO = nullptr;             // scratch for A x in outer precision
+    void *xpk = nullptr;             // scratch for xO + xI (peeks; null: no peeks)
     bool mixed = false;              // fold xI into xO at each true residual
     int restart = 0;                 // p = r at a true residual: 0 never, 1 always,
                                      // 2 once the true residual stagnates
@@ -153,6 +154,24 @@ struct Engine {
         return SIPG_OK;
     }
 
+    // A peek between two iterations: the true residual of xO + xI (the
+    // iterate a fold would give) without replacing r; when it meets the
+    // tolerance the slot the next iteration reads is finished on it, and
+    // the final fold produces that same iterate.  Ap is free between
+    // iterations and takes A (xO + xI).
+    int peek() {
+        const int slot = (int)(launched & 1);
+        int rc;
+        if ((rc = launch_sum(opO->dtype, xO, xI, xpk, opO->n, s))) return rc;
+        if ((rc = launch_apply(opO->dtype, opO->p, xpk, AxO, opO->d_tab, opO->dimx, opO->dimy, opO->pen, nullptr,
+                               nullptr, 0, s, opO->kernel)))
+            return rc;
+        if ((rc = launch_true_residual(opO->dtype, opO->dtype, b, AxO, nullptr, nullptr, opO->n, P.misc, nullptr, nb,
+                                       s)))
+            return rc;
+        return launch_peek_finalize(P.misc, nb, st, slot, s);
+    }
+
     int iteration(bool replace, bool restart_p) {
         if (pcg) return iteration_pcg(replace, restart_p);
         const int slot = (int)(launched & 1);
@@ -220,6 +239,16 @@ struct Engine {
         double ref_at_rep = 0.0;           // the true r.r before the pending one
         int lag_count = 0, lag_it = 0, rep_it = 0;
         double rec_min = -1.0;             // lowest recursive r.r since the last true residual
+        // Peeks: once the recursion has met the tolerance, the finishing
+        // check waits for it to fall 4x below the last true residual, which
+        // near the FP64 floor takes hundreds of iterations per check while
+        // each true residual scatters around the floor by rounding.  Every
+        // kPeekEvery iterations in that phase a true residual is computed
+        // without replacing r (about one apply's cost): the solve ends on
+        // the first that meets the tolerance, and the recursion is left as
+        // it was.
+        constexpr int kPeekEvery = 32;
+        int last_peek_it = 0;
         // Each pass: decide on the newest polled state `cur`, enqueue one
         // chunk of iterations (4 growing to 32; 8 near the tolerance) and a
         // state copy, and block on the older of two outstanding copies.  Iterations past the finish are no-ops on
@@ -277,6 +306,11 @@ struct Engine {
                     restart_p = true;
                 }
             }
+            if (xpk && rep_visible && !want_rep && !check_pending && (cs.flags & kSlotWantCheck) &&
+                cs.it - last_peek_it >= kPeekEvery) {
+                if ((rc = peek())) break;
+                last_peek_it = cs.it;
+            }
             // near the tolerance, short chunks keep the host's lag small
             const bool near = true_stop && cs.rr <= 1e4 * cur.stop * cur.stop;
             const int n_it = near ? std::min(chunk, 8) : chunk;
@@ -323,7 +357,7 @@ struct Engine {
             info->iters = fs.it;
             info->replacements = f.nrep;
             info->res_norm = std::sqrt(fs.rr);
-            info->true_res_norm = f.nrep > 0 || fs.it == 0 ? std::sqrt(f.rr_ref) : -1.0;
+            info->true_res_norm = f.nrep > 0 || fs.it == 0 || fs.checked ? std::sqrt(f.rr_ref) : -1.0;
             info->b_norm = std::sqrt(f.bb);
         }
         return status;
@@ -422,8 +456,9 @@ int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg
     // into x only when r is recomputed, so the rounding of x += alpha p
     // does not accumulate at the scale of |x| between recomputations.
     const bool group = o->replace_factor > 0.0 || o->true_residual_stop != 0;
+    const bool peeks = group && o->true_residual_stop != 0;
     const int nvec = 3 + (group ? 1 : 0) + (pcg ? 1 : 0);
-    if ((rc = ensure_ws(op, nvec * vb))) return rc;
+    if ((rc = ensure_ws(op, (nvec + (peeks ? 1 : 0)) * vb))) return rc;
     char *ws = (char *)op->ws;
     Engine E;
     E.opI = E.opO = op;
@@ -439,6 +474,7 @@ int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg
     E.coarse = two_level ? &op->coarse : nullptr;
     E.ncp = two_level ? coarse_num_partials(op->dimx, op->dimy) : 0;
     E.AxO = E.Ap;        // the initial and the replacement A x use the Ap slot
+    E.xpk = peeks ? ws + (size_t)nvec * vb : nullptr;
     E.b = b;
     E.replace_factor = o->replace_factor;
     E.true_stop = o->true_residual_stop != 0;

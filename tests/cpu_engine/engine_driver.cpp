@@ -12,6 +12,11 @@ extern "C" {
 }
 
 namespace sipg_internal {
+int &mock_peeks();   // mock_kernels.cpp: solves finished by a peek
+}
+using sipg_internal::mock_peeks;
+
+namespace sipg_internal {
 void coarse_dst(int dim, std::vector<double> &Q, std::vector<double> &lamK, std::vector<double> &lamM, int p);
 }
 
@@ -109,6 +114,7 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
     sipg_upload(op, xh.data(), x);
     sipg_cg_opts o{tol, SIPG_TOL_REL, maxit, rf, true_stop, restart};
     sipg_cg_info info{};
+    mock_peeks() = 0;
     if (mode == 0) {
         int it = 0;
         double res = 0;
@@ -133,8 +139,9 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
     double dx = 0, nx = 0;
     for (int64_t i = 0; i < n; ++i) { dx += (xh[i] - xo[i]) * (xh[i] - xo[i]); nx += xo[i] * xo[i]; }
     printf("{\"case\":\"%s\",\"rc\":%d,\"iters\":%d,\"reps\":%d,\"oracle_iters\":%d,"
-           "\"relres\":%.6e,\"diff\":%.6e}\n",
-           name, rc, info.iters, info.replacements, ito, relres(d, p, pen, xh, bh), std::sqrt(dx / nx));
+           "\"relres\":%.6e,\"diff\":%.6e,\"peeks\":%d,\"true_res\":%.6e,\"b_norm\":%.6e}\n",
+           name, rc, info.iters, info.replacements, ito, relres(d, p, pen, xh, bh), std::sqrt(dx / nx), mock_peeks(),
+           info.true_res_norm, info.b_norm);
     sipg_free(x);
     sipg_free(b);
     sipg_op_destroy(op);
@@ -207,6 +214,10 @@ int main() {
     run("ex_floor", 8, 4, 125.0, 1, 1e-17, 0, 0.1, 0);
     run("pcg_floor", 8, 4, 125.0, 3, 1e-17, 0, 0.1, 0);
     run("pcg2_floor", 12, 3, 80.0, 5, 1e-17, 0, 0.1, 0);
+    // tolerances a little above the floor: between the finishing checks,
+    // peeks at the true residual end the solve
+    run("ex_near_floor", 8, 4, 125.0, 1, 2e-14, 0, 0.1, 0);
+    run("pcg_near_floor", 8, 4, 125.0, 3, 2e-14, 0, 0.1, 0);
     coarse_check(5, 5, 3);
     coarse_check(6, 4, 2);
     coarse_check(4, 5, 1);
