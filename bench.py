@@ -28,6 +28,7 @@ TOL = 1e-10
 # mixed one about 40k FP32 iterations; a solve that hits its cap fails
 # loudly instead of running to the default cap of n iterations.
 MAXIT_FP64 = 30000
+LOOP_EVERY = 16     # the batched leg's loop baseline solves every 16th instance alone
 MAXIT_MIXED = 60000
 
 
@@ -370,7 +371,8 @@ def batched_leg(sipg, log_every=True):
     forcing_draw(i, 1..4 modes, m,n <= 4), pen = 5 (p+1)^2, 1e-10 true
     relative residual each, by batched CG and by batched block-Jacobi PCG.
     Each is compared with solving a sample of the same instances (every
-    64th) one after another with the single-problem solver of that method."""
+    16th: 128 of them) one after another with the single-problem solver of
+    that method."""
     N, dims, ps, pens, B, b = c4_batch(sipg)
     shapes = {int(p): B.tuning(int(p))[:2] for p in sorted(set(ps.tolist()))}
     x = B.vector()
@@ -390,7 +392,7 @@ def batched_leg(sipg, log_every=True):
         failed = [(int(dims[i]), int(ps[i]), int(infos[i].iters), sipg.strerror(infos[i].status))
                   for i in range(N) if infos[i].status != sipg.OK]
         tl, dof = 0.0, 0
-        for i in range(0, N, 64):
+        for i in range(0, N, LOOP_EVERY):
             d, p = int(dims[i]), int(ps[i])
             op = sipg.Operator(d, d, p, pens[i], sipg.F64)
             bi = op.vector()
@@ -415,7 +417,7 @@ def batched_leg(sipg, log_every=True):
                 "status": sipg.strerror(st), "failed_instances": failed[:20], "n_failed": len(failed),
                 "iterations": {"min": int(its.min()), "median": int(np.median(its)), "max": int(its.max())},
                 "max_true_relres": rel,
-                "loop_single_solver": {"instances": len(range(0, N, 64)), "dof_per_s": dof / tl},
+                "loop_single_solver": {"instances": len(range(0, N, LOOP_EVERY)), "dof_per_s": dof / tl},
                 "speedup_vs_loop": (B.n / t) / (dof / tl)}
 
     out = {"instances": N, "dof": int(B.n), "block_shape_per_degree": shapes}
