@@ -169,7 +169,10 @@ int sipg_cg(sipg_op *op, void *x, const void *b, double tol, int tol_mode, int m
  * (the increments since the last one are kept apart and folded into x
  * there); with `true_residual_stop` the iteration finishes only on a true
  * residual below the tolerance (once the recursive one meets it, the true
- * residual is inserted within a few iterations).  This has no counterpart
+ * residual is inserted within a few iterations; if that one fails, b - A x
+ * of the current iterate is also evaluated every 32 iterations without
+ * replacing r, and the first that meets the tolerance ends the solve --
+ * the workspace holds one more vector for it).  This has no counterpart
  * in the reference, whose CG stops on the recursive residual only; it is
  * what makes a relative residual near the FP64 floor reachable. */
 typedef struct {
