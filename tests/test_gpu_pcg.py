@@ -41,8 +41,11 @@ def test_prec_inverts_the_interior_block(sipg, oracle, p, standard):
     assert np.max(np.abs(z[:, 4] - want)) <= 1e-10 * np.max(np.abs(want)) * n1
 
 
-def test_prec_fp32_matches_fp64(sipg):
-    dim, p = 16, 8
+@pytest.mark.parametrize("p", [3, 4, 5, 6, 7, 8, 11, 16])
+def test_prec_fp32_matches_fp64(sipg, p):
+    # FP32 takes the wave-owned block-Jacobi kernel at p = 4, 6 and from 8,
+    # the workgroup one at 3, 5, 7; FP64 the wave one from p = 4
+    dim = 16
     pen = sipg.pen_sigma(p)
     op64 = sipg.Operator(dim, dim, p, pen, sipg.F64)
     op32 = sipg.Operator(dim, dim, p, pen, sipg.F32)
@@ -275,8 +278,11 @@ def test_prec2_returns_a_coarse_function(sipg, oracle, dx, dy, p):
         assert op.true_relres(x, b) <= 1e-8
 
 
-def test_prec2_fp32_matches_fp64_and_is_spd(sipg):
-    dim, p = 12, 5
+@pytest.mark.parametrize("p", [2, 5, 6, 9, 16])
+def test_prec2_fp32_matches_fp64_and_is_spd(sipg, p):
+    # the restriction's corner sums come from the block-Jacobi kernel
+    # (wave-owned: every degree it runs at) or, elsewhere, restrict_kernel
+    dim = 12
     ops = {t: sipg.Operator(dim, dim, p, sipg.pen_sigma(p), t) for t in (sipg.F64, sipg.F32)}
     rng = np.random.default_rng(21)
     a, b = rng.uniform(-1, 1, ops[sipg.F64].n), rng.uniform(-1, 1, ops[sipg.F64].n)
