@@ -45,13 +45,14 @@ SOLVERS = {
     "mixed": "mixed: FP32 two-level PCG cycles (coarse solve in FP64) inside FP64 iterative refinement",
     "pcg2": "FP64 PCG with a two-level preconditioner (the block-Jacobi blocks plus an exact coarse solve "
             "on the continuous bilinear functions of the element vertices, by sine transforms) with "
-            "reliable updates (true residual every 10x drop, search direction kept) and a true-residual stop",
+            "reliable updates (true residual every 100x drop, search direction kept) and a true-residual stop",
 }
 
 
-def gpu_solve(sipg, op, b, x, method):
+def gpu_solve(sipg, op, b, x, method, rf2=1e-2):
     """One solve from x = 0 to a 1e-10 true relative residual; method is
-    "fp64" (CG), "pcg" (block-Jacobi PCG), "pcg2" (two-level PCG) or
+    "fp64" (CG), "pcg" (block-Jacobi PCG), "pcg2" (two-level PCG, a true
+    residual every time the recursive one has fallen by the factor rf2) or
     "mixed"."""
     x.zero()
     sipg.synchronize()
@@ -65,9 +66,11 @@ def gpu_solve(sipg, op, b, x, method):
         st, info = op.pcg(x, b, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=3e-3,
                           true_stop=True, restart=0)
     elif method == "pcg2":
-        # a true residual at every 10x drop: the coarse level's large early
-        # corrections leave the recursion ~1e-9 off b - A x otherwise
-        st, info = op.pcg2(x, b, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=0.1,
+        # the solver keeps the rounding errors of its first (coarse-level)
+        # increments, so b - A x follows the recursion: one true residual
+        # after a 100x drop is enough (61 iterations here; a replacement at
+        # every 10x drop perturbs the recursion: 93)
+        st, info = op.pcg2(x, b, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=rf2,
                            true_stop=True, restart=0)
     else:
         # reliable updates: a true residual each time the recursive one has
@@ -211,10 +214,13 @@ def c3_leg(sipg, steps):
         if kernel is not None:
             op.set_kernel(kernel)
         x = op.vector()
-        gpu_solve(sipg, op, b, x, method)                      # warm-up (tables, FP32 twin)
+        # p=15: the FP64 floor lies closer to 1e-10 than at p=8, and a true
+        # residual at every 10x drop keeps the two-level solve at 125
+        # iterations (157 with one every 100x)
+        gpu_solve(sipg, op, b, x, method, rf2=0.1)             # warm-up (tables, FP32 twin)
         ts = []
         for _ in range(max(steps, 3)):
-            t, info = gpu_solve(sipg, op, b, x, method)
+            t, info = gpu_solve(sipg, op, b, x, method, rf2=0.1)
             ts.append(t)
         rel = op.true_relres(x, b)
         if not rel <= TOL:
@@ -265,8 +271,8 @@ def c1_leg(sipg):
     u = g["u"]
     for method in ("fp64", "pcg2", "mixed"):
         x = op.vector()
-        t, info = gpu_solve(sipg, op, b, x, method)
-        t, info = gpu_solve(sipg, op, b, x, method)
+        t, info = gpu_solve(sipg, op, b, x, method, rf2=0.1)
+        t, info = gpu_solve(sipg, op, b, x, method, rf2=0.1)
         xh = op.download(x)
         out[method] = {"ms_per_solve": 1e3 * t, "iterations": info.iters, "true_relres": op.true_relres(x, b),
                        "rel_l2_vs_lapack": float(np.linalg.norm(xh - u) / np.linalg.norm(u)),
@@ -322,11 +328,12 @@ def large_leg(sipg):
         st, info = op.pcg2(x, b, tol, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=3e-3)
         t = time.perf_counter() - t0
         rel = op.true_relres(x, b)
-        log(f"1024^2 pcg2 to {tol:.0e}: {sipg.strerror(st)}, {1e3 * t:.0f} ms, {info.iters} iterations "
+        status = "ok" if st == sipg.OK else ("stopped at the FP64 floor" if st == sipg.NOT_CONVERGED
+                                             else sipg.strerror(st))
+        log(f"1024^2 pcg2 to {tol:.0e}: {status}, {1e3 * t:.0f} ms, {info.iters} iterations "
             f"({1e6 * t / max(info.iters, 1):.0f} us/iter), true relres {rel:.2e}")
         out[f"pcg2_tol_{tol:.0e}"] = {
-            "status": "ok" if st == sipg.OK else ("stopped at the FP64 floor" if st == sipg.NOT_CONVERGED
-                                                 else sipg.strerror(st)),
+            "status": status,
             "ms_per_solve": 1e3 * t, "iterations": info.iters,
             "us_per_iteration": 1e6 * t / max(info.iters, 1), "true_relres": rel, "dof_per_s": n / t}
     return out

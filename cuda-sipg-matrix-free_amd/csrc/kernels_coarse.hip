@@ -292,7 +292,7 @@ __global__ __launch_bounds__(VT) void pcg2_update_p_kernel(T *__restrict__ x, T 
                                                            const double *__restrict__ pc, int npc,
                                                            const double *__restrict__ Cpad,
                                                            const double *__restrict__ xi, int dimx, cg_state *st,
-                                                           int slot) {
+                                                           int slot, T *__restrict__ xlo) {
     constexpr int N1 = P + 1, NQ = N1 * N1;
     __shared__ double sx[N1];
     const cg_slot cur = st->s[slot];
@@ -330,7 +330,16 @@ __global__ __launch_bounds__(VT) void pcg2_update_p_kernel(T *__restrict__ x, T 
         const int e = (int)(i / NQ), node = (int)(i - (int64_t)e * NQ);
         const int i1 = node / N1, i2 = node - (node / N1) * N1;
         const T pi = p[i];
-        if (!replaced) x[i] = x[i] + alpha * pi;
+        if (!replaced) {
+            if (xlo) {
+                T xh = x[i], xl = xlo[i];
+                comp_axpy(xh, xl, alpha, pi);
+                x[i] = xh;
+                xlo[i] = xl;
+            } else {
+                x[i] = x[i] + alpha * pi;
+            }
+        }
         p[i] = (T)((double)z[i] + prolong<P>(Cpad, sx, dimx, e, i1, i2)) + beta * pi;
     }
 }
@@ -349,15 +358,15 @@ struct CoarseFns {
     template <typename T>
     static void update_(void *x, void *p, const void *z, int64_t n, const double *ppAp, int np_pAp, const double *prr,
                         const double *prz, int np, const double *pc, int npc, const double *Cpad, const double *xi,
-                        int dimx, int nblk, cg_state *st, int slot, hipStream_t s) {
+                        int dimx, int nblk, cg_state *st, int slot, hipStream_t s, void *xlo) {
         hipLaunchKernelGGL((pcg2_update_p_kernel<T, P>), dim3(nblk), dim3(VT), 0, s, (T *)x, (T *)p, (const T *)z, n,
-                           ppAp, np_pAp, prr, prz, np, pc, npc, Cpad, xi, dimx, st, slot);
+                           ppAp, np_pAp, prr, prz, np, pc, npc, Cpad, xi, dimx, st, slot, (T *)xlo);
     }
 };
 
 struct coarse_args {
     const void *r = nullptr;
-    void *v = nullptr, *x = nullptr;
+    void *v = nullptr, *x = nullptr, *xlo = nullptr;
     const void *z = nullptr;
     const double *xi = nullptr, *Cpad = nullptr, *ppAp = nullptr, *prr = nullptr, *prz = nullptr, *pc = nullptr;
     double *E4 = nullptr;
@@ -384,10 +393,10 @@ int dispatch_p(int p, int dtype, int what, const coarse_args &a) {
     } else {
         if (f64)
             F::template update_<double>(a.x, a.v, a.z, a.n, a.ppAp, a.np_pAp, a.prr, a.prz, a.np, a.pc, a.npc, a.Cpad,
-                                        a.xi, a.dimx, a.nblk, a.st, a.slot, a.s);
+                                        a.xi, a.dimx, a.nblk, a.st, a.slot, a.s, a.xlo);
         else
             F::template update_<float>(a.x, a.v, a.z, a.n, a.ppAp, a.np_pAp, a.prr, a.prz, a.np, a.pc, a.npc, a.Cpad,
-                                       a.xi, a.dimx, a.nblk, a.st, a.slot, a.s);
+                                       a.xi, a.dimx, a.nblk, a.st, a.slot, a.s, a.xlo);
     }
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
@@ -566,9 +575,10 @@ int launch_coarse_add(int dtype, int p, void *v, int dimx, int64_t n, const coar
 
 int launch_pcg2_update_p(int dtype, int p, void *x, void *pv, const void *z, int64_t n, const double *part_pAp,
                          int np_pAp, const double *part_rr, const double *part_rz, int np, int npc, int dimx,
-                         const coarse_tables &c, int nblk, cg_state *st, int slot, hipStream_t s) {
+                         const coarse_tables &c, int nblk, cg_state *st, int slot, hipStream_t s, void *xlo) {
     coarse_args a{};
     a.x = x;
+    a.xlo = xlo;
     a.v = pv;
     a.z = z;
     a.n = n;

@@ -623,7 +623,7 @@ __global__ __launch_bounds__(VT) void pcg_update_p_kernel(T *__restrict__ x, T *
                                                           const double *__restrict__ ppAp, int np_pAp,
                                                           const double *__restrict__ prr,
                                                           const double *__restrict__ prz, int np,
-                                                          cg_state *st, int slot) {
+                                                          cg_state *st, int slot, T *__restrict__ xlo) {
     const cg_slot cur = st->s[slot];
     if (cur.done | st->err) {
         if (blockIdx.x == 0 && threadIdx.x == 0) st->s[slot ^ 1] = cur;   // carry forward
@@ -656,7 +656,16 @@ __global__ __launch_bounds__(VT) void pcg_update_p_kernel(T *__restrict__ x, T *
     const T beta = (T)beta_d;
     for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < n; i += (int64_t)gridDim.x * VT) {
         const T pi = p[i];
-        if (!replaced) x[i] = x[i] + alpha * pi;
+        if (!replaced) {
+            if (xlo) {
+                T xh = x[i], xl = xlo[i];
+                comp_axpy(xh, xl, alpha, pi);
+                x[i] = xh;
+                xlo[i] = xl;
+            } else {
+                x[i] = x[i] + alpha * pi;
+            }
+        }
         p[i] = z[i] + beta * pi;
     }
 }
@@ -666,12 +675,20 @@ __global__ __launch_bounds__(VT) void pcg_update_p_kernel(T *__restrict__ x, T *
 template <typename T>
 __global__ __launch_bounds__(VT) void pcg_x_kernel(T *__restrict__ x, const T *__restrict__ p, int64_t n,
                                                    const double *__restrict__ ppAp, int np,
-                                                   const cg_state *st, int slot) {
+                                                   const cg_state *st, int slot, T *__restrict__ xlo) {
     if (st->s[slot].done | st->err) return;
     const double pAp = vreduce(ppAp, np);
     const T alpha = (T)(st->s[slot].rz / pAp);
-    for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < n; i += (int64_t)gridDim.x * VT)
-        x[i] = x[i] + alpha * p[i];
+    for (int64_t i = (int64_t)blockIdx.x * VT + threadIdx.x; i < n; i += (int64_t)gridDim.x * VT) {
+        if (xlo) {
+            T xh = x[i], xl = xlo[i];
+            comp_axpy(xh, xl, alpha, p[i]);
+            x[i] = xh;
+            xlo[i] = xl;
+        } else {
+            x[i] = x[i] + alpha * p[i];
+        }
+    }
 }
 
 // Initial state: r.r and r.z from the preconditioner's block sums, b.b
@@ -735,25 +752,25 @@ int launch_pcg_rz(int dtype, int p, double pen, int mode, void *r, const void *A
 
 int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, const double *part_pAp,
                         int np_pAp, const double *part_rr, const double *part_rz, int np, int nblk,
-                        cg_state *st, int slot, hipStream_t s) {
+                        cg_state *st, int slot, hipStream_t s, void *xlo) {
     if (dtype == SIPG_F64)
         hipLaunchKernelGGL(pcg_update_p_kernel<double>, dim3(nblk), dim3(VT), 0, s, (double *)x, (double *)p,
-                           (const double *)z, n, part_pAp, np_pAp, part_rr, part_rz, np, st, slot);
+                           (const double *)z, n, part_pAp, np_pAp, part_rr, part_rz, np, st, slot, (double *)xlo);
     else
         hipLaunchKernelGGL(pcg_update_p_kernel<float>, dim3(nblk), dim3(VT), 0, s, (float *)x, (float *)p,
-                           (const float *)z, n, part_pAp, np_pAp, part_rr, part_rz, np, st, slot);
+                           (const float *)z, n, part_pAp, np_pAp, part_rr, part_rz, np, st, slot, (float *)xlo);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }
 
 int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *part_pAp, int np_pAp, int nblk,
-                 const cg_state *st, int slot, hipStream_t s) {
+                 const cg_state *st, int slot, hipStream_t s, void *xlo) {
     if (dtype == SIPG_F64)
         hipLaunchKernelGGL(pcg_x_kernel<double>, dim3(nblk), dim3(VT), 0, s, (double *)x, (const double *)p, n,
-                           part_pAp, np_pAp, st, slot);
+                           part_pAp, np_pAp, st, slot, (double *)xlo);
     else
         hipLaunchKernelGGL(pcg_x_kernel<float>, dim3(nblk), dim3(VT), 0, s, (float *)x, (const float *)p, n,
-                           part_pAp, np_pAp, st, slot);
+                           part_pAp, np_pAp, st, slot, (float *)xlo);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(NT) void cg_update_xr_kernel(T *__restrict__ x, con
                                                           T *__restrict__ r, const T *__restrict__ Ap,
                                                           int64_t n, const double *__restrict__ ppAp,
                                                           int np, double *__restrict__ prr,
-                                                          cg_state *st, int slot) {
+                                                          cg_state *st, int slot, T *__restrict__ xlo) {
     if (st->s[slot].done | st->err) return;
     const double pAp = reduce_partials_dev(ppAp, np);
     const double rr = st->s[slot].rr;
@@ -158,7 +158,14 @@ __global__ __launch_bounds__(NT) void cg_update_xr_kernel(T *__restrict__ x, con
     const T alpha = (T)alpha_d;
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-        x[i] = x[i] + alpha * p[i];
+        if (xlo) {
+            T xh = x[i], xl = xlo[i];
+            comp_axpy(xh, xl, alpha, p[i]);
+            x[i] = xh;
+            xlo[i] = xl;
+        } else {
+            x[i] = x[i] + alpha * p[i];
+        }
         const T ri = r[i] - alpha * Ap[i];
         r[i] = ri;
         s += (double)ri * (double)ri;
@@ -210,13 +217,14 @@ __global__ __launch_bounds__(NT) void cg_update_p_kernel(const T *__restrict__ r
         p[i] = r[i] + beta * p[i];
 }
 
-// out = x + xl: the iterate that folding the pending increments xl into x
-// would give (the same rounded sums as accumulate_kernel), without folding.
+// out = x + xl (+ its compensation xlo): the iterate that folding the
+// pending increments into x would give (the same rounded sums as
+// accumulate_kernel), without folding.
 template <typename T>
 __global__ __launch_bounds__(NT) void sum_kernel(const T *__restrict__ x, const T *__restrict__ xl,
-                                                 T *__restrict__ out, int64_t n) {
+                                                 const T *__restrict__ xlo, T *__restrict__ out, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
-        out[i] = x[i] + xl[i];
+        out[i] = x[i] + (xlo ? xl[i] + xlo[i] : xl[i]);
 }
 
 // A true residual that leaves r alone (prr: the block sums of its r.r),
@@ -278,10 +286,24 @@ __global__ __launch_bounds__(NT) void true_residual_kernel(const TI *__restrict_
 
 template <typename TO, typename TL>
 __global__ __launch_bounds__(NT) void accumulate_kernel(TO *__restrict__ x, TL *__restrict__ xl,
-                                                        int64_t n) {
+                                                        TL *__restrict__ xlo, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-        x[i] += (TO)xl[i];
+        x[i] += (TO)(xlo ? xl[i] + xlo[i] : xl[i]);
         xl[i] = TL(0);
+        if (xlo) xlo[i] = TL(0);
+    }
+}
+
+// x += xi exactly as x + xlo (the sum's rounding error joins xlo), xi = 0
+template <typename T>
+__global__ __launch_bounds__(NT) void comp_fold_kernel(T *__restrict__ x, T *__restrict__ xlo, T *__restrict__ xi,
+                                                       int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        T xh = x[i], xl = xlo[i];
+        comp_axpy(xh, xl, T(1), xi[i]);
+        x[i] = xh;
+        xlo[i] = xl;
+        xi[i] = T(0);
     }
 }
 
@@ -525,10 +547,10 @@ int launch_cg_init_finalize(const double *part_rr, const double *part_bb, int np
 
 int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *Ap, int64_t n,
                         const double *part_pAp, int np_pAp, double *part_rr, int nblk, cg_state *st,
-                        int slot, hipStream_t s) {
+                        int slot, hipStream_t s, void *xlo) {
     DISPATCH_T(dtype,
-        hipLaunchKernelGGL(cg_update_xr_kernel<double>, dim3(nblk), dim3(NT), 0, s, (double *)x, (const double *)p, (double *)r, (const double *)Ap, n, part_pAp, np_pAp, part_rr, st, slot),
-        hipLaunchKernelGGL(cg_update_xr_kernel<float>, dim3(nblk), dim3(NT), 0, s, (float *)x, (const float *)p, (float *)r, (const float *)Ap, n, part_pAp, np_pAp, part_rr, st, slot));
+        hipLaunchKernelGGL(cg_update_xr_kernel<double>, dim3(nblk), dim3(NT), 0, s, (double *)x, (const double *)p, (double *)r, (const double *)Ap, n, part_pAp, np_pAp, part_rr, st, slot, (double *)xlo),
+        hipLaunchKernelGGL(cg_update_xr_kernel<float>, dim3(nblk), dim3(NT), 0, s, (float *)x, (const float *)p, (float *)r, (const float *)Ap, n, part_pAp, np_pAp, part_rr, st, slot, (float *)xlo));
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }
@@ -565,16 +587,27 @@ int launch_true_residual(int in_dtype, int out_dtype, const void *b, const void 
     return SIPG_OK;
 }
 
-int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t s) {
+int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t s, void *xlo) {
     const int nb = vec_num_blocks(n);
     if (x_dtype == SIPG_F64 && xl_dtype == SIPG_F64)
-        hipLaunchKernelGGL((accumulate_kernel<double, double>), dim3(nb), dim3(NT), 0, s, (double *)x, (double *)xl, n);
+        hipLaunchKernelGGL((accumulate_kernel<double, double>), dim3(nb), dim3(NT), 0, s, (double *)x, (double *)xl, (double *)xlo, n);
     else if (x_dtype == SIPG_F64 && xl_dtype == SIPG_F32)
-        hipLaunchKernelGGL((accumulate_kernel<double, float>), dim3(nb), dim3(NT), 0, s, (double *)x, (float *)xl, n);
+        hipLaunchKernelGGL((accumulate_kernel<double, float>), dim3(nb), dim3(NT), 0, s, (double *)x, (float *)xl, (float *)xlo, n);
     else if (x_dtype == SIPG_F32 && xl_dtype == SIPG_F32)
-        hipLaunchKernelGGL((accumulate_kernel<float, float>), dim3(nb), dim3(NT), 0, s, (float *)x, (float *)xl, n);
+        hipLaunchKernelGGL((accumulate_kernel<float, float>), dim3(nb), dim3(NT), 0, s, (float *)x, (float *)xl, (float *)xlo, n);
     else
         return SIPG_ERR_INVALID;
+    SIPG_HIP_TRY(hipGetLastError());
+    return SIPG_OK;
+}
+
+int launch_comp_fold(int dtype, void *x, void *xlo, void *xi, int64_t n, hipStream_t s) {
+    const int nb = vec_num_blocks(n);
+    DISPATCH_T(dtype,
+        hipLaunchKernelGGL(comp_fold_kernel<double>, dim3(nb), dim3(NT), 0, s, (double *)x, (double *)xlo,
+                           (double *)xi, n),
+        hipLaunchKernelGGL(comp_fold_kernel<float>, dim3(nb), dim3(NT), 0, s, (float *)x, (float *)xlo, (float *)xi,
+                           n));
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }
@@ -585,13 +618,13 @@ int launch_mark_replaced(cg_state *st, int slot, int restart, hipStream_t s) {
     return SIPG_OK;
 }
 
-int launch_sum(int dtype, const void *x, const void *xl, void *out, int64_t n, hipStream_t s) {
+int launch_sum(int dtype, const void *x, const void *xl, void *out, int64_t n, hipStream_t s, const void *xlo) {
     const int nb = vec_num_blocks(n);
     DISPATCH_T(dtype,
         hipLaunchKernelGGL(sum_kernel<double>, dim3(nb), dim3(NT), 0, s, (const double *)x, (const double *)xl,
-                           (double *)out, n),
+                           (const double *)xlo, (double *)out, n),
         hipLaunchKernelGGL(sum_kernel<float>, dim3(nb), dim3(NT), 0, s, (const float *)x, (const float *)xl,
-                           (float *)out, n));
+                           (const float *)xlo, (float *)out, n));
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

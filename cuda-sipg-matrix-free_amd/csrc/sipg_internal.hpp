@@ -207,7 +207,7 @@ int launch_cg_init_finalize(const double *part_rr, const double *part_bb, int np
                             double tol, int tol_mode, int maxit, int true_stop, hipStream_t s);
 int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *Ap, int64_t n,
                         const double *part_pAp, int np_pAp, double *part_rr, int nblk,
-                        cg_state *st, int slot, hipStream_t s);
+                        cg_state *st, int slot, hipStream_t s, void *xlo = nullptr);
 int launch_cg_update_p(int dtype, const void *r, void *p, int64_t n, const double *part_rr,
                        int np_rr, int nblk, cg_state *st, int slot, hipStream_t s);
 int launch_residual(int dtype, const void *b, const void *Ax, void *r, int64_t n, hipStream_t s);
@@ -216,10 +216,12 @@ int launch_residual(int dtype, const void *b, const void *Ax, void *r, int64_t n
 int launch_true_residual(int in_dtype, int out_dtype, const void *b, const void *Ax, void *out,
                          void *p_copy, int64_t n, double *prr, double *pbb, int nblk, hipStream_t s);
 // x += xl; xl = 0   (x in x_dtype, xl in xl_dtype; f64+=f64, f64+=f32, f32+=f32)
-int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t s);
-// out = x + xl, and the peek: with prr the block sums of a true residual's
+int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t s, void *xlo = nullptr);
+// x += xi with the rounding error added to xlo (both x's dtype), xi = 0
+int launch_comp_fold(int dtype, void *x, void *xlo, void *xi, int64_t n, hipStream_t s);
+// out = x + (xl + xlo), and the peek: with prr the block sums of a true residual's
 // r.r that did not replace r, finish slot `slot` on it if it meets st->stop
-int launch_sum(int dtype, const void *x, const void *xl, void *out, int64_t n, hipStream_t s);
+int launch_sum(int dtype, const void *x, const void *xl, void *out, int64_t n, hipStream_t s, const void *xlo = nullptr);
 int launch_peek_finalize(const double *prr, int np, cg_state *st, int slot, hipStream_t s);
 // Marks the slot an iteration reads as replaced (r is now b - A x and the
 // rr block sums are its own), optionally with a restart p = r; a no-op
@@ -246,10 +248,10 @@ int launch_pcg_rz(int dtype, int p, double pen, int mode, void *r, const void *A
 // writes the next slot
 int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, const double *part_pAp,
                         int np_pAp, const double *part_rr, const double *part_rz, int np, int nblk,
-                        cg_state *st, int slot, hipStream_t s);
+                        cg_state *st, int slot, hipStream_t s, void *xlo = nullptr);
 // x += alpha p, before a true residual replaces r in the same iteration
 int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *part_pAp, int np_pAp, int nblk,
-                 const cg_state *st, int slot, hipStream_t s);
+                 const cg_state *st, int slot, hipStream_t s, void *xlo = nullptr);
 int launch_pcg_init_finalize(const double *part_rr, const double *part_rz, int np, const double *part_bb,
                              int npb, cg_state *st, double tol, int tol_mode, int maxit, int true_stop,
                              hipStream_t s, const double *part_c = nullptr, int npc = 0);
@@ -275,7 +277,7 @@ int launch_coarse_add(int dtype, int p, void *v, int dimx, int64_t n, const coar
 // the PCG p update with the coarse level (r.z gains S.C, p gains P0^T C)
 int launch_pcg2_update_p(int dtype, int p, void *x, void *pv, const void *z, int64_t n, const double *part_pAp,
                          int np_pAp, const double *part_rr, const double *part_rz, int np, int npc, int dimx,
-                         const coarse_tables &c, int nblk, cg_state *st, int slot, hipStream_t s);
+                         const coarse_tables &c, int nblk, cg_state *st, int slot, hipStream_t s, void *xlo = nullptr);
 // host tables of the coarse solve: Q (n x n, n = dim - 1), and the 1-D
 // eigenvalues of K and of the GLL-quadrature Q1 mass
 void coarse_dst(int dim, std::vector<double> &Q, std::vector<double> &lamK, std::vector<double> &lamM, int p);

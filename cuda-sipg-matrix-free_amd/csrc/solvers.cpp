@@ -36,6 +36,13 @@ struct Engine {
     const void *b = nullptr;         // right-hand side in outer precision
     void *AxO = nullptr;             // scratch for A x in outer precision
     void *xpk = nullptr;             // scratch for xO + xI (peeks; null: no peeks)
+    void *xL = nullptr;              // rounding errors of the xI updates (compensated
+                                     // sum; null: plain updates)
+    // The first kCompIters iterations update xI with compensation (their
+    // increments carry the coarse solution, a large part of x); then xI is
+    // folded into xO with its rounding error kept in xL, and the later,
+    // much smaller increments accumulate plainly in a fresh xI.
+    static constexpr int kCompIters = 16;
     bool mixed = false;              // fold xI into xO at each true residual
     int restart = 0;                 // p = r at a true residual: 0 never, 1 always,
                                      // 2 once the true residual stagnates
@@ -72,6 +79,7 @@ struct Engine {
                                opO->pen, nullptr, nullptr, 0, s, opO->kernel)))
             return rc;
         if (mixed) SIPG_HIP_TRY(hipMemsetAsync(xI, 0, (size_t)opI->n * dtype_size(opI->dtype), s));
+        if (xL) SIPG_HIP_TRY(hipMemsetAsync(xL, 0, (size_t)opI->n * dtype_size(opI->dtype), s));
         if (pcg) {
             // r = b - A x ; z = M^-1 r ; p = z ; r.r, r.z, b.b
             // (conjugate_gradient_gpu.hpp:136-160)
@@ -109,6 +117,8 @@ struct Engine {
     int iteration_pcg(bool replace, bool restart_p) {
         const int slot = (int)(launched & 1);
         int rc;
+        if (xL && launched == kCompIters && (rc = launch_comp_fold(opO->dtype, xO, xL, xI, opO->n, s))) return rc;
+        void *xLu = xL && launched < kCompIters ? xL : nullptr;     // compensate this iteration's update
         if ((rc = launch_apply(opI->dtype, opI->p, p, Ap, opI->d_tab, opI->dimx, opI->dimy, opI->pen,
                                P.pAp, st, slot, s, opI->kernel)))
             return rc;
@@ -127,8 +137,8 @@ struct Engine {
             return rc;
         int nrz = npc(1);
         if (replace) {
-            if ((rc = launch_pcg_x(opI->dtype, xI, p, opI->n, src, np, nb, st, slot, s))) return rc;
-            if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s))) return rc;
+            if ((rc = launch_pcg_x(opI->dtype, xI, p, opI->n, src, np, nb, st, slot, s, xLu))) return rc;
+            if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s, xL))) return rc;
             if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
                                    opO->pen, nullptr, nullptr, 0, s, opO->kernel)))
                 return rc;
@@ -146,9 +156,9 @@ struct Engine {
         }
         if (coarse)
             rc = launch_pcg2_update_p(opI->dtype, opI->p, xI, p, z, opI->n, src, np, P.rr, P.rz, nrz, ncp, opI->dimx,
-                                      *coarse, nb, st, slot, s);
+                                      *coarse, nb, st, slot, s, xLu);
         else
-            rc = launch_pcg_update_p(opI->dtype, xI, p, z, opI->n, src, np, P.rr, P.rz, nrz, nb, st, slot, s);
+            rc = launch_pcg_update_p(opI->dtype, xI, p, z, opI->n, src, np, P.rr, P.rz, nrz, nb, st, slot, s, xLu);
         if (rc) return rc;
         ++launched;
         return SIPG_OK;
@@ -162,7 +172,7 @@ struct Engine {
     int peek() {
         const int slot = (int)(launched & 1);
         int rc;
-        if ((rc = launch_sum(opO->dtype, xO, xI, xpk, opO->n, s))) return rc;
+        if ((rc = launch_sum(opO->dtype, xO, xI, xpk, opO->n, s, xL))) return rc;
         if ((rc = launch_apply(opO->dtype, opO->p, xpk, AxO, opO->d_tab, opO->dimx, opO->dimy, opO->pen, nullptr,
                                nullptr, 0, s, opO->kernel)))
             return rc;
@@ -186,10 +196,10 @@ struct Engine {
             src = P.misc;
             np = 1;
         }
-        if ((rc = launch_cg_update_xr(opI->dtype, xI, p, r, Ap, opI->n, src, np, P.rr, nb, st, slot, s)))
+        if ((rc = launch_cg_update_xr(opI->dtype, xI, p, r, Ap, opI->n, src, np, P.rr, nb, st, slot, s, xL)))
             return rc;
         if (replace) {
-            if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s))) return rc;
+            if (mixed && (rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s, xL))) return rc;
             if ((rc = launch_apply(opO->dtype, opO->p, xO, AxO, opO->d_tab, opO->dimx, opO->dimy,
                                    opO->pen, nullptr, nullptr, 0, s, opO->kernel)))
                 return rc;
@@ -340,7 +350,7 @@ struct Engine {
         }
         // fold whatever the last cycle accumulated (a no-op after a final
         // true residual, which already folded it)
-        if (!rc && mixed) rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s);
+        if (!rc && mixed) rc = launch_accumulate(opO->dtype, opI->dtype, xO, xI, opI->n, s, xL);
         SIPG_HIP_TRY(hipStreamSynchronize(s));
         (void)hipEventDestroy(ev[0]);
         (void)hipEventDestroy(ev[1]);
@@ -456,9 +466,16 @@ int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg
     // into x only when r is recomputed, so the rounding of x += alpha p
     // does not accumulate at the scale of |x| between recomputations.
     const bool group = o->replace_factor > 0.0 || o->true_residual_stop != 0;
+    // Two-level PCG keeps the rounding errors of its first increments (xL,
+    // Engine::kCompIters): its first directions carry the coarse solution,
+    // so plainly accumulated roundings (eps |x| per update, amplified by
+    // |A|) leave b - A x about 1e-9 of |b| away from the recursion at
+    // 256 x 256, p = 8, and every true residual then costs iterations;
+    // compensated, they stay near one rounding of x.
     const bool peeks = group && o->true_residual_stop != 0;
+    const bool comp = group && two_level;
     const int nvec = 3 + (group ? 1 : 0) + (pcg ? 1 : 0);
-    if ((rc = ensure_ws(op, (nvec + (peeks ? 1 : 0)) * vb))) return rc;
+    if ((rc = ensure_ws(op, (nvec + (peeks ? 1 : 0) + (comp ? 1 : 0)) * vb))) return rc;
     char *ws = (char *)op->ws;
     Engine E;
     E.opI = E.opO = op;
@@ -475,6 +492,7 @@ int solve_plain(sipg_op *op, void *x, const void *b, const sipg_cg_opts *o, sipg
     E.ncp = two_level ? coarse_num_partials(op->dimx, op->dimy) : 0;
     E.AxO = E.Ap;        // the initial and the replacement A x use the Ap slot
     E.xpk = peeks ? ws + (size_t)nvec * vb : nullptr;
+    E.xL = comp ? ws + (size_t)(nvec + (peeks ? 1 : 0)) * vb : nullptr;
     E.b = b;
     E.replace_factor = o->replace_factor;
     E.true_stop = o->true_residual_stop != 0;

@@ -243,7 +243,12 @@ int sipg_pcg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts
  * method is preconditioned_conjugate_gradient (conjugate_gradient_gpu.hpp:
  * 116-213); the coarse level has no counterpart in the reference, whose
  * preconditioner is block-diagonal only.  The iteration count no longer
- * grows with the mesh.  Options and results as sipg_pcg. */
+ * grows with the mesh.  Options and results as sipg_pcg.  With
+ * replace_factor > 0 the x increments of the first 16 iterations (the
+ * coarse-level ones, large against the final solution) are accumulated in
+ * compensated sums (a low-order part beside each entry, in one more
+ * workspace vector) and then folded into x, so that b - A x keeps following
+ * the recursive residual and a replacement every 100x drop is enough. */
 int sipg_pcg2(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts, sipg_cg_info *info);
 /* z = M2^-1 r (two-level, as sipg_pcg2), asynchronous on the op's stream. */
 int sipg_prec2_apply(sipg_op *op, const void *r, void *z);

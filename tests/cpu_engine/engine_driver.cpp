@@ -218,6 +218,7 @@ int main() {
     // peeks at the true residual end the solve
     run("ex_near_floor", 8, 4, 125.0, 1, 2e-14, 0, 0.1, 0);
     run("pcg_near_floor", 8, 4, 125.0, 3, 2e-14, 0, 0.1, 0);
+    run("pcg2_near_floor", 12, 3, 80.0, 5, 2e-14, 0, 0.1, 0);
     coarse_check(5, 5, 3);
     coarse_check(6, 4, 2);
     coarse_check(4, 5, 1);

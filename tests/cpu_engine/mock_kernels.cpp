@@ -34,6 +34,18 @@ int vec_num_blocks(int64_t) { return kMockVecBlocks; }
 template <typename T> static T *tp(void *p) { return (T *)p; }
 template <typename T> static const T *tp(const void *p) { return (const T *)p; }
 
+// x_h += a p with the rounding errors kept in x_l (TwoProd by fma, TwoSum),
+// as comp_axpy of the device kernels
+template <typename T> static void comp_axpy_h(T &xh, T &xl, T a, T pv) {
+    const T t = a * pv;
+    const T tp_ = std::fma(a, pv, -t);
+    const T s = xh + t;
+    const T bb = s - xh;
+    const T e = (xh - (s - bb)) + (t - bb);
+    xh = s;
+    xl = xl + (e + tp_);
+}
+
 static double dotv(int dtype, const void *a, const void *b, int64_t n) {
     double s = 0;
     for (int64_t i = 0; i < n; ++i)
@@ -145,7 +157,7 @@ int launch_cg_init_finalize(const double *prr, const double *pbb, int np, cg_sta
 
 template <typename T>
 static void update_xr_t(T *x, const T *p, T *r, const T *Ap, int64_t n, const double *ppAp, int np,
-                        double *prr, int nblk, cg_state *st, int slot) {
+                        double *prr, int nblk, cg_state *st, int slot, T *xlo) {
     if (st->s[slot].done | st->err) return;
     const double pAp = sum_partials(ppAp, np);
     const double rr = st->s[slot].rr;
@@ -154,7 +166,8 @@ static void update_xr_t(T *x, const T *p, T *r, const T *Ap, int64_t n, const do
     const T a = (T)alpha_d;
     double s = 0;
     for (int64_t i = 0; i < n; ++i) {
-        x[i] = x[i] + a * p[i];
+        if (xlo) comp_axpy_h(x[i], xlo[i], a, p[i]);
+        else x[i] = x[i] + a * p[i];
         const T ri = r[i] - a * Ap[i];
         r[i] = ri;
         s += (double)ri * (double)ri;
@@ -163,11 +176,11 @@ static void update_xr_t(T *x, const T *p, T *r, const T *Ap, int64_t n, const do
 }
 int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *Ap, int64_t n,
                         const double *part_pAp, int np_pAp, double *part_rr, int nblk, cg_state *st,
-                        int slot, hipStream_t) {
+                        int slot, hipStream_t, void *xlo) {
     if (dtype == SIPG_F64)
-        update_xr_t<double>(tp<double>(x), tp<double>(p), tp<double>(r), tp<double>(Ap), n, part_pAp, np_pAp, part_rr, nblk, st, slot);
+        update_xr_t<double>(tp<double>(x), tp<double>(p), tp<double>(r), tp<double>(Ap), n, part_pAp, np_pAp, part_rr, nblk, st, slot, tp<double>(xlo));
     else
-        update_xr_t<float>(tp<float>(x), tp<float>(p), tp<float>(r), tp<float>(Ap), n, part_pAp, np_pAp, part_rr, nblk, st, slot);
+        update_xr_t<float>(tp<float>(x), tp<float>(p), tp<float>(r), tp<float>(Ap), n, part_pAp, np_pAp, part_rr, nblk, st, slot, tp<float>(xlo));
     return SIPG_OK;
 }
 
@@ -238,14 +251,34 @@ int launch_true_residual(int in_dtype, int out_dtype, const void *b, const void 
     return SIPG_OK;
 }
 
-int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t) {
+int launch_accumulate(int x_dtype, int xl_dtype, void *x, void *xl, int64_t n, hipStream_t, void *xlo) {
     if (x_dtype == SIPG_F32 && xl_dtype == SIPG_F64) return SIPG_ERR_INVALID;
     for (int64_t i = 0; i < n; ++i) {
-        const double v = xl_dtype == SIPG_F64 ? tp<double>(xl)[i] : (double)tp<float>(xl)[i];
-        if (x_dtype == SIPG_F64) tp<double>(x)[i] += xl_dtype == SIPG_F64 ? v : (double)tp<float>(xl)[i];
-        else tp<float>(x)[i] += tp<float>(xl)[i];
-        if (xl_dtype == SIPG_F64) tp<double>(xl)[i] = 0;
-        else tp<float>(xl)[i] = 0;
+        if (xl_dtype == SIPG_F64) {
+            const double v = xlo ? tp<double>(xl)[i] + tp<double>(xlo)[i] : tp<double>(xl)[i];
+            tp<double>(x)[i] += v;
+            tp<double>(xl)[i] = 0;
+            if (xlo) tp<double>(xlo)[i] = 0;
+        } else {
+            const float v = xlo ? tp<float>(xl)[i] + tp<float>(xlo)[i] : tp<float>(xl)[i];
+            if (x_dtype == SIPG_F64) tp<double>(x)[i] += (double)v;
+            else tp<float>(x)[i] += v;
+            tp<float>(xl)[i] = 0;
+            if (xlo) tp<float>(xlo)[i] = 0;
+        }
+    }
+    return SIPG_OK;
+}
+
+int launch_comp_fold(int dtype, void *x, void *xlo, void *xi, int64_t n, hipStream_t) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (dtype == SIPG_F64) {
+            comp_axpy_h(tp<double>(x)[i], tp<double>(xlo)[i], 1.0, tp<double>(xi)[i]);
+            tp<double>(xi)[i] = 0;
+        } else {
+            comp_axpy_h(tp<float>(x)[i], tp<float>(xlo)[i], 1.0f, tp<float>(xi)[i]);
+            tp<float>(xi)[i] = 0;
+        }
     }
     return SIPG_OK;
 }
@@ -256,10 +289,12 @@ int &mock_peeks() {
     return n;
 }
 
-int launch_sum(int dtype, const void *x, const void *xl, void *out, int64_t n, hipStream_t) {
+int launch_sum(int dtype, const void *x, const void *xl, void *out, int64_t n, hipStream_t, const void *xlo) {
     for (int64_t i = 0; i < n; ++i) {
-        if (dtype == SIPG_F64) tp<double>(out)[i] = tp<double>(x)[i] + tp<double>(xl)[i];
-        else tp<float>(out)[i] = tp<float>(x)[i] + tp<float>(xl)[i];
+        if (dtype == SIPG_F64)
+            tp<double>(out)[i] = tp<double>(x)[i] + (xlo ? tp<double>(xl)[i] + tp<double>(xlo)[i] : tp<double>(xl)[i]);
+        else
+            tp<float>(out)[i] = tp<float>(x)[i] + (xlo ? tp<float>(xl)[i] + tp<float>(xlo)[i] : tp<float>(xl)[i]);
     }
     return SIPG_OK;
 }
@@ -449,7 +484,7 @@ int launch_pcg_rz(int dtype, int p, double, int mode, void *r, const void *Ap, v
 
 template <typename T>
 static void pcg_update_p_t(T *x, T *p, const T *z, int64_t n, const double *ppAp, int np_pAp,
-                           const double *prr, const double *prz, int np, cg_state *st, int slot) {
+                           const double *prr, const double *prz, int np, cg_state *st, int slot, T *xlo) {
     const cg_slot cur = st->s[slot];
     if (cur.done | st->err) { st->s[slot ^ 1] = cur; return; }
     const double rr_new = sum_partials(prr, np), rz_new = sum_partials(prz, np);
@@ -473,20 +508,23 @@ static void pcg_update_p_t(T *x, T *p, const T *z, int64_t n, const double *ppAp
     const T a = (T)(cur.rz / pAp), b = (T)beta_d;
     for (int64_t i = 0; i < n; ++i) {
         const T pi = p[i];
-        if (!replaced) x[i] = x[i] + a * pi;
+        if (!replaced) {
+            if (xlo) comp_axpy_h(x[i], xlo[i], a, pi);
+            else x[i] = x[i] + a * pi;
+        }
         p[i] = z[i] + b * pi;
     }
 }
 
 int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, const double *part_pAp,
                         int np_pAp, const double *part_rr, const double *part_rz, int np, int, cg_state *st,
-                        int slot, hipStream_t) {
+                        int slot, hipStream_t, void *xlo) {
     if (dtype == SIPG_F64)
         pcg_update_p_t<double>(tp<double>(x), tp<double>(p), tp<double>(z), n, part_pAp, np_pAp, part_rr, part_rz,
-                               np, st, slot);
+                               np, st, slot, tp<double>(xlo));
     else
         pcg_update_p_t<float>(tp<float>(x), tp<float>(p), tp<float>(z), n, part_pAp, np_pAp, part_rr, part_rz, np,
-                              st, slot);
+                              st, slot, tp<float>(xlo));
     return SIPG_OK;
 }
 
@@ -565,7 +603,7 @@ int launch_coarse_add(int dtype, int p, void *v, int dimx, int64_t n, const coar
 
 int launch_pcg2_update_p(int dtype, int p, void *x, void *pv, const void *z, int64_t n, const double *part_pAp,
                          int np_pAp, const double *part_rr, const double *part_rz, int np, int npc, int dimx,
-                         const coarse_tables &c, int, cg_state *st, int slot, hipStream_t) {
+                         const coarse_tables &c, int, cg_state *st, int slot, hipStream_t, void *xlo) {
     if (dtype != SIPG_F64) return SIPG_ERR_UNSUPPORTED;
     // z + P0^T C through a scratch copy, then the plain update with r.z + S.C
     std::vector<double> zc(tp<const double>(z), tp<const double>(z) + n);
@@ -573,17 +611,22 @@ int launch_pcg2_update_p(int dtype, int p, void *x, void *pv, const void *z, int
     std::vector<double> prz(part_rz, part_rz + np);
     if (np > 0) prz[np - 1] += sum_partials(c.part, npc);
     pcg_update_p_t<double>(tp<double>(x), tp<double>(pv), zc.data(), n, part_pAp, np_pAp, part_rr, prz.data(), np,
-                           st, slot);
+                           st, slot, tp<double>(xlo));
     return SIPG_OK;
 }
 
 int launch_pcg_x(int dtype, void *x, const void *p, int64_t n, const double *part_pAp, int np_pAp, int,
-                 const cg_state *st, int slot, hipStream_t) {
+                 const cg_state *st, int slot, hipStream_t, void *xlo) {
     if (st->s[slot].done | st->err) return SIPG_OK;
     const double a = st->s[slot].rz / sum_partials(part_pAp, np_pAp);
     for (int64_t i = 0; i < n; ++i) {
-        if (dtype == SIPG_F64) tp<double>(x)[i] += (double)a * tp<double>(p)[i];
-        else tp<float>(x)[i] += (float)a * tp<float>(p)[i];
+        if (dtype == SIPG_F64) {
+            if (xlo) comp_axpy_h(tp<double>(x)[i], tp<double>(xlo)[i], a, tp<double>(p)[i]);
+            else tp<double>(x)[i] += (double)a * tp<double>(p)[i];
+        } else {
+            if (xlo) comp_axpy_h(tp<float>(x)[i], tp<float>(xlo)[i], (float)a, tp<float>(p)[i]);
+            else tp<float>(x)[i] += (float)a * tp<float>(p)[i];
+        }
     }
     return SIPG_OK;
 }

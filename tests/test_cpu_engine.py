@@ -104,7 +104,7 @@ def test_preconditioned_solvers_stop_at_the_fp64_floor(results, case, n):
     assert r["relres"] < 1e-13, r
 
 
-@pytest.mark.parametrize("case", ["ex_near_floor", "pcg_near_floor"])
+@pytest.mark.parametrize("case", ["ex_near_floor", "pcg_near_floor", "pcg2_near_floor"])
 def test_near_the_floor_a_peek_can_finish_the_solve(results, case):
     # tolerance 2e-14, just above this problem's FP64 floor (~1.3e-14):
     # the finishing checks need the recursion to fall 4x below the last

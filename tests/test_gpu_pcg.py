@@ -327,14 +327,17 @@ def test_pcg2_c2_reaches_1e10_in_a_fraction_of_the_iterations(sipg):
     modes = sipg.forcing_draw(0, 4, 4, 4)
     op.rhs_fourier(modes, b)
     x.zero()
-    # a true residual at every 10x drop keeps the recursion within reach of
-    # b - A x (the coarse solve's large early corrections leave a gap near
-    # 1e-9 relative otherwise, and rarer replacements then perturb r by a
-    # sizeable fraction of itself)
-    st, info = op.pcg2(x, b, 1e-10, sipg.TOL_REL, maxit=20000, replace_factor=0.1, true_stop=True)
+    # the first increments carry the coarse solution; summed with their
+    # rounding errors kept, b - A x follows the recursion to ~5e-11, and one
+    # true residual after a 100x drop suffices: 61 iterations (a true
+    # residual at every 10x drop: 93)
+    st, info = op.pcg2(x, b, 1e-10, sipg.TOL_REL, maxit=20000, replace_factor=1e-2, true_stop=True)
     assert st == sipg.OK, (st, info.iters)
     assert op.true_relres(x, b) <= 1e-10
-    assert info.iters < 150, info.iters
+    assert info.iters < 80, info.iters
+    x.zero()
+    st, info = op.pcg2(x, b, 1e-10, sipg.TOL_REL, maxit=20000, replace_factor=0.1, true_stop=True)
+    assert st == sipg.OK and op.true_relres(x, b) <= 1e-10 and info.iters < 150, (st, info.iters)
     err = op.err_norms_fourier(modes, x)
     assert err["L2"] < 1e-10, err
 
