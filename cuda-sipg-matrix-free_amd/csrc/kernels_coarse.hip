@@ -142,7 +142,10 @@ template <int SIDE, int MODE>
 __global__ __launch_bounds__(GEMM_T) void dst_gemm_kernel(const double *__restrict__ X,
                                                           const double *__restrict__ Q, int M, int n, int N,
                                                           double *__restrict__ W, double *__restrict__ out,
-                                                          double *__restrict__ part) {
+                                                          double *__restrict__ part, const cg_state *st, int slot) {
+    // inside a solve: nothing to do once it has finished (the launches
+    // queued past the finish run as no-ops)
+    if (st && (st->s[slot].done | st->err)) return;
     // SIDE 0: A = Q rows (plain), B0/B1 = folded X (+/-); SIDE 1: A0/A1 =
     // folded X rows (+/-), B = Q columns
     __shared__ double Aq[SIDE == 1 ? 2 : 1][GT][KC2 + 1];
@@ -536,7 +539,7 @@ int coarse_num_partials(int dimx, int dimy) {
 }
 
 int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t s,
-                        bool have_e4) {
+                        bool have_e4, const cg_state *st, int slot) {
     const int nvx = dimx - 1, nvy = dimy - 1;
     const int64_t noe = (int64_t)dimx * dimy;
     if (nvx < 1 || nvy < 1) return SIPG_OK;          // no interior vertex: C = 0, no partials
@@ -552,10 +555,13 @@ int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, con
     // T1 = S Qx (S gathered from E4 on the fly, and stored); T2 = (Qy T1) ./
     // Lambda; T1 = T2 Qx; C = Qy T1 (into Cpad), partials of S.C
     const dim3 g = gemm_grid(nvy, nvx), b(GEMM_T);
-    hipLaunchKernelGGL((dst_gemm_kernel<1, 3>), g, b, 0, s, c.E4, c.Qx, nvy, nvx, nvx, c.S, c.T1, nullptr);
-    hipLaunchKernelGGL((dst_gemm_kernel<0, 1>), g, b, 0, s, c.T1, c.Qy, nvy, nvy, nvx, c.Linv, c.T2, nullptr);
-    hipLaunchKernelGGL((dst_gemm_kernel<1, 0>), g, b, 0, s, c.T2, c.Qx, nvy, nvx, nvx, nullptr, c.T1, nullptr);
-    hipLaunchKernelGGL((dst_gemm_kernel<0, 2>), g, b, 0, s, c.T1, c.Qy, nvy, nvy, nvx, c.S, c.Cpad, c.part);
+    hipLaunchKernelGGL((dst_gemm_kernel<1, 3>), g, b, 0, s, c.E4, c.Qx, nvy, nvx, nvx, c.S, c.T1, nullptr, st, slot);
+    hipLaunchKernelGGL((dst_gemm_kernel<0, 1>), g, b, 0, s, c.T1, c.Qy, nvy, nvy, nvx, c.Linv, c.T2, nullptr, st,
+                       slot);
+    hipLaunchKernelGGL((dst_gemm_kernel<1, 0>), g, b, 0, s, c.T2, c.Qx, nvy, nvx, nvx, nullptr, c.T1, nullptr, st,
+                       slot);
+    hipLaunchKernelGGL((dst_gemm_kernel<0, 2>), g, b, 0, s, c.T1, c.Qy, nvy, nvy, nvx, c.S, c.Cpad, c.part, st,
+                       slot);
     SIPG_HIP_TRY(hipGetLastError());
     return SIPG_OK;
 }

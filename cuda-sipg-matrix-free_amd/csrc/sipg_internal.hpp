@@ -268,9 +268,10 @@ struct coarse_tables {
 int coarse_num_partials(int dimx, int dimy);
 // C = A0^-1 P0 r into Cpad, with the S.C partials (nothing when a side has
 // one element: no interior vertex); have_e4: the corner sums are already in
-// c.E4 (launch_pcg_rz wrote them)
+// c.E4 (launch_pcg_rz wrote them); st/slot: inside a solve, the product
+// kernels return at once when that slot is done
 int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t s,
-                        bool have_e4 = false);
+                        bool have_e4 = false, const cg_state *st = nullptr, int slot = 0);
 // v += P0^T C
 int launch_coarse_add(int dtype, int p, void *v, int dimx, int64_t n, const coarse_tables &c, int nblk,
                       hipStream_t s);

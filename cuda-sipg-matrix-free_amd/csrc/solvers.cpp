@@ -133,7 +133,8 @@ struct Engine {
                                 slot, s, replace ? nullptr : e4())))
             return rc;
         if (coarse && !replace &&
-            (rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s, e4() != nullptr)))
+            (rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s, e4() != nullptr, st,
+                                      slot)))
             return rc;
         int nrz = npc(1);
         if (replace) {
@@ -149,7 +150,7 @@ struct Engine {
                                     P.rz, nullptr, 0, s, e4())))
                 return rc;
             if (coarse && (rc = launch_coarse_solve(opI->dtype, opI->p, r, opI->dimx, opI->dimy, *coarse, s,
-                                                    e4() != nullptr)))
+                                                    e4() != nullptr, st, slot)))
                 return rc;
             nrz = npc(0);
             if ((rc = launch_mark_replaced(st, slot, restart_p ? 1 : 0, s))) return rc;
@@ -259,6 +260,15 @@ struct Engine {
         // it was.
         constexpr int kPeekEvery = 32;
         int last_peek_it = 0;
+        // Tail: the launches queued past the finish run as no-ops, but each
+        // still costs its launch and its share of the final synchronisation
+        // (a chunk of 32 queued two polls late left 39 of them after a
+        // 61-iteration two-level solve).  The finish is predicted from the
+        // average fall of the polled r.r per iteration since the start, and
+        // a chunk goes at most kTailIters past it (never below kMinChunk,
+        // which keeps the device ahead of the host's polls).
+        constexpr int kTailIters = 2, kMinChunk = 4;
+        const double rr_start = cur.s[0].rr;
         // Each pass: decide on the newest polled state `cur`, enqueue one
         // chunk of iterations (4 growing to 32; 8 near the tolerance) and a
         // state copy, and block on the older of two outstanding copies.  Iterations past the finish are no-ops on
@@ -323,7 +333,16 @@ struct Engine {
             }
             // near the tolerance, short chunks keep the host's lag small
             const bool near = true_stop && cs.rr <= 1e4 * cur.stop * cur.stop;
-            const int n_it = near ? std::min(chunk, 8) : chunk;
+            int n_it = near ? std::min(chunk, 8) : chunk;
+            const double goal = cur.stop * cur.stop;
+            if (cs.it > 0 && rr_start > 0.0 && cs.rr < rr_start) {
+                const int64_t queued = launched - cur_launched;      // launched past the polled state
+                double left = 0.0;                                  // predicted iterations to the finish
+                if (cs.rr > goal) left = std::ceil(std::log(cs.rr / goal) * cs.it / std::log(rr_start / cs.rr));
+                const int64_t cap =
+                    std::max<int64_t>(kMinChunk, (int64_t)std::min(left, 1e9) + kTailIters - queued);
+                n_it = (int)std::min<int64_t>(n_it, cap);
+            }
             for (int c = 0; c < n_it; ++c)
                 if ((rc = iteration(want_rep && c == 0, restart_p))) break;
             if (rc) break;

@@ -538,8 +538,9 @@ static double corner_weight(int c, double xa, double yb) {
 }
 
 int launch_coarse_solve(int dtype, int p, const void *r, int dimx, int dimy, const coarse_tables &c, hipStream_t,
-                        bool) {
+                        bool, const cg_state *st, int slot) {
     if (dtype != SIPG_F64) return SIPG_ERR_UNSUPPORTED;
+    if (st && (st->s[slot].done | st->err)) return SIPG_OK;
     const int nvx = dimx - 1, nvy = dimy - 1, n1 = p + 1, nq = n1 * n1;
     if (nvx < 1 || nvy < 1) return SIPG_OK;
     const double *R = tp<const double>(r);
