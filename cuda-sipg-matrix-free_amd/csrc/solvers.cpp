@@ -267,7 +267,7 @@ struct Engine {
         // average fall of the polled r.r per iteration since the start, and
         // a chunk goes at most kTailIters past it (never below kMinChunk,
         // which keeps the device ahead of the host's polls).
-        constexpr int kTailIters = 2, kMinChunk = 4;
+        constexpr int kTailIters = 2, kMinChunk = 2;
         const double rr_start = cur.s[0].rr;
         // Each pass: decide on the newest polled state `cur`, enqueue one
         // chunk of iterations (4 growing to 32; 8 near the tolerance) and a
