@@ -265,9 +265,11 @@ struct Engine {
         // (a chunk of 32 queued two polls late left 39 of them after a
         // 61-iteration two-level solve).  The finish is predicted from the
         // average fall of the polled r.r per iteration since the start, and
-        // a chunk goes at most kTailIters past it (never below kMinChunk,
-        // which keeps the device ahead of the host's polls).
-        constexpr int kTailIters = 2, kMinChunk = 2;
+        // a chunk ends short of it (kAheadFrac of the predicted distance;
+        // never below kMinChunk, which keeps the device ahead of the host's
+        // polls), so that a poll falls just before the finish.
+        constexpr int kMinChunk = 2;
+        constexpr double kAheadFrac = 0.85;
         const double rr_start = cur.s[0].rr;
         // Each pass: decide on the newest polled state `cur`, enqueue one
         // chunk of iterations (4 growing to 32; 8 near the tolerance) and a
@@ -335,13 +337,31 @@ struct Engine {
             const bool near = true_stop && cs.rr <= 1e4 * cur.stop * cur.stop;
             int n_it = near ? std::min(chunk, 8) : chunk;
             const double goal = cur.stop * cur.stop;
+            // drain: this pass waits for the newest state copy (the device
+            // idles for one host round trip), not the older one
+            bool drain = false;
             if (cs.it > 0 && rr_start > 0.0 && cs.rr < rr_start) {
                 const int64_t queued = launched - cur_launched;      // launched past the polled state
                 double left = 0.0;                                  // predicted iterations to the finish
                 if (cs.rr > goal) left = std::ceil(std::log(cs.rr / goal) * cs.it / std::log(rr_start / cs.rr));
-                const int64_t cap =
-                    std::max<int64_t>(kMinChunk, (int64_t)std::min(left, 1e9) + kTailIters - queued);
+                left = std::min(left, 1e9);
+                const int64_t cap = std::max<int64_t>(kMinChunk, (int64_t)(kAheadFrac * left) - queued);
                 n_it = (int)std::min<int64_t>(n_it, cap);
+                // the recursion is predicted to meet the tolerance within
+                // what is queued and this chunk: the chunk ends there and
+                // its state is waited for, so that the confirming true
+                // residual follows the recursion's flag at once instead of
+                // two chunks later
+                if (true_stop && floor_watch && cs.rr > goal && !(cs.flags & kSlotWantCheck) && !check_pending &&
+                    left <= (double)(queued + n_it)) {
+                    drain = true;
+                    n_it = (int)std::max<int64_t>(1, (int64_t)left - queued);
+                }
+            }
+            // the final check alone, its outcome waited for
+            if (want_rep && true_stop && floor_watch && (cs.flags & kSlotWantCheck)) {
+                drain = true;
+                n_it = 1;
             }
             for (int c = 0; c < n_it; ++c)
                 if ((rc = iteration(want_rep && c == 0, restart_p))) break;
@@ -364,6 +384,13 @@ struct Engine {
                 SIPG_HIP_TRY(hipEventSynchronize(ev[buf]));
                 cur = hst[buf];
                 cur_launched = pending_launched[buf];
+                --inflight;
+            }
+            if (drain && inflight == 1) {
+                // and the newest in the other
+                SIPG_HIP_TRY(hipEventSynchronize(ev[buf ^ 1]));
+                cur = hst[buf ^ 1];
+                cur_launched = pending_launched[buf ^ 1];
                 --inflight;
             }
         }

@@ -214,10 +214,10 @@ int main() {
     run("ex_floor", 8, 4, 125.0, 1, 1e-17, 0, 0.1, 0);
     run("pcg_floor", 8, 4, 125.0, 3, 1e-17, 0, 0.1, 0);
     run("pcg2_floor", 12, 3, 80.0, 5, 1e-17, 0, 0.1, 0);
-    // tolerances a little above the floor: between the finishing checks,
-    // peeks at the true residual end the solve (CG: 1.3e-14, its floor
-    // about 1.26e-14)
-    run("ex_near_floor", 8, 4, 125.0, 1, 1.3e-14, 0, 0.1, 0);
+    // tolerances a little above the floor (about 1.26e-14): between the
+    // finishing checks, peeks at the true residual end the solve
+    run("ex_near_floor", 8, 4, 125.0, 1, 2e-14, 0, 0.1, 0);
+    run("ex_nearer_floor", 8, 4, 125.0, 1, 1.3e-14, 0, 0.1, 0);
     run("pcg_near_floor", 8, 4, 125.0, 3, 2e-14, 0, 0.1, 0);
     run("pcg2_near_floor", 12, 3, 80.0, 5, 2e-14, 0, 0.1, 0);
     coarse_check(5, 5, 3);
