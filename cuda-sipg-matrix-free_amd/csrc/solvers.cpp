@@ -41,8 +41,10 @@ struct Engine {
     // The first kCompIters iterations update xI with compensation (their
     // increments carry the coarse solution, a large part of x); then xI is
     // folded into xO with its rounding error kept in xL, and the later,
-    // much smaller increments accumulate plainly in a fresh xI.
-    static constexpr int kCompIters = 16;
+    // much smaller increments accumulate plainly in a fresh xI.  (At
+    // 256 x 256, p = 8, 4, 8 and 16 such iterations give the same iteration
+    // counts and true residuals; each costs two more vector streams.)
+    static constexpr int kCompIters = 4;
     bool mixed = false;              // fold xI into xO at each true residual
     int restart = 0;                 // p = r at a true residual: 0 never, 1 always,
                                      // 2 once the true residual stagnates
@@ -250,6 +252,8 @@ struct Engine {
         double ref_at_rep = 0.0;           // the true r.r before the pending one
         int lag_count = 0, lag_it = 0, rep_it = 0;
         double rec_min = -1.0;             // lowest recursive r.r since the last true residual
+        double rec_mark = -1.0;            // rec_min at its last 4x improvement, and when
+        int rec_mark_it = 0;
         // Peeks: once the recursion has met the tolerance, the finishing
         // check waits for it to fall 4x below the last true residual, which
         // near the FP64 floor takes hundreds of iterations per check while
@@ -306,6 +310,17 @@ struct Engine {
             if (floor_watch && pcg && seen_rep > 0 && !check_pending) {
                 rec_min = rec_min < 0.0 ? cs.rr : std::min(rec_min, cs.rr);
                 if (floor_in_reach(cur.rr_ref, cur.bb) && rec_min > 0.25 * cur.rr_ref &&
+                    cs.it - rep_it >= stall_window(cs.it))
+                    break;
+                // nor one that fell that far but has hovered since: its
+                // lowest r.r not 4x lower for stall_window iterations, short
+                // of the next replacement's drop, so no true residual comes
+                // for the lag rule to judge
+                if (rec_mark < 0.0 || rec_min <= 0.25 * rec_mark) {
+                    rec_mark = rec_min;
+                    rec_mark_it = cs.it;
+                }
+                if (floor_in_reach(cur.rr_ref, cur.bb) && cs.it - rec_mark_it >= stall_window(cs.it) &&
                     cs.it - rep_it >= stall_window(cs.it))
                     break;
             }
@@ -372,6 +387,7 @@ struct Engine {
                 ref_at_rep = cur.rr_ref;
                 rep_it = cs.it;
                 rec_min = -1.0;
+                rec_mark = -1.0;
             }
             if (chunk < 32) chunk *= 2;
             SIPG_HIP_TRY(hipMemcpyAsync(&hst[buf], st, sizeof(cg_state), hipMemcpyDeviceToHost, s));

@@ -244,7 +244,7 @@ int sipg_pcg_mixed(sipg_op *op, void *x, const void *b, const sipg_cg_opts *opts
  * 116-213); the coarse level has no counterpart in the reference, whose
  * preconditioner is block-diagonal only.  The iteration count no longer
  * grows with the mesh.  Options and results as sipg_pcg.  With
- * replace_factor > 0 the x increments of the first 16 iterations (the
+ * replace_factor > 0 the x increments of the first 4 iterations (the
  * coarse-level ones, large against the final solution) are accumulated in
  * compensated sums (a low-order part beside each entry, in one more
  * workspace vector) and then folded into x, so that b - A x keeps following
