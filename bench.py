@@ -68,8 +68,8 @@ def gpu_solve(sipg, op, b, x, method, rf2=1e-2):
     elif method == "pcg2":
         # the solver keeps the rounding errors of its first (coarse-level)
         # increments, so b - A x follows the recursion: one true residual
-        # after a 100x drop is enough (61 iterations here; a replacement at
-        # every 10x drop perturbs the recursion: 93)
+        # after a 100x drop is enough (60 iterations at 256x256, p=8; a
+        # replacement at every 10x drop perturbs the recursion: 62)
         st, info = op.pcg2(x, b, TOL, sipg.TOL_REL, maxit=MAXIT_FP64, replace_factor=rf2,
                            true_stop=True, restart=0)
     else:
@@ -216,7 +216,7 @@ def c3_leg(sipg, steps):
         x = op.vector()
         # p=15: the FP64 floor lies closer to 1e-10 than at p=8, and a true
         # residual at every 10x drop keeps the two-level solve at 125
-        # iterations (157 with one every 100x)
+        # iterations (136 with one every 100x)
         gpu_solve(sipg, op, b, x, method, rf2=0.1)             # warm-up (tables, FP32 twin)
         ts = []
         for _ in range(max(steps, 3)):
@@ -271,8 +271,8 @@ def c1_leg(sipg):
     u = g["u"]
     for method in ("fp64", "pcg2", "mixed"):
         x = op.vector()
-        t, info = gpu_solve(sipg, op, b, x, method, rf2=0.1)
-        t, info = gpu_solve(sipg, op, b, x, method, rf2=0.1)
+        t, info = gpu_solve(sipg, op, b, x, method)
+        t, info = gpu_solve(sipg, op, b, x, method)
         xh = op.download(x)
         out[method] = {"ms_per_solve": 1e3 * t, "iterations": info.iters, "true_relres": op.true_relres(x, b),
                        "rel_l2_vs_lapack": float(np.linalg.norm(xh - u) / np.linalg.norm(u)),

@@ -329,8 +329,8 @@ def test_pcg2_c2_reaches_1e10_in_a_fraction_of_the_iterations(sipg):
     x.zero()
     # the first increments carry the coarse solution; summed with their
     # rounding errors kept, b - A x follows the recursion to ~5e-11, and one
-    # true residual after a 100x drop suffices: 61 iterations (a true
-    # residual at every 10x drop: 93)
+    # true residual after a 100x drop suffices: 60 iterations (a true
+    # residual at every 10x drop: 62)
     st, info = op.pcg2(x, b, 1e-10, sipg.TOL_REL, maxit=20000, replace_factor=1e-2, true_stop=True)
     assert st == sipg.OK, (st, info.iters)
     assert op.true_relres(x, b) <= 1e-10
