@@ -12,8 +12,10 @@ extern "C" {
 }
 
 namespace sipg_internal {
-int &mock_peeks();   // mock_kernels.cpp: solves finished by a peek
+int &mock_peeks();       // mock_kernels.cpp: solves finished by a peek
+int &mock_p_updates();   // mock_kernels.cpp: iterations enqueued
 }
+using sipg_internal::mock_p_updates;
 using sipg_internal::mock_peeks;
 
 namespace sipg_internal {
@@ -115,6 +117,7 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
     sipg_cg_opts o{tol, SIPG_TOL_REL, maxit, rf, true_stop, restart};
     sipg_cg_info info{};
     mock_peeks() = 0;
+    mock_p_updates() = 0;
     if (mode == 0) {
         int it = 0;
         double res = 0;
@@ -132,6 +135,7 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
     } else {
         rc = sipg_pcg_mixed(op, x, b, &o, &info);
     }
+    const int launched = mock_p_updates();
     sipg_download(op, x, xh.data());
     std::vector<double> xo(n, 0.0);
     double rro = 0;
@@ -139,9 +143,9 @@ static void run(const char *name, int d, int p, double pen, int mode, double tol
     double dx = 0, nx = 0;
     for (int64_t i = 0; i < n; ++i) { dx += (xh[i] - xo[i]) * (xh[i] - xo[i]); nx += xo[i] * xo[i]; }
     printf("{\"case\":\"%s\",\"rc\":%d,\"iters\":%d,\"reps\":%d,\"oracle_iters\":%d,"
-           "\"relres\":%.6e,\"diff\":%.6e,\"peeks\":%d,\"true_res\":%.6e,\"b_norm\":%.6e}\n",
+           "\"relres\":%.6e,\"diff\":%.6e,\"peeks\":%d,\"true_res\":%.6e,\"b_norm\":%.6e,\"launched\":%d}\n",
            name, rc, info.iters, info.replacements, ito, relres(d, p, pen, xh, bh), std::sqrt(dx / nx), mock_peeks(),
-           info.true_res_norm, info.b_norm);
+           info.true_res_norm, info.b_norm, launched);
     sipg_free(x);
     sipg_free(b);
     sipg_op_destroy(op);

@@ -174,6 +174,8 @@ static void update_xr_t(T *x, const T *p, T *r, const T *Ap, int64_t n, const do
     }
     put_partials(prr, nblk, s);
 }
+int &mock_p_updates();
+
 int launch_cg_update_xr(int dtype, void *x, const void *p, void *r, const void *Ap, int64_t n,
                         const double *part_pAp, int np_pAp, double *part_rr, int nblk, cg_state *st,
                         int slot, hipStream_t, void *xlo) {
@@ -209,6 +211,7 @@ static void update_p_t(const T *r, T *p, int64_t n, const double *prr, int np, c
 }
 int launch_cg_update_p(int dtype, const void *r, void *p, int64_t n, const double *part_rr, int np_rr,
                        int, cg_state *st, int slot, hipStream_t) {
+    ++mock_p_updates();
     if (dtype == SIPG_F64) update_p_t<double>(tp<double>(r), tp<double>(p), n, part_rr, np_rr, st, slot);
     else update_p_t<float>(tp<float>(r), tp<float>(p), n, part_rr, np_rr, st, slot);
     return SIPG_OK;
@@ -285,6 +288,13 @@ int launch_comp_fold(int dtype, void *x, void *xlo, void *xi, int64_t n, hipStre
 
 // solves the peek finished (read and reset by the engine driver)
 int &mock_peeks() {
+    static int n = 0;
+    return n;
+}
+
+// p-update launches, one per iteration the engine enqueues, including the
+// no-op ones past a solve's finish (read and reset by the engine driver)
+int &mock_p_updates() {
     static int n = 0;
     return n;
 }
@@ -519,6 +529,7 @@ static void pcg_update_p_t(T *x, T *p, const T *z, int64_t n, const double *ppAp
 int launch_pcg_update_p(int dtype, void *x, void *p, const void *z, int64_t n, const double *part_pAp,
                         int np_pAp, const double *part_rr, const double *part_rz, int np, int, cg_state *st,
                         int slot, hipStream_t, void *xlo) {
+    ++mock_p_updates();
     if (dtype == SIPG_F64)
         pcg_update_p_t<double>(tp<double>(x), tp<double>(p), tp<double>(z), n, part_pAp, np_pAp, part_rr, part_rz,
                                np, st, slot, tp<double>(xlo));
@@ -605,6 +616,7 @@ int launch_coarse_add(int dtype, int p, void *v, int dimx, int64_t n, const coar
 int launch_pcg2_update_p(int dtype, int p, void *x, void *pv, const void *z, int64_t n, const double *part_pAp,
                          int np_pAp, const double *part_rr, const double *part_rz, int np, int npc, int dimx,
                          const coarse_tables &c, int, cg_state *st, int slot, hipStream_t, void *xlo) {
+    ++mock_p_updates();
     if (dtype != SIPG_F64) return SIPG_ERR_UNSUPPORTED;
     // z + P0^T C through a scratch copy, then the plain update with r.z + S.C
     std::vector<double> zc(tp<const double>(z), tp<const double>(z) + n);
